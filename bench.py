@@ -1,0 +1,247 @@
+"""Benchmark of the HIP tile voxelizer on one MI355X.
+
+One step = one full voxelizer call and its backward at the voxelizer
+microbenchmark workload: TileVoxelizer.forward (fp32 prologue, GPU binning,
+forward tile kernel) on a 256x256x128 volume with 1M Gaussians (means uniform
+in the volume, log-normal voxel sigma with median 1.5 and sigma 0.5, random
+rotations, N(0,1)+iN(0,1) amplitudes, 8^3 tiles), then autograd of
+<G, vol> with a fixed random upstream gradient G down to positions,
+log-scales, quaternions and complex amplitudes.  Inputs are resident in HBM
+before the timed region.
+
+Prints one JSON line (rank 0).  The roofline entry is the dominant tile
+kernel's algorithmic FP32 rate (the reference kernel's FLOP count per
+evaluation, times the exact evaluation counts of this input) over its average
+duration, timed with HIP events recorded by the library on the launch stream
+during the timed steps.  The CPU baseline is a PyTorch CPU restatement of the
+reference's per-tile voxelizer, timed on a bounded sample of tiles and scaled
+by the sample's share of the evaluations.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "voxelizer fwd+bwd ms (1M Gauss,256²×128)"
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector, published (packed FP32)
+SHAPE = (256, 256, 128)
+N_GAUSS = 1_000_000
+TILE = 8
+
+# Algorithmic FLOPs per evaluation, counted from the reference kernel
+# (FMA = 2, mul/add/sub/div = 1, exp = 1): forward 26 ungated + 6 gated,
+# backward 26 ungated + 50 gated.
+FWD_UNGATED, FWD_GATED = 26, 6
+BWD_UNGATED, BWD_GATED = 26, 50
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_inputs(dev, seed=0):
+    import numpy as np
+    import torch
+
+    from gs_tmi_amd.synthetic import make_gaussians
+
+    pos, scales, rot, rr, ri = make_gaussians(N_GAUSS, SHAPE, seed, sigma_median=1.5, sigma_log_sd=0.5)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    params = dict(pos=t(pos), log_s=t(np.log(scales)), rot=t(rot), dr=t(rr), di=t(ri))
+    for v in params.values():
+        v.requires_grad_()
+    g = torch.Generator(device="cpu").manual_seed(seed + 1)
+    grad_r = torch.randn(SHAPE, generator=g).to(dev)
+    grad_i = torch.randn(SHAPE, generator=g).to(dev)
+    return params, grad_r, grad_i
+
+
+def cpu_baseline(params, budget_s=15.0):
+    """Times the PyTorch CPU restatement of the reference's per-tile forward +
+    backward on a sample of tiles; returns (extrapolated ms, sample note, cores)."""
+    import numpy as np
+    import torch
+
+    from oracle import cpu_tile_torch as cpu
+    from oracle import voxelizer_oracle as vo
+
+    cores = os.cpu_count() or 1
+    torch.set_num_threads(cores)
+    pos = params["pos"].detach().cpu().numpy()
+    sc = np.exp(params["log_s"].detach().cpu().numpy())
+    rot = params["rot"].detach().cpu().numpy()
+    c32, r32, _, hs = vo.prologue(pos, sc, rot, SHAPE)
+    cov = vo.fp32_cov_inv(sc, rot)
+    ranges, pids = vo.bin_tiles(c32, r32, SHAPE, TILE)
+    pairs = (ranges[:, 1] - ranges[:, 0]).astype(np.int64)
+    e_per_tile = pairs * TILE ** 3  # all tiles are full at this shape
+    tc = torch.from_numpy(c32)
+    tm = torch.from_numpy(cov)
+    trr = params["dr"].detach().cpu()
+    tri = params["di"].detach().cpu()
+    ths = torch.from_numpy(hs)
+    tr = torch.from_numpy(ranges)
+    tp = torch.from_numpy(pids)
+    g = torch.complex(torch.randn(SHAPE), torch.randn(SHAPE))
+    rng = np.random.default_rng(0)
+    order = rng.permutation(ranges.shape[0])
+    # Pilot on 4 tiles to size the sample.
+    t0 = time.perf_counter()
+    cpu.tile_forward_backward(tc, tm, trr, tri, ths, tr, tp, g, SHAPE, TILE, order[:4])
+    pilot = max(time.perf_counter() - t0, 1e-3)
+    ntiles = int(max(8, min(ranges.shape[0], 4 * budget_s / pilot)))
+    sample = order[4:4 + ntiles]
+    t0 = time.perf_counter()
+    cpu.tile_forward_backward(tc, tm, trr, tri, ths, tr, tp, g, SHAPE, TILE, sample)
+    dt = time.perf_counter() - t0
+    share = e_per_tile[sample].sum() / e_per_tile.sum()
+    ms = dt / share * 1e3
+    note = (f"per-tile fp32 torch forward+autograd backward on {len(sample)} random tiles of {ranges.shape[0]} "
+            f"({share * 100:.2f}% of evaluations, {dt:.1f} s), extrapolated by evaluation share; "
+            f"{cores} threads; CPU {cpu_model()}")
+    return ms, note, cores
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the multi-GPU design is "
+              "described, not run", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from gs_tmi_amd import TileVoxelizer, _native, build_tile_pairs, voxel_space_params
+
+    dev = torch.device("cuda:0")
+    log("[bench] building inputs")
+    params, grad_r, grad_i = make_inputs(dev)
+    vox = TileVoxelizer(SHAPE, tile_size=TILE, device=str(dev))
+
+    def step():
+        for p in params.values():
+            p.grad = None
+        vr, vi = vox(params["pos"], torch.exp(params["log_s"]), params["rot"],
+                     torch.complex(params["dr"], params["di"]))
+        loss = (vr * grad_r).sum() + (vi * grad_i).sum()
+        loss.backward()
+
+    log(f"[bench] warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # Exact evaluation counts of this input (outside the timed region).
+    with torch.no_grad():
+        center, radius, cov, hs = voxel_space_params(params["pos"], torch.exp(params["log_s"]), params["rot"],
+                                                     SHAPE)
+        ranges, pids = build_tile_pairs(center, radius, SHAPE, TILE)
+        e_total, e_active, e_culled = _native.count_evaluations(
+            center, cov.contiguous(), params["dr"].detach(), params["di"].detach(), ranges, pids, hs, TILE, *SHAPE)
+        pairs = int(pids.numel())
+        # Binning alone, for the breakdown.
+        bt = []
+        for _ in range(5):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            build_tile_pairs(center, radius, SHAPE, TILE)
+            e1.record()
+            torch.cuda.synchronize()
+            bt.append(e0.elapsed_time(e1))
+    log(f"[bench] pairs {pairs}, E_total {e_total:.4e}, E_active {e_active:.4e}, E_culled {e_culled:.4e}")
+
+    log(f"[bench] timing {args.steps} steps")
+    per_step = []
+    torch.cuda.synchronize()
+    _native.profile_begin()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        per_step.append((time.perf_counter() - t0) * 1e3)
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t_start
+    prof = _native.profile_end()
+    ms_per_step = t_total * 1e3 / args.steps
+
+    fwd_ms = prof["forward"][0] / max(prof["forward"][1], 1)
+    bwd_ms = prof["backward"][0] / max(prof["backward"][1], 1)
+    fwd_flop = FWD_UNGATED * e_total + FWD_GATED * e_active
+    bwd_flop = BWD_UNGATED * e_total + BWD_GATED * e_active
+    dominant = "backward" if bwd_ms >= fwd_ms else "forward"
+    k_flop, k_ms = (bwd_flop, bwd_ms) if dominant == "backward" else (fwd_flop, fwd_ms)
+    achieved = k_flop / (k_ms * 1e-3) / 1e12
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline")
+        cpu_ms, note, cores = cpu_baseline(params, args.cpu_budget)
+        cpu = {"value": round(cpu_ms, 1), "unit": "ms", "cores": cores, "kind": "port", "sample": note}
+
+    out = {
+        "metric": METRIC,
+        "value": round(ms_per_step, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "voxelizer microbenchmark: 256x256x128 complex64 volume, 1M Gaussians, 8^3 tiles, "
+                               "fwd+bwd with random upstream gradient", "volume": list(SHAPE),
+                   "gaussians": N_GAUSS, "tile": TILE, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "kernel": f"k_{dominant}", "achieved": round(achieved, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": None},
+        "cpu_baseline": cpu,
+        "detail": {
+            "median_step_ms": round(statistics.median(per_step), 4),
+            "forward_kernel_ms": round(fwd_ms, 4),
+            "backward_kernel_ms": round(bwd_ms, 4),
+            "binning_ms": round(statistics.median(bt), 4),
+            "pairs": pairs,
+            "E_total": e_total,
+            "E_active": e_active,
+            "E_culled": e_culled,
+            "evals_per_s_fwd": e_total / (fwd_ms * 1e-3),
+            "evals_per_s_bwd": e_total / (bwd_ms * 1e-3),
+            "step_algorithmic_tflops": (fwd_flop + bwd_flop) / (ms_per_step * 1e-3) / 1e12,
+            "step_roofline_frac": (fwd_flop + bwd_flop) / (ms_per_step * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
+            "kernels_roofline_frac": (fwd_flop + bwd_flop) / ((fwd_ms + bwd_ms) * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,229 @@
+"""ctypes binding of ``libgstv.so`` (C ABI declared in include/gstv.h).
+
+torch is imported first on purpose: the library then binds to the HIP runtime
+torch already loaded, so streams and device pointers are shared.  The library
+is loaded from this package's ``_lib/`` directory, built in-tree by
+``gs_tmi_amd.build``.  There is no fallback: if the library or a GPU is
+missing, every entry point raises RuntimeError.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libgstv.so"
+
+_lock = threading.Lock()
+_lib = None
+_load_error: str | None = None
+
+_c_p = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_sz = ctypes.c_size_t
+
+# name -> (restype, argtypes); mirrors include/gstv.h.
+PROTOTYPES = {
+    "gstv_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "gstv_version": (ctypes.c_int, []),
+    "gstv_bin_count_workspace_bytes": (_sz, [_i32]),
+    "gstv_bin_count": (ctypes.c_int, [_c_p, _c_p, _i32, _i32, _i32, _i32, _i32, _c_p, _c_p, _c_p, _sz, _c_p]),
+    "gstv_bin_fill_workspace_bytes": (_sz, [_i32, _i64, _i32]),
+    "gstv_bin_fill": (ctypes.c_int,
+                      [_c_p, _c_p, _i32, _i32, _i32, _i32, _i32, _c_p, _i64, _c_p, _c_p, _c_p, _sz, _c_p]),
+    "gstv_forward_workspace_bytes": (_sz, [_i32]),
+    "gstv_forward": (ctypes.c_int,
+                     [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32,
+                      _c_p, _c_p, _c_p, _sz, _c_p]),
+    "gstv_backward_workspace_bytes": (_sz, [_i32]),
+    "gstv_backward": (ctypes.c_int,
+                      [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32,
+                       _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _sz, _c_p]),
+    "gstv_count_evaluations": (ctypes.c_int,
+                               [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32,
+                                _c_p, _c_p, _sz, _c_p]),
+    "gstv_profile_begin": (ctypes.c_int, []),
+    "gstv_profile_end": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32),
+                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]),
+}
+
+
+def load_library(path: Path | str | None = None):
+    """Load the library and declare its prototypes (no GPU call is made)."""
+    lib = ctypes.CDLL(str(path or LIB_PATH))
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def get_lib(strict: bool = True):
+    """Memoised loader with the reference's strict/non-strict contract
+    (gaussian/tile_voxelizer.py:23-74): a failure is remembered; strict mode
+    raises RuntimeError, non-strict returns None."""
+    global _lib, _load_error
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if _load_error is None:
+            try:
+                if not LIB_PATH.exists():
+                    raise FileNotFoundError(f"{LIB_PATH} is not built (run `python -m gs_tmi_amd.build`)")
+                _lib = load_library()
+                return _lib
+            except Exception as exc:  # noqa: BLE001 - any loader failure is reported the same way
+                _load_error = f"Failed to load the HIP tile voxelizer library: {exc}"
+        if strict:
+            raise RuntimeError(_load_error)
+        return None
+
+
+def _reset_for_tests():
+    global _lib, _load_error
+    with _lock:
+        _lib = None
+        _load_error = None
+
+
+def _check(status: int, what: str):
+    if status != 0:
+        msg = get_lib().gstv_status_string(status).decode()
+        raise RuntimeError(f"{what} failed: {msg} (status {status})")
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def _stream(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def _workspace(nbytes: int, device: torch.device) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+def _require_gpu(t: torch.Tensor, name: str):
+    if not t.is_cuda:
+        raise RuntimeError(f"{name} must be a ROCm GPU tensor; the HIP tile voxelizer has no CPU path")
+
+
+def num_tiles_for(shape, tile_size: int) -> int:
+    d, h, w = shape
+    return (-(-d // tile_size)) * (-(-h // tile_size)) * (-(-w // tile_size))
+
+
+def bin_count(center_vox: torch.Tensor, radius_vox: torch.Tensor, shape, tile_size: int):
+    """Phase one of binning.  Returns (gauss_offsets int32 [n+1], total int64 [1]) on the device."""
+    lib = get_lib()
+    _require_gpu(center_vox, "center_vox")
+    dev = center_vox.device
+    c = center_vox.detach().contiguous().float()
+    r = radius_vox.detach().contiguous().float()
+    n = c.shape[0]
+    offsets = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    total = torch.empty(1, dtype=torch.int64, device=dev)
+    ws_bytes = lib.gstv_bin_count_workspace_bytes(n)
+    ws = _workspace(ws_bytes, dev)
+    d, h, w = shape
+    with torch.cuda.device(dev):
+        st = lib.gstv_bin_count(_ptr(c), _ptr(r), n, d, h, w, tile_size, _ptr(offsets), _ptr(total), _ptr(ws),
+                                ws_bytes, _stream(dev))
+    _check(st, "gstv_bin_count")
+    return offsets, total
+
+
+def bin_fill(center_vox, radius_vox, shape, tile_size, offsets, total_pairs: int):
+    """Phase two of binning.  Returns (tile_ranges int32 [T,2], pair_ids int32 [P])."""
+    lib = get_lib()
+    dev = center_vox.device
+    c = center_vox.detach().contiguous().float()
+    r = radius_vox.detach().contiguous().float()
+    n = c.shape[0]
+    num_tiles = num_tiles_for(shape, tile_size)
+    if total_pairs >= 2**31 - 1:
+        _check(3, "gstv_bin_fill")
+    tile_ranges = torch.empty(num_tiles, 2, dtype=torch.int32, device=dev)
+    pair_ids = torch.empty(total_pairs, dtype=torch.int32, device=dev)
+    ws_bytes = lib.gstv_bin_fill_workspace_bytes(n, total_pairs, num_tiles)
+    ws = _workspace(ws_bytes, dev)
+    d, h, w = shape
+    with torch.cuda.device(dev):
+        st = lib.gstv_bin_fill(_ptr(c), _ptr(r), n, d, h, w, tile_size, _ptr(offsets), total_pairs,
+                               _ptr(tile_ranges), _ptr(pair_ids), _ptr(ws), ws_bytes, _stream(dev))
+    _check(st, "gstv_bin_fill")
+    return tile_ranges, pair_ids
+
+
+def forward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape, tile_size, d, h, w):
+    """Planar forward.  Returns (vol_r, vol_i), each fp32 [d*h*w]."""
+    lib = get_lib()
+    _require_gpu(center_vox, "center_vox")
+    dev = center_vox.device
+    n = center_vox.shape[0]
+    vol_r = torch.empty(d * h * w, dtype=torch.float32, device=dev)
+    vol_i = torch.empty(d * h * w, dtype=torch.float32, device=dev)
+    ws_bytes = lib.gstv_forward_workspace_bytes(n)
+    ws = _workspace(ws_bytes, dev)
+    with torch.cuda.device(dev):
+        st = lib.gstv_forward(_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i), _ptr(tile_ranges),
+                              tile_ranges.shape[0], _ptr(pair_ids), _ptr(half_shape), n, tile_size, d, h, w,
+                              _ptr(vol_r), _ptr(vol_i), _ptr(ws), ws_bytes, _stream(dev))
+    _check(st, "gstv_forward")
+    return vol_r, vol_i
+
+
+def backward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape, grad_r, grad_i, tile_size,
+             d, h, w):
+    """Adjoint.  Returns (grad_center [n,3], grad_cov_inv [n,9], grad_rho_r [n], grad_rho_i [n])."""
+    lib = get_lib()
+    _require_gpu(center_vox, "center_vox")
+    dev = center_vox.device
+    n = center_vox.shape[0]
+    g_c = torch.empty(n, 3, dtype=torch.float32, device=dev)
+    g_m = torch.empty(n, 9, dtype=torch.float32, device=dev)
+    g_rr = torch.empty(n, dtype=torch.float32, device=dev)
+    g_ri = torch.empty(n, dtype=torch.float32, device=dev)
+    ws_bytes = lib.gstv_backward_workspace_bytes(n)
+    ws = _workspace(ws_bytes, dev)
+    with torch.cuda.device(dev):
+        st = lib.gstv_backward(_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i), _ptr(tile_ranges),
+                               tile_ranges.shape[0], _ptr(pair_ids), _ptr(half_shape), n, _ptr(grad_r),
+                               _ptr(grad_i), tile_size, d, h, w, _ptr(g_c), _ptr(g_m), _ptr(g_rr), _ptr(g_ri),
+                               _ptr(ws), ws_bytes, _stream(dev))
+    _check(st, "gstv_backward")
+    return g_c, g_m, g_rr, g_ri
+
+
+def count_evaluations(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape, tile_size, d, h, w):
+    """Exact (E_total, E_active, E_culled) of the tile kernels for this binning (synchronises)."""
+    lib = get_lib()
+    dev = center_vox.device
+    n = center_vox.shape[0]
+    counts = torch.zeros(3, dtype=torch.int64, device=dev)
+    ws_bytes = lib.gstv_forward_workspace_bytes(n)
+    ws = _workspace(ws_bytes, dev)
+    with torch.cuda.device(dev):
+        st = lib.gstv_count_evaluations(_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i),
+                                        _ptr(tile_ranges), tile_ranges.shape[0], _ptr(pair_ids), _ptr(half_shape),
+                                        n, tile_size, d, h, w, _ptr(counts), _ptr(ws), ws_bytes, _stream(dev))
+    _check(st, "gstv_count_evaluations")
+    return tuple(int(v) for v in counts.cpu().tolist())
+
+
+def profile_begin():
+    _check(get_lib().gstv_profile_begin(), "gstv_profile_begin")
+
+
+def profile_end():
+    """Returns {"forward": (ms_total, launches), "backward": (ms_total, launches)}."""
+    fms, bms = ctypes.c_double(), ctypes.c_double()
+    fl, bl = ctypes.c_int32(), ctypes.c_int32()
+    _check(get_lib().gstv_profile_end(ctypes.byref(fms), ctypes.byref(fl), ctypes.byref(bms), ctypes.byref(bl)),
+           "gstv_profile_end")
+    return {"forward": (fms.value, fl.value), "backward": (bms.value, bl.value)}

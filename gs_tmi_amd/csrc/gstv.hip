@@ -1,0 +1,1302 @@
+// gstv.hip - MI355X (gfx950, CDNA4) tile voxelizer: binning, forward, adjoint.
+//
+// Hot path of 3DGSMR's tile voxelizer (gaussian/tile_voxelizer.py:87-162,
+// gaussian/cuda_ext/tile_voxelize_cuda_kernel.cu:48-349), written for 64-lane
+// wavefronts.  Everything sits behind the C ABI declared in include/gstv.h.
+//
+// Layout of the work
+//   * Binning: per-Gaussian tile counts -> device-wide exclusive scan ->
+//     expansion of (tile id, Gaussian id) pairs in Gaussian-major order -> stable
+//     LSD radix sort on the tile id (8-bit digits, wave-level multisplit
+//     ranking) -> per-tile [start, end) ranges.  Stability makes the order
+//     inside a tile ascending in Gaussian id.
+//   * A per-Gaussian record (64 B: centre, density, quadratic form in voxel
+//     units pre-scaled by -log2(e)/2, and the exact axis-aligned extent of the
+//     q <= 9 ellipsoid) is built once per call, so the tile kernels gather one
+//     cache line per (tile, Gaussian) pair.
+//   * Tile kernels: one 512-thread block per 8^3 tile (one block per 8
+//     sub-bricks for larger tiles); each wave owns a 4x4x4 sub-brick.  A batch
+//     of up to 512 pairs is staged in LDS; every wave compacts the Gaussians
+//     whose exact extent touches its sub-brick (ballot) and evaluates them 16
+//     at a time: lane = (Gaussian slot 0..15, W offset 0..3), each lane walks
+//     the 4x4 (D, H) positions of its W column, so the quadratic form costs
+//     two FMAs per evaluation.  The forward keeps 16 complex partial sums per
+//     lane and reduces them across the 16 Gaussian slots once per tile.  The
+//     backward accumulates the 11 gradient moments of its Gaussian over the
+//     16 positions, reduces across the 4 W lanes, adds them into a per-batch
+//     LDS table, and flushes each (tile, Gaussian) total with one fp32 atomic
+//     per term.  A final per-Gaussian kernel maps moments to the reference's
+//     outputs (centre, cov_inv, density gradients).
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+#include "gstv.h"
+
+namespace {
+
+// ------------------------------------------------------------------ common --
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;                      // items per thread in scans
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+constexpr int kExpandThreads = 256;
+constexpr int kExpandTile = 1024;                  // pairs per expansion block
+
+constexpr int kRadixThreads = 256;
+constexpr int kRadixRounds = 16;
+constexpr int kRadixTile = kRadixThreads * kRadixRounds;  // keys per radix block
+constexpr int kRadixBits = 8;
+
+constexpr int kTileThreads = 512;                  // 8 waves per tile block
+constexpr int kWaves = kTileThreads / 64;
+constexpr int kBatch = 512;                        // pairs staged per LDS batch
+constexpr int kSub = 4;                            // sub-brick edge (voxels)
+
+// -log2(e)/2: q' = kQScale * q, w = exp(-q/2) = exp2(q').
+constexpr double kQScaleD = -0.72134752044448170368;  // -0.5 / ln(2)
+// q <= 9  <=>  q' >= 9 * kQScale
+constexpr float kThr = (float)(9.0 * kQScaleD);
+
+constexpr uint32_t kEmptyBox = 0x80000000u;
+
+thread_local char g_last_detail[256] = "";
+
+int hip_status(hipError_t e, const char* where) {
+  if (e == hipSuccess) return GSTV_OK;
+  std::snprintf(g_last_detail, sizeof(g_last_detail), "%s: %s", where, hipGetErrorString(e));
+  return GSTV_ERR_HIP;
+}
+
+#define GSTV_LAUNCH_CHECK(where)                                   \
+  do {                                                             \
+    int _st = hip_status(hipGetLastError(), where);                \
+    if (_st != GSTV_OK) return _st;                                \
+  } while (0)
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+inline int ceil_div_i(int a, int b) { return (a + b - 1) / b; }
+inline int64_t ceil_div_l(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Carves consecutive 256-byte aligned regions out of a workspace.
+struct Carve {
+  char* base;
+  size_t off = 0;
+  explicit Carve(void* p) : base(static_cast<char*>(p)) {}
+  template <typename T>
+  T* take(size_t count) {
+    off = align_up(off, 256);
+    T* p = reinterpret_cast<T*>(base ? base + off : nullptr);
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+// Blocks are dispatched to the 8 XCDs round-robin.  Give each XCD a contiguous
+// range of logical block ids so neighbouring tiles (which share Gaussians) hit
+// the same L2.  grid is a multiple of 8; the map is a bijection on [0, grid).
+__device__ __forceinline__ int xcd_remap(int b, int grid) {
+  const int per = grid >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
+// ---------------------------------------------------------------- scanning --
+
+__device__ __forceinline__ int64_t wave_inclusive_scan(int64_t v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int64_t o = __shfl_up(v, off, 64);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+
+// Exclusive scan across a 256-thread block; *total gets the block sum.
+__device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t* sh, int64_t* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int64_t inc = wave_inclusive_scan(v, lane);
+  if (lane == 63) sh[wid] = inc;
+  __syncthreads();
+  int64_t base = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kScanThreads / 64; ++i) {
+    int64_t s = sh[i];
+    base += (i < wid) ? s : 0;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + inc - v;
+}
+
+// Phase 1: per-block sums of int32 input.
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int32_t* __restrict__ in, int64_t n,
+                                                              int64_t* __restrict__ block_sums) {
+  __shared__ int64_t sh[kScanThreads / 64];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  int64_t s = 0;
+#pragma unroll
+  for (int r = 0; r < kScanItems; ++r) {
+    int64_t i = base + (int64_t)r * kScanThreads + threadIdx.x;
+    if (i < n) s += in[i];
+  }
+  int64_t tot;
+  block_exclusive_scan(s, sh, &tot);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+// Phase 2: exclusive scan of the block sums in one block; total to *total.
+__global__ __launch_bounds__(kScanThreads) void k_scan_block_sums(int64_t* __restrict__ sums, int nb,
+                                                                  int64_t* __restrict__ total) {
+  __shared__ int64_t sh[kScanThreads / 64];
+  int64_t carry = 0;
+  for (int base = 0; base < nb; base += kScanThreads) {
+    const int i = base + threadIdx.x;
+    int64_t v = i < nb ? sums[i] : 0;
+    int64_t tot;
+    int64_t ex = block_exclusive_scan(v, sh, &tot);
+    if (i < nb) sums[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total != nullptr) *total = carry;
+}
+
+// Phase 3: exclusive scan inside each block plus the block offset.  Each
+// thread owns kScanItems consecutive items.  out may alias in.
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int32_t* in, int64_t n,
+                                                             const int64_t* __restrict__ block_offsets,
+                                                             int32_t* out) {
+  __shared__ int64_t sh[kScanThreads / 64];
+  const int64_t first = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int32_t v[kScanItems];
+  int64_t s = 0;
+#pragma unroll
+  for (int r = 0; r < kScanItems; ++r) {
+    const int64_t i = first + r;
+    v[r] = i < n ? in[i] : 0;
+    s += v[r];
+  }
+  int64_t tot;
+  int64_t run = block_exclusive_scan(s, sh, &tot) + block_offsets[blockIdx.x];
+#pragma unroll
+  for (int r = 0; r < kScanItems; ++r) {
+    const int64_t i = first + r;
+    if (i < n) out[i] = (int32_t)run;
+    run += v[r];
+  }
+}
+
+// Device-wide exclusive scan of n int32 values into int32 (values must fit).
+// block_sums needs ceil(n / kScanTile) entries.
+int launch_exclusive_scan(const int32_t* in, int64_t n, int32_t* out, int64_t* block_sums,
+                          int64_t* total, hipStream_t s) {
+  const int nb = (int)ceil_div_l(n, kScanTile);
+  if (nb == 0) return GSTV_OK;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanThreads), 0, s, in, n, block_sums);
+  GSTV_LAUNCH_CHECK("scan_reduce");
+  hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(kScanThreads), 0, s, block_sums, nb, total);
+  GSTV_LAUNCH_CHECK("scan_block_sums");
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanThreads), 0, s, in, n,
+                     (const int64_t*)block_sums, out);
+  GSTV_LAUNCH_CHECK("scan_apply");
+  return GSTV_OK;
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, int64_t n, float v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ----------------------------------------------------------------- binning --
+
+struct BinGeom {
+  int size[3];
+  int nt[3];
+  int tile;
+};
+
+// float -> int32 conversion with truncation, saturation and NaN -> 0 (the
+// semantics of a device-side float->int cast, which the reference's GPU path
+// applies in `.int()` at gaussian/tile_voxelizer.py:98-99).
+__device__ __forceinline__ int sat_f2i(float x) {
+  if (!(x == x)) return 0;
+  if (x >= 2147483648.0f) return INT_MAX;
+  if (x <= -2147483648.0f) return INT_MIN;
+  return (int)x;
+}
+
+// Tile box of one Gaussian; gaussian/tile_voxelizer.py:97-116.
+__device__ __forceinline__ void gauss_tile_box(const float* __restrict__ center, const float* __restrict__ radius,
+                                               int g, const BinGeom& geo, int tmin[3], int tcnt[3]) {
+  const float r = radius[g];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float c = center[3 * (int64_t)g + a];
+    int lo = sat_f2i(floorf(c - r));
+    int hi = sat_f2i(ceilf(c + r));
+    lo = min(max(lo, 0), geo.size[a] - 1);
+    hi = min(max(hi, 0), geo.size[a] - 1);
+    int tl = min(max(lo / geo.tile, 0), geo.nt[a] - 1);
+    int th = min(max(hi / geo.tile, 0), geo.nt[a] - 1);
+    tmin[a] = tl;
+    tcnt[a] = max(th - tl + 1, 0);  // a negative radius can invert the box: no pairs
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bin_count(const float* __restrict__ center,
+                                                   const float* __restrict__ radius, int n, BinGeom geo,
+                                                   int32_t* __restrict__ counts) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= n) return;
+  int tmin[3], tcnt[3];
+  gauss_tile_box(center, radius, g, geo, tmin, tcnt);
+  counts[g] = tcnt[0] * tcnt[1] * tcnt[2];  // <= num_tiles < 2^31
+}
+
+// Largest i in [0, len) with a[i] <= x (a non-decreasing, a[0] <= x).
+__device__ __forceinline__ int last_le(const int32_t* a, int len, int32_t x) {
+  int lo = 0, hi = len - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a[mid] <= x) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Expands pairs in Gaussian-major, row-major-local order
+// (gaussian/tile_voxelizer.py:126-145).  Each block owns kExpandTile
+// consecutive pair slots; it locates the Gaussians covering them by binary
+// search in the offsets, caches their tile boxes in LDS, and writes keys
+// (tile id) and values (Gaussian id) coalesced.
+__global__ __launch_bounds__(kExpandThreads) void k_bin_expand(
+    const float* __restrict__ center, const float* __restrict__ radius, int n, BinGeom geo,
+    const int32_t* __restrict__ offsets, int32_t total, int32_t* __restrict__ keys,
+    int32_t* __restrict__ vals) {
+  __shared__ int32_t sh_off[kExpandTile + 1];
+  __shared__ int32_t sh_t0[kExpandTile], sh_t1[kExpandTile], sh_t2[kExpandTile];
+  __shared__ int32_t sh_n1[kExpandTile], sh_n2[kExpandTile];
+  __shared__ int sh_g0, sh_ng;
+  const int j0 = blockIdx.x * kExpandTile;
+  const int j1 = min(j0 + kExpandTile, total);
+  if (threadIdx.x == 0) {
+    const int g_first = last_le(offsets, n + 1, j0);
+    const int g_last = last_le(offsets, n + 1, j1 - 1);
+    sh_g0 = g_first;
+    sh_ng = g_last - g_first + 1;
+  }
+  __syncthreads();
+  const int g0 = sh_g0, ng = sh_ng;
+  const int ntw = geo.nt[2];
+  const int nthw = geo.nt[1] * geo.nt[2];
+  if (ng > kExpandTile) {
+    // Only reachable when Gaussians with zero pairs sit inside this block's
+    // range: locate each pair's Gaussian in global memory instead.
+    for (int j = j0 + threadIdx.x; j < j1; j += kExpandThreads) {
+      const int g = last_le(offsets, n + 1, j);
+      int tmin[3], tcnt[3];
+      gauss_tile_box(center, radius, g, geo, tmin, tcnt);
+      const int l = j - offsets[g];
+      const int n12 = tcnt[1] * tcnt[2];
+      const int ld = l / n12;
+      const int rem = l - ld * n12;
+      const int lh = rem / tcnt[2];
+      const int lw = rem - lh * tcnt[2];
+      keys[j] = (tmin[0] + ld) * nthw + (tmin[1] + lh) * ntw + (tmin[2] + lw);
+      vals[j] = g;
+    }
+    return;
+  }
+  for (int i = threadIdx.x; i < ng; i += kExpandThreads) {
+    int tmin[3], tcnt[3];
+    gauss_tile_box(center, radius, g0 + i, geo, tmin, tcnt);
+    sh_off[i] = offsets[g0 + i];
+    sh_t0[i] = tmin[0];
+    sh_t1[i] = tmin[1];
+    sh_t2[i] = tmin[2];
+    sh_n1[i] = tcnt[1];
+    sh_n2[i] = tcnt[2];
+  }
+  __syncthreads();
+  for (int j = j0 + threadIdx.x; j < j1; j += kExpandThreads) {
+    const int i = last_le(sh_off, ng, j);
+    const int l = j - sh_off[i];
+    const int n2 = sh_n2[i];
+    const int n12 = sh_n1[i] * n2;
+    const int ld = l / n12;
+    const int rem = l - ld * n12;
+    const int lh = rem / n2;
+    const int lw = rem - lh * n2;
+    keys[j] = (sh_t0[i] + ld) * nthw + (sh_t1[i] + lh) * ntw + (sh_t2[i] + lw);
+    vals[j] = g0 + i;
+  }
+}
+
+// Radix upsweep: digit histogram of each kRadixTile-key block, stored
+// digit-major (hist[digit * nblocks + block]) so one exclusive scan gives
+// every block's scatter base per digit.
+__global__ __launch_bounds__(kRadixThreads) void k_radix_hist(const int32_t* __restrict__ keys, int total,
+                                                             int shift, int bits, int nblocks,
+                                                             int32_t* __restrict__ hist) {
+  __shared__ int32_t sh_h[1 << kRadixBits];
+  sh_h[threadIdx.x] = 0;
+  __syncthreads();
+  const int mask = (1 << bits) - 1;
+  const int base = blockIdx.x * kRadixTile;
+#pragma unroll 4
+  for (int r = 0; r < kRadixRounds; ++r) {
+    const int idx = base + r * kRadixThreads + threadIdx.x;
+    if (idx < total) atomicAdd(&sh_h[(keys[idx] >> shift) & mask], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < (1 << bits)) hist[threadIdx.x * nblocks + blockIdx.x] = sh_h[threadIdx.x];
+}
+
+// Radix downsweep: stable scatter.  Keys of a block are visited in 16 rounds
+// of 256 in index order; inside a round a wave ranks equal digits with a
+// ballot multisplit, and per-wave digit counts give the order across waves.
+__global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
+    const int32_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in, int total, int shift,
+    int bits, int nblocks, const int32_t* __restrict__ hist_scanned, int32_t* __restrict__ keys_out,
+    int32_t* __restrict__ vals_out) {
+  constexpr int kDigits = 1 << kRadixBits;
+  constexpr int kW = kRadixThreads / 64;
+  __shared__ int32_t sh_base[kDigits];
+  __shared__ int32_t sh_wcnt[kW][kDigits];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int ndig = 1 << bits;
+  const int mask = ndig - 1;
+  sh_base[threadIdx.x] = threadIdx.x < ndig ? hist_scanned[threadIdx.x * nblocks + blockIdx.x] : 0;
+#pragma unroll
+  for (int w = 0; w < kW; ++w) sh_wcnt[w][threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int base = blockIdx.x * kRadixTile;
+  for (int r = 0; r < kRadixRounds; ++r) {
+    const int idx = base + r * kRadixThreads + threadIdx.x;
+    const bool valid = idx < total;
+    const int32_t key = valid ? keys_in[idx] : 0;
+    const int32_t val = valid ? vals_in[idx] : 0;
+    const int dig = (key >> shift) & mask;
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < bits; ++b) {
+      const bool bit = (dig >> b) & 1;
+      const uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const int wrank = __popcll(peers & lt_mask);
+    if (valid && wrank == 0) sh_wcnt[wid][dig] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      int pos = sh_base[dig] + wrank;
+#pragma unroll
+      for (int w = 0; w < kW; ++w) pos += (w < wid) ? sh_wcnt[w][dig] : 0;
+      keys_out[pos] = key;
+      vals_out[pos] = val;
+    }
+    __syncthreads();
+    {
+      int add = 0;
+#pragma unroll
+      for (int w = 0; w < kW; ++w) {
+        add += sh_wcnt[w][threadIdx.x];
+        sh_wcnt[w][threadIdx.x] = 0;
+      }
+      sh_base[threadIdx.x] += add;
+    }
+    __syncthreads();
+  }
+}
+
+// tile_ranges from sorted keys; gaussian/tile_voxelizer.py:151-161.
+__global__ void k_ranges(const int32_t* __restrict__ keys, int total, int32_t* __restrict__ ranges) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= total) return;
+  const int t = keys[j];
+  if (j == 0 || keys[j - 1] != t) ranges[2 * t] = j;
+  if (j == total - 1 || keys[j + 1] != t) ranges[2 * t + 1] = j + 1;
+}
+
+// -------------------------------------------------------- Gaussian records --
+
+struct __align__(16) GRec {
+  float c0, c1, c2, rr;      // centre (voxel units, axes D,H,W), density real
+  float ri, a00, a11, a22;   // density imag, diagonal of the scaled form
+  float b01, b02, b12, e0;   // doubled off-diagonals of the scaled form; extent D
+  float e1, e2, pad0, pad1;  // extents H, W
+};
+static_assert(sizeof(GRec) == 64, "record must be one 64-byte line");
+
+// Record of one Gaussian.  The quadratic form of the reference,
+// q = d^T M d with d = (x - c) / hs, is rewritten in voxel units u = x - c as
+// q = sum_ij A_ij u_i u_j with A = sym(M) / (hs hs^T), and pre-scaled by
+// -log2(e)/2 so that w = exp2(q').  The extent e_a bounds |u_a| on the
+// ellipsoid q <= 9: e_a = 3 hs_a sqrt((S^-1)_aa), S = sym(M), evaluated in
+// fp64 and widened by 2% + 0.01 voxel so that no voxel whose fp32 q passes the
+// cutoff is culled.  A matrix that is not positive definite, or not finite,
+// gets an infinite extent (no culling).
+__global__ __launch_bounds__(256) void k_prep(const float* __restrict__ center, const float* __restrict__ cov,
+                                              const float* __restrict__ rho_r, const float* __restrict__ rho_i,
+                                              const float* __restrict__ half_shape, int n,
+                                              GRec* __restrict__ rec) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= n) return;
+  const float hs0 = half_shape[0], hs1 = half_shape[1], hs2 = half_shape[2];
+  const float* m = cov + 9 * (int64_t)g;
+  float mm[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) mm[i] = m[i];
+  const float kq = (float)kQScaleD;
+  GRec r;
+  r.c0 = center[3 * (int64_t)g + 0];
+  r.c1 = center[3 * (int64_t)g + 1];
+  r.c2 = center[3 * (int64_t)g + 2];
+  r.rr = rho_r[g];
+  r.ri = rho_i[g];
+  r.a00 = kq * (mm[0] / (hs0 * hs0));
+  r.a11 = kq * (mm[4] / (hs1 * hs1));
+  r.a22 = kq * (mm[8] / (hs2 * hs2));
+  r.b01 = kq * ((mm[1] + mm[3]) / (hs0 * hs1));
+  r.b02 = kq * ((mm[2] + mm[6]) / (hs0 * hs2));
+  r.b12 = kq * ((mm[5] + mm[7]) / (hs1 * hs2));
+  // Extents in fp64.
+  const double s00 = mm[0], s11 = mm[4], s22 = mm[8];
+  const double s01 = 0.5 * ((double)mm[1] + mm[3]);
+  const double s02 = 0.5 * ((double)mm[2] + mm[6]);
+  const double s12 = 0.5 * ((double)mm[5] + mm[7]);
+  const double cof00 = s11 * s22 - s12 * s12;
+  const double cof11 = s00 * s22 - s02 * s02;
+  const double cof22 = s00 * s11 - s01 * s01;
+  const double cof01 = s02 * s12 - s01 * s22;
+  const double cof02 = s01 * s12 - s02 * s11;
+  const double det = s00 * cof00 + s01 * cof01 + s02 * cof02;
+  float e0 = INFINITY, e1 = INFINITY, e2 = INFINITY;
+  const bool pd = (s00 > 0.0) && (cof22 > 0.0) && (det > 0.0) && isfinite(det) && cof00 > 0.0 &&
+                  cof11 > 0.0;
+  if (pd) {
+    const double f = 3.0 * 1.02;
+    const double x0 = f * (double)hs0 * sqrt(cof00 / det) + 0.01;
+    const double x1 = f * (double)hs1 * sqrt(cof11 / det) + 0.01;
+    const double x2 = f * (double)hs2 * sqrt(cof22 / det) + 0.01;
+    if (isfinite(x0) && isfinite(x1) && isfinite(x2)) {
+      e0 = (float)x0;
+      e1 = (float)x1;
+      e2 = (float)x2;
+    }
+  }
+  r.e0 = e0;
+  r.e1 = e1;
+  r.e2 = e2;
+  r.pad0 = 0.0f;
+  r.pad1 = 0.0f;
+  rec[g] = r;
+}
+
+// ------------------------------------------------------------ tile kernels --
+
+struct VoxGeom {
+  int D, H, W, tile;
+  int nTH, nTW, num_tiles;
+  int nsb;     // sub-bricks per tile axis, ceil(tile / 4)
+  int chunks;  // blocks per tile, ceil(nsb^3 / 8)
+  int blocks;  // num_tiles * chunks
+};
+
+struct Stage {
+  float c0[kBatch], c1[kBatch], c2[kBatch], rr[kBatch], ri[kBatch];
+  float a00[kBatch], a11[kBatch], a22[kBatch], b01[kBatch], b02[kBatch], b12[kBatch];
+  uint32_t box[kBatch];
+  uint16_t list[kWaves][kBatch];
+};
+
+struct BlockCtx {
+  int tile;
+  int o0, o1, o2;  // tile origin (voxels)
+  int e0, e1, e2;  // valid tile extent per axis
+  int s0, s1, s2;  // this wave's sub-brick origin, tile-local
+  int n0, n1, n2;  // valid voxels of the sub-brick per axis (<= 4)
+  bool active;     // the wave owns at least one valid voxel
+  int p_begin, p_end;
+};
+
+__device__ __forceinline__ BlockCtx make_ctx(int logical, const VoxGeom& geo, const int32_t* __restrict__ ranges,
+                                             int wave) {
+  BlockCtx b;
+  const int tile = logical / geo.chunks;
+  const int chunk = logical - tile * geo.chunks;
+  b.tile = tile;
+  const int nthw = geo.nTH * geo.nTW;
+  const int td = tile / nthw;
+  const int rem = tile - td * nthw;
+  const int th = rem / geo.nTW;
+  const int tw = rem - th * geo.nTW;
+  b.o0 = td * geo.tile;
+  b.o1 = th * geo.tile;
+  b.o2 = tw * geo.tile;
+  b.e0 = min(geo.tile, geo.D - b.o0);
+  b.e1 = min(geo.tile, geo.H - b.o1);
+  b.e2 = min(geo.tile, geo.W - b.o2);
+  const int sb = chunk * kWaves + wave;
+  const int nsb = geo.nsb;
+  const int sb0 = sb / (nsb * nsb);
+  const int sb1 = (sb / nsb) % nsb;
+  const int sb2 = sb % nsb;
+  b.s0 = sb0 * kSub;
+  b.s1 = sb1 * kSub;
+  b.s2 = sb2 * kSub;
+  b.n0 = min(kSub, b.e0 - b.s0);
+  b.n1 = min(kSub, b.e1 - b.s1);
+  b.n2 = min(kSub, b.e2 - b.s2);
+  b.active = (sb < nsb * nsb * nsb) && b.n0 > 0 && b.n1 > 0 && b.n2 > 0;
+  b.p_begin = ranges[2 * tile];
+  b.p_end = ranges[2 * tile + 1];
+  return b;
+}
+
+// Stages one pair: copies the record fields and computes the Gaussian's
+// exact extent clipped to the tile, in tile-local voxel indices, packed as
+// 6 x 5 bits (lo0, lo1, lo2, hi0, hi1, hi2); kEmptyBox if it misses the tile.
+__device__ __forceinline__ void stage_pair(Stage& sh, int k, const GRec& r, const BlockCtx& b) {
+  sh.c0[k] = r.c0;
+  sh.c1[k] = r.c1;
+  sh.c2[k] = r.c2;
+  sh.rr[k] = r.rr;
+  sh.ri[k] = r.ri;
+  sh.a00[k] = r.a00;
+  sh.a11[k] = r.a11;
+  sh.a22[k] = r.a22;
+  sh.b01[k] = r.b01;
+  sh.b02[k] = r.b02;
+  sh.b12[k] = r.b12;
+  // fmaxf / fminf return the non-NaN operand, so a NaN centre keeps the whole
+  // tile (its q is NaN and fails the cutoff, as in the reference).
+  const float lo0 = fmaxf(ceilf(r.c0 - r.e0), (float)b.o0);
+  const float lo1 = fmaxf(ceilf(r.c1 - r.e1), (float)b.o1);
+  const float lo2 = fmaxf(ceilf(r.c2 - r.e2), (float)b.o2);
+  const float hi0 = fminf(floorf(r.c0 + r.e0), (float)(b.o0 + b.e0 - 1));
+  const float hi1 = fminf(floorf(r.c1 + r.e1), (float)(b.o1 + b.e1 - 1));
+  const float hi2 = fminf(floorf(r.c2 + r.e2), (float)(b.o2 + b.e2 - 1));
+  uint32_t box = kEmptyBox;
+  if (lo0 <= hi0 && lo1 <= hi1 && lo2 <= hi2) {
+    const uint32_t l0 = (uint32_t)((int)lo0 - b.o0), l1 = (uint32_t)((int)lo1 - b.o1),
+                   l2 = (uint32_t)((int)lo2 - b.o2);
+    const uint32_t h0 = (uint32_t)((int)hi0 - b.o0), h1 = (uint32_t)((int)hi1 - b.o1),
+                   h2 = (uint32_t)((int)hi2 - b.o2);
+    box = l0 | (l1 << 5) | (l2 << 10) | (h0 << 15) | (h1 << 20) | (h2 << 25);
+  }
+  sh.box[k] = box;
+}
+
+// Compacts, in batch order, the staged Gaussians whose box touches this
+// wave's sub-brick into sh.list[wave]; returns the count (wave-uniform).
+__device__ __forceinline__ int build_list(Stage& sh, int cnt, const BlockCtx& b, int wave, int lane) {
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int nhit = 0;
+  for (int base = 0; base < cnt; base += 64) {
+    const int k = base + lane;
+    bool hit = false;
+    if (k < cnt) {
+      const uint32_t bx = sh.box[k];
+      const int lo0 = bx & 31, lo1 = (bx >> 5) & 31, lo2 = (bx >> 10) & 31;
+      const int hi0 = (bx >> 15) & 31, hi1 = (bx >> 20) & 31, hi2 = (bx >> 25) & 31;
+      hit = !(bx & kEmptyBox) && lo0 < b.s0 + kSub && hi0 >= b.s0 && lo1 < b.s1 + kSub && hi1 >= b.s1 &&
+            lo2 < b.s2 + kSub && hi2 >= b.s2;
+    }
+    const uint64_t m = __ballot(hit);
+    if (hit) sh.list[wave][nhit + __popcll(m & lt_mask)] = (uint16_t)k;
+    nhit += __popcll(m);
+  }
+  return nhit;
+}
+
+struct GaussLane {
+  float c0, c1, rr, ri, a00, a11, b01;
+  float k0, k1, k2;  // u2-dependent terms of the form
+  float u1[kSub];    // H offsets of the 4 sub-brick rows
+  float u2;
+};
+
+__device__ __forceinline__ GaussLane load_gauss(const Stage& sh, int k, const BlockCtx& b, int xi) {
+  GaussLane g;
+  g.c0 = sh.c0[k];
+  g.c1 = sh.c1[k];
+  const float c2 = sh.c2[k];
+  g.rr = sh.rr[k];
+  g.ri = sh.ri[k];
+  g.a00 = sh.a00[k];
+  g.a11 = sh.a11[k];
+  const float a22 = sh.a22[k];
+  g.b01 = sh.b01[k];
+  const float b02 = sh.b02[k];
+  const float b12 = sh.b12[k];
+  g.u2 = (float)(b.o2 + b.s2 + xi) - c2;
+  g.k0 = a22 * g.u2 * g.u2;
+  g.k1 = b02 * g.u2;
+  g.k2 = b12 * g.u2;
+#pragma unroll
+  for (int y = 0; y < kSub; ++y) g.u1[y] = (float)(b.o1 + b.s1 + y) - g.c1;
+  return g;
+}
+
+__global__ __launch_bounds__(kTileThreads) void k_forward(const GRec* __restrict__ rec,
+                                                          const int32_t* __restrict__ ranges,
+                                                          const int32_t* __restrict__ pair_ids, VoxGeom geo,
+                                                          float* __restrict__ vol_r, float* __restrict__ vol_i) {
+  __shared__ Stage sh;
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  if (logical >= geo.blocks) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int gl = lane & 15, xi = lane >> 4;
+  const BlockCtx b = make_ctx(logical, geo, ranges, wave);
+  const bool x_ok = xi < b.n2;
+
+  float acc_r[kSub * kSub], acc_i[kSub * kSub];
+#pragma unroll
+  for (int i = 0; i < kSub * kSub; ++i) {
+    acc_r[i] = 0.0f;
+    acc_i[i] = 0.0f;
+  }
+
+  for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
+    const int cnt = min(kBatch, b.p_end - bs);
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) stage_pair(sh, threadIdx.x, rec[pair_ids[bs + threadIdx.x]], b);
+    __syncthreads();
+    if (!b.active) continue;
+    const int nhit = build_list(sh, cnt, b, wave, lane);
+    for (int gb = 0; gb < nhit; gb += 16) {
+      const int idx = gb + gl;
+      const bool g_ok = (idx < nhit) && x_ok;
+      const int k = sh.list[wave][idx < nhit ? idx : gb];
+      const GaussLane g = load_gauss(sh, k, b, xi);
+#pragma unroll
+      for (int z = 0; z < kSub; ++z) {
+        if (z < b.n0) {
+          const float u0 = (float)(b.o0 + b.s0 + z) - g.c0;
+          const float A = fmaf(fmaf(g.a00, u0, g.k1), u0, g.k0);
+          const float B = fmaf(g.b01, u0, g.k2);
+#pragma unroll
+          for (int y = 0; y < kSub; ++y) {
+            if (y < b.n1) {
+              const float u1 = g.u1[y];
+              const float q = fmaf(fmaf(g.a11, u1, B), u1, A);
+              const bool ok = g_ok && (q >= kThr);
+              const float w = ok ? __builtin_amdgcn_exp2f(q) : 0.0f;
+              acc_r[z * kSub + y] = fmaf(w, g.rr, acc_r[z * kSub + y]);
+              acc_i[z * kSub + y] = fmaf(w, g.ri, acc_i[z * kSub + y]);
+            }
+          }
+        }
+      }
+    }
+  }
+
+  // Sum the 16 Gaussian slots of each W column (a row of 16 lanes).
+#pragma unroll
+  for (int i = 0; i < kSub * kSub; ++i) {
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      acc_r[i] += __shfl_xor(acc_r[i], off, 16);
+      acc_i[i] += __shfl_xor(acc_i[i], off, 16);
+    }
+  }
+  // Lane gl of row xi writes position (z, y) = (gl / 4, gl % 4) of column xi.
+  float out_r = acc_r[0], out_i = acc_i[0];
+#pragma unroll
+  for (int i = 1; i < kSub * kSub; ++i) {
+    out_r = (gl == i) ? acc_r[i] : out_r;
+    out_i = (gl == i) ? acc_i[i] : out_i;
+  }
+  const int z = gl >> 2, y = gl & 3;
+  if (b.active && x_ok && z < b.n0 && y < b.n1) {
+    const int64_t flat = ((int64_t)(b.o0 + b.s0 + z) * geo.H + (b.o1 + b.s1 + y)) * geo.W + (b.o2 + b.s2 + xi);
+    vol_r[flat] = out_r;
+    vol_i[flat] = out_i;
+  }
+}
+
+constexpr int kMoments = 11;  // rho_r, rho_i, 3 first moments, 6 second moments
+constexpr int kAccStride = 12;
+
+__global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restrict__ rec,
+                                                           const int32_t* __restrict__ ranges,
+                                                           const int32_t* __restrict__ pair_ids, VoxGeom geo,
+                                                           const float* __restrict__ grad_r,
+                                                           const float* __restrict__ grad_i,
+                                                           float* __restrict__ gacc) {
+  __shared__ Stage sh;
+  __shared__ float sh_acc[kMoments][kBatch];
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  if (logical >= geo.blocks) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int gl = lane & 15, xi = lane >> 4;
+  const BlockCtx b = make_ctx(logical, geo, ranges, wave);
+  const bool x_ok = xi < b.n2;
+
+  // Upstream gradient of this lane's W column (16 positions), zero outside.
+  float gr[kSub * kSub], gi[kSub * kSub];
+#pragma unroll
+  for (int z = 0; z < kSub; ++z) {
+#pragma unroll
+    for (int y = 0; y < kSub; ++y) {
+      float vr = 0.0f, vi = 0.0f;
+      if (b.active && x_ok && z < b.n0 && y < b.n1) {
+        const int64_t flat =
+            ((int64_t)(b.o0 + b.s0 + z) * geo.H + (b.o1 + b.s1 + y)) * geo.W + (b.o2 + b.s2 + xi);
+        vr = grad_r[flat];
+        vi = grad_i[flat];
+      }
+      gr[z * kSub + y] = vr;
+      gi[z * kSub + y] = vi;
+    }
+  }
+
+  for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
+    const int cnt = min(kBatch, b.p_end - bs);
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) stage_pair(sh, threadIdx.x, rec[pair_ids[bs + threadIdx.x]], b);
+#pragma unroll
+    for (int f = 0; f < kMoments; ++f) sh_acc[f][threadIdx.x] = 0.0f;
+    __syncthreads();
+    if (b.active) {
+      const int nhit = build_list(sh, cnt, b, wave, lane);
+      for (int gb = 0; gb < nhit; gb += 16) {
+        const int idx = gb + gl;
+        const bool slot_ok = idx < nhit;
+        const bool g_ok = slot_ok && x_ok;
+        const int k = sh.list[wave][slot_ok ? idx : gb];
+        const GaussLane g = load_gauss(sh, k, b, xi);
+        float dr = 0.0f, di = 0.0f;
+        float m = 0.0f, m0 = 0.0f, m00 = 0.0f, m1 = 0.0f, m01 = 0.0f, m11 = 0.0f;
+#pragma unroll
+        for (int z = 0; z < kSub; ++z) {
+          if (z < b.n0) {
+            const float u0 = (float)(b.o0 + b.s0 + z) - g.c0;
+            const float A = fmaf(fmaf(g.a00, u0, g.k1), u0, g.k0);
+            const float B = fmaf(g.b01, u0, g.k2);
+            float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+#pragma unroll
+            for (int y = 0; y < kSub; ++y) {
+              if (y < b.n1) {
+                const float u1 = g.u1[y];
+                const float q = fmaf(fmaf(g.a11, u1, B), u1, A);
+                const bool ok = g_ok && (q >= kThr);
+                const float w = ok ? __builtin_amdgcn_exp2f(q) : 0.0f;
+                const float vr = gr[z * kSub + y], vi = gi[z * kSub + y];
+                dr = fmaf(w, vr, dr);
+                di = fmaf(w, vi, di);
+                const float wt = w * fmaf(g.ri, vi, g.rr * vr);
+                t0 += wt;
+                const float sy = wt * u1;
+                t1 += sy;
+                t2 = fmaf(sy, u1, t2);
+              }
+            }
+            m += t0;
+            const float tz = t0 * u0;
+            m0 += tz;
+            m00 = fmaf(tz, u0, m00);
+            m1 += t1;
+            m01 = fmaf(t1, u0, m01);
+            m11 += t2;
+          }
+        }
+        // Moments of w*t over the column: sum(w t u_a), sum(w t u_a u_b).
+        float v[kMoments];
+        v[0] = dr;
+        v[1] = di;
+        v[2] = m0;
+        v[3] = m1;
+        v[4] = g.u2 * m;
+        v[5] = m00;
+        v[6] = m11;
+        v[7] = g.u2 * g.u2 * m;
+        v[8] = m01;
+        v[9] = g.u2 * m0;
+        v[10] = g.u2 * m1;
+#pragma unroll
+        for (int f = 0; f < kMoments; ++f) {
+          v[f] += __shfl_xor(v[f], 16, 64);
+          v[f] += __shfl_xor(v[f], 32, 64);
+        }
+        if (xi == 0 && slot_ok) {
+#pragma unroll
+          for (int f = 0; f < kMoments; ++f) atomicAdd(&sh_acc[f][k], v[f]);
+        }
+      }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) {
+      float v[kMoments];
+      bool any = false;
+#pragma unroll
+      for (int f = 0; f < kMoments; ++f) {
+        v[f] = sh_acc[f][threadIdx.x];
+        any |= (v[f] != 0.0f);
+      }
+      if (any) {
+        float* dst = gacc + (int64_t)pair_ids[bs + threadIdx.x] * kAccStride;
+#pragma unroll
+        for (int f = 0; f < kMoments; ++f) unsafeAtomicAdd(dst + f, v[f]);
+      }
+    }
+  }
+}
+
+// Moments -> reference outputs (gaussian/cuda_ext/tile_voxelize_cuda_kernel.cu:237-252):
+//   s = -w t / 2;  grad_rho = sum w g;  grad_c_i = -2/hs_i sum_j M_ij sum(s u_j)/hs_j;
+//   grad_M_ij = sum(s u_i u_j) / (hs_i hs_j).
+__global__ __launch_bounds__(256) void k_finalize(const float* __restrict__ gacc, const float* __restrict__ cov,
+                                                  const float* __restrict__ half_shape, int n,
+                                                  float* __restrict__ g_center, float* __restrict__ g_cov,
+                                                  float* __restrict__ g_rr, float* __restrict__ g_ri) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= n) return;
+  const float* a = gacc + (int64_t)g * kAccStride;
+  const float hs[3] = {half_shape[0], half_shape[1], half_shape[2]};
+  g_rr[g] = a[0];
+  g_ri[g] = a[1];
+  const float m[3] = {-0.5f * a[2], -0.5f * a[3], -0.5f * a[4]};
+  const float mm[3][3] = {{-0.5f * a[5], -0.5f * a[8], -0.5f * a[9]},
+                          {-0.5f * a[8], -0.5f * a[6], -0.5f * a[10]},
+                          {-0.5f * a[9], -0.5f * a[10], -0.5f * a[7]}};
+  const float* M = cov + 9 * (int64_t)g;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) acc = fmaf(M[3 * i + j], m[j] / hs[j], acc);
+    g_center[3 * (int64_t)g + i] = -2.0f * acc / hs[i];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) g_cov[9 * (int64_t)g + 3 * i + j] = mm[i][j] / (hs[i] * hs[j]);
+  }
+}
+
+// Exact evaluation counts for the roofline accounting: the forward kernel's
+// loop structure with the accumulation replaced by counting.
+__global__ __launch_bounds__(kTileThreads) void k_count(const GRec* __restrict__ rec,
+                                                        const int32_t* __restrict__ ranges,
+                                                        const int32_t* __restrict__ pair_ids, VoxGeom geo,
+                                                        unsigned long long* __restrict__ counts) {
+  __shared__ Stage sh;
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  if (logical >= geo.blocks) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int gl = lane & 15, xi = lane >> 4;
+  const BlockCtx b = make_ctx(logical, geo, ranges, wave);
+  const bool x_ok = xi < b.n2;
+  if (threadIdx.x == 0 && logical % geo.chunks == 0) {
+    const unsigned long long pairs = (unsigned long long)(b.p_end - b.p_begin);
+    atomicAdd(&counts[0], pairs * (unsigned long long)(b.e0 * b.e1 * b.e2));
+  }
+  unsigned int active = 0, culled = 0;
+  for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
+    const int cnt = min(kBatch, b.p_end - bs);
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) stage_pair(sh, threadIdx.x, rec[pair_ids[bs + threadIdx.x]], b);
+    __syncthreads();
+    if (!b.active) continue;
+    const int nhit = build_list(sh, cnt, b, wave, lane);
+    for (int gb = 0; gb < nhit; gb += 16) {
+      const int idx = gb + gl;
+      const bool g_ok = (idx < nhit) && x_ok;
+      const int k = sh.list[wave][idx < nhit ? idx : gb];
+      const GaussLane g = load_gauss(sh, k, b, xi);
+#pragma unroll
+      for (int z = 0; z < kSub; ++z) {
+        if (z < b.n0) {
+          const float u0 = (float)(b.o0 + b.s0 + z) - g.c0;
+          const float A = fmaf(fmaf(g.a00, u0, g.k1), u0, g.k0);
+          const float B = fmaf(g.b01, u0, g.k2);
+#pragma unroll
+          for (int y = 0; y < kSub; ++y) {
+            if (y < b.n1) {
+              const float q = fmaf(fmaf(g.a11, g.u1[y], B), g.u1[y], A);
+              active += (g_ok && (q >= kThr)) ? 1u : 0u;
+              culled += g_ok ? 1u : 0u;
+            }
+          }
+        }
+      }
+    }
+  }
+  unsigned long long a = active, c = culled;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    c += __shfl_xor(c, off, 64);
+  }
+  if (lane == 0) {
+    atomicAdd(&counts[1], a);
+    atomicAdd(&counts[2], c);
+  }
+}
+
+// ------------------------------------------------------------- host helpers --
+
+bool tile_geometry(int32_t d, int32_t h, int32_t w, int32_t tile, int nt[3], int64_t* num_tiles) {
+  if (d <= 0 || h <= 0 || w <= 0 || tile <= 0 || tile > 32) return false;
+  nt[0] = ceil_div_i(d, tile);
+  nt[1] = ceil_div_i(h, tile);
+  nt[2] = ceil_div_i(w, tile);
+  *num_tiles = (int64_t)nt[0] * nt[1] * nt[2];
+  return *num_tiles < (int64_t)INT_MAX;
+}
+
+int radix_passes(int num_tiles) {
+  int bits = 0;
+  while ((1ll << bits) < num_tiles) ++bits;
+  return (bits + kRadixBits - 1) / kRadixBits;
+}
+
+int key_bits(int num_tiles) {
+  int bits = 0;
+  while ((1ll << bits) < num_tiles) ++bits;
+  return bits;
+}
+
+VoxGeom make_vox_geom(int d, int h, int w, int tile, const int nt[3], int num_tiles) {
+  VoxGeom g;
+  g.D = d;
+  g.H = h;
+  g.W = w;
+  g.tile = tile;
+  g.nTH = nt[1];
+  g.nTW = nt[2];
+  g.num_tiles = num_tiles;
+  g.nsb = ceil_div_i(tile, kSub);
+  g.chunks = ceil_div_i(g.nsb * g.nsb * g.nsb, kWaves);
+  g.blocks = num_tiles * g.chunks;
+  return g;
+}
+
+size_t fill_ws_layout(int64_t total, int num_tiles, Carve& cv, int32_t** ka, int32_t** va, int32_t** kb,
+                      int32_t** vb, int32_t** hist, int64_t** sums) {
+  const int64_t nblk = ceil_div_l(total, kRadixTile);
+  const int64_t hist_n = (int64_t)(1 << kRadixBits) * nblk;
+  *ka = cv.take<int32_t>((size_t)total);
+  *va = cv.take<int32_t>((size_t)total);
+  *kb = cv.take<int32_t>((size_t)total);
+  *vb = cv.take<int32_t>((size_t)total);
+  *hist = cv.take<int32_t>((size_t)hist_n);
+  *sums = cv.take<int64_t>((size_t)ceil_div_l(hist_n, kScanTile) + 1);
+  (void)num_tiles;
+  return align_up(cv.off, 256);
+}
+
+// Opt-in kernel timing (gstv_profile_begin / gstv_profile_end).
+constexpr int kProfMax = 1024;
+struct Profile {
+  bool on = false;
+  hipEvent_t start[2][kProfMax] = {};
+  hipEvent_t stop[2][kProfMax] = {};
+  int used[2] = {0, 0};
+};
+Profile g_prof;
+
+// Records the start event of a kernel launch of `kind` (0 forward, 1 backward);
+// returns the slot to close with prof_stop, or -1 when not recording.
+int prof_start(int kind, hipStream_t s) {
+  if (!g_prof.on || g_prof.used[kind] >= kProfMax) return -1;
+  const int i = g_prof.used[kind];
+  if (g_prof.start[kind][i] == nullptr) {
+    if (hipEventCreate(&g_prof.start[kind][i]) != hipSuccess) return -1;
+    if (hipEventCreate(&g_prof.stop[kind][i]) != hipSuccess) return -1;
+  }
+  if (hipEventRecord(g_prof.start[kind][i], s) != hipSuccess) return -1;
+  return i;
+}
+
+void prof_stop(int kind, int slot, hipStream_t s) {
+  if (slot < 0) return;
+  if (hipEventRecord(g_prof.stop[kind][slot], s) == hipSuccess) g_prof.used[kind] = slot + 1;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+const char* gstv_status_string(int status) {
+  switch (status) {
+    case GSTV_OK: return "success";
+    case GSTV_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case GSTV_ERR_WORKSPACE_TOO_SMALL: return "workspace too small";
+    case GSTV_ERR_TOO_MANY_PAIRS: return "too many (tile, Gaussian) pairs: total must be < 2^31";
+    case GSTV_ERR_HIP: return g_last_detail[0] ? g_last_detail : "HIP runtime error";
+    default: return "unknown status";
+  }
+}
+
+int gstv_version(void) { return 100; }
+
+size_t gstv_bin_count_workspace_bytes(int32_t n) {
+  if (n < 0) return 0;
+  Carve cv(nullptr);
+  cv.take<int64_t>((size_t)ceil_div_l((int64_t)n + 1, kScanTile) + 1);
+  return align_up(cv.off, 256);
+}
+
+int gstv_bin_count(const float* center_vox, const float* radius_vox, int32_t n, int32_t d, int32_t h,
+                   int32_t w, int32_t tile, int32_t* gauss_offsets, int64_t* total_pairs, void* workspace,
+                   size_t workspace_bytes, void* stream) {
+  BinGeom geo;
+  int64_t num_tiles;
+  if (n < 0 || gauss_offsets == nullptr || total_pairs == nullptr ||
+      !tile_geometry(d, h, w, tile, geo.nt, &num_tiles))
+    return GSTV_ERR_INVALID_ARGUMENT;
+  if (n > 0 && (center_vox == nullptr || radius_vox == nullptr)) return GSTV_ERR_INVALID_ARGUMENT;
+  if (workspace_bytes < gstv_bin_count_workspace_bytes(n) || workspace == nullptr)
+    return GSTV_ERR_WORKSPACE_TOO_SMALL;
+  geo.size[0] = d;
+  geo.size[1] = h;
+  geo.size[2] = w;
+  geo.tile = tile;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Carve cv(workspace);
+  int64_t* sums = cv.take<int64_t>((size_t)ceil_div_l((int64_t)n + 1, kScanTile) + 1);
+  // counts[n] = 0 so the scan of n+1 entries leaves the total in offsets[n].
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, gauss_offsets + n, (int64_t)1, 0);
+  GSTV_LAUNCH_CHECK("bin_count_fill");
+  if (n > 0) {
+    hipLaunchKernelGGL(k_bin_count, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, center_vox, radius_vox, n, geo,
+                       gauss_offsets);
+    GSTV_LAUNCH_CHECK("bin_count");
+  }
+  return launch_exclusive_scan(gauss_offsets, (int64_t)n + 1, gauss_offsets, sums, total_pairs, s);
+}
+
+size_t gstv_bin_fill_workspace_bytes(int32_t n, int64_t total_pairs, int32_t num_tiles) {
+  (void)n;
+  if (total_pairs < 0) return 0;
+  Carve cv(nullptr);
+  int32_t *ka, *va, *kb, *vb, *hist;
+  int64_t* sums;
+  return fill_ws_layout(total_pairs, num_tiles, cv, &ka, &va, &kb, &vb, &hist, &sums);
+}
+
+int gstv_bin_fill(const float* center_vox, const float* radius_vox, int32_t n, int32_t d, int32_t h,
+                  int32_t w, int32_t tile, const int32_t* gauss_offsets, int64_t total_pairs,
+                  int32_t* tile_ranges, int32_t* pair_ids, void* workspace, size_t workspace_bytes,
+                  void* stream) {
+  BinGeom geo;
+  int64_t num_tiles64;
+  if (n < 0 || total_pairs < 0 || tile_ranges == nullptr || !tile_geometry(d, h, w, tile, geo.nt, &num_tiles64))
+    return GSTV_ERR_INVALID_ARGUMENT;
+  if (total_pairs >= (int64_t)INT_MAX) return GSTV_ERR_TOO_MANY_PAIRS;
+  const int num_tiles = (int)num_tiles64;
+  geo.size[0] = d;
+  geo.size[1] = h;
+  geo.size[2] = w;
+  geo.tile = tile;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_fill_i32, dim3(ceil_div_i(2 * num_tiles, 256)), dim3(256), 0, s, tile_ranges,
+                     (int64_t)2 * num_tiles, 0);
+  GSTV_LAUNCH_CHECK("ranges_clear");
+  const int total = (int)total_pairs;
+  if (total == 0) return GSTV_OK;
+  if (n <= 0 || center_vox == nullptr || radius_vox == nullptr || gauss_offsets == nullptr ||
+      pair_ids == nullptr)
+    return GSTV_ERR_INVALID_ARGUMENT;
+  if (workspace == nullptr || workspace_bytes < gstv_bin_fill_workspace_bytes(n, total_pairs, num_tiles))
+    return GSTV_ERR_WORKSPACE_TOO_SMALL;
+  Carve cv(workspace);
+  int32_t *ka, *va, *kb, *vb, *hist;
+  int64_t* sums;
+  fill_ws_layout(total_pairs, num_tiles, cv, &ka, &va, &kb, &vb, &hist, &sums);
+
+  const int passes = radix_passes(num_tiles);
+  const int bits_total = key_bits(num_tiles);
+  // With no radix pass (a single tile) the expansion order is already final.
+  int32_t* first_vals = passes == 0 ? pair_ids : va;
+  hipLaunchKernelGGL(k_bin_expand, dim3(ceil_div_i(total, kExpandTile)), dim3(kExpandThreads), 0, s, center_vox,
+                     radius_vox, n, geo, gauss_offsets, total, ka, first_vals);
+  GSTV_LAUNCH_CHECK("bin_expand");
+
+  const int nblk = ceil_div_i(total, kRadixTile);
+  int32_t* kin = ka;
+  int32_t* vin = va;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * kRadixBits;
+    const int bits = std::min(kRadixBits, bits_total - shift);
+    int32_t* kout = (kin == ka) ? kb : ka;
+    int32_t* vout = (p == passes - 1) ? pair_ids : ((vin == va) ? vb : va);
+    hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kRadixThreads), 0, s, kin, total, shift, bits, nblk, hist);
+    GSTV_LAUNCH_CHECK("radix_hist");
+    const int64_t hist_n = (int64_t)(1 << bits) * nblk;
+    int st = launch_exclusive_scan(hist, hist_n, hist, sums, nullptr, s);
+    if (st != GSTV_OK) return st;
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kRadixThreads), 0, s, kin, vin, total, shift, bits, nblk,
+                       hist, kout, vout);
+    GSTV_LAUNCH_CHECK("radix_scatter");
+    kin = kout;
+    vin = vout;
+  }
+  hipLaunchKernelGGL(k_ranges, dim3(ceil_div_i(total, 256)), dim3(256), 0, s, kin, total, tile_ranges);
+  GSTV_LAUNCH_CHECK("ranges");
+  return GSTV_OK;
+}
+
+size_t gstv_forward_workspace_bytes(int32_t n) {
+  if (n < 0) return 0;
+  return align_up((size_t)n * sizeof(GRec), 256);
+}
+
+namespace {
+int check_tile_args(const int32_t* tile_ranges, int32_t num_tiles, int32_t n, int32_t tile, int32_t d, int32_t h,
+                    int32_t w, int nt[3]) {
+  int64_t expect;
+  if (n < 0 || !tile_geometry(d, h, w, tile, nt, &expect)) return GSTV_ERR_INVALID_ARGUMENT;
+  if ((int64_t)num_tiles != expect || tile_ranges == nullptr) return GSTV_ERR_INVALID_ARGUMENT;
+  return GSTV_OK;
+}
+
+int launch_prep(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
+                const float* half_shape, int32_t n, GRec* rec, hipStream_t s) {
+  if (n == 0) return GSTV_OK;
+  hipLaunchKernelGGL(k_prep, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, center_vox, cov_inv, rho_r, rho_i,
+                     half_shape, n, rec);
+  GSTV_LAUNCH_CHECK("prep");
+  return GSTV_OK;
+}
+
+int tile_grid(const VoxGeom& g) { return ((g.blocks + 7) / 8) * 8; }
+}  // namespace
+
+int gstv_forward(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
+                 const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
+                 int32_t n, int32_t tile, int32_t d, int32_t h, int32_t w, float* vol_r, float* vol_i,
+                 void* workspace, size_t workspace_bytes, void* stream) {
+  int nt[3];
+  int st = check_tile_args(tile_ranges, num_tiles, n, tile, d, h, w, nt);
+  if (st != GSTV_OK) return st;
+  if (vol_r == nullptr || vol_i == nullptr || half_shape == nullptr) return GSTV_ERR_INVALID_ARGUMENT;
+  if (n > 0 && (center_vox == nullptr || cov_inv == nullptr || rho_r == nullptr || rho_i == nullptr ||
+                pair_ids == nullptr))
+    return GSTV_ERR_INVALID_ARGUMENT;
+  if (n > 0 && (workspace == nullptr || workspace_bytes < gstv_forward_workspace_bytes(n)))
+    return GSTV_ERR_WORKSPACE_TOO_SMALL;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  GRec* rec = static_cast<GRec*>(workspace);
+  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, half_shape, n, rec, s);
+  if (st != GSTV_OK) return st;
+  const VoxGeom geo = make_vox_geom(d, h, w, tile, nt, num_tiles);
+  const int slot = prof_start(0, s);
+  hipLaunchKernelGGL(k_forward, dim3(tile_grid(geo)), dim3(kTileThreads), 0, s, (const GRec*)rec, tile_ranges,
+                     pair_ids, geo, vol_r, vol_i);
+  GSTV_LAUNCH_CHECK("forward");
+  prof_stop(0, slot, s);
+  return GSTV_OK;
+}
+
+size_t gstv_backward_workspace_bytes(int32_t n) {
+  if (n < 0) return 0;
+  Carve cv(nullptr);
+  cv.take<GRec>((size_t)n);
+  cv.take<float>((size_t)n * kAccStride);
+  return align_up(cv.off, 256);
+}
+
+int gstv_backward(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
+                  const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
+                  int32_t n, const float* grad_r, const float* grad_i, int32_t tile, int32_t d, int32_t h,
+                  int32_t w, float* grad_center, float* grad_cov_inv, float* grad_rho_r, float* grad_rho_i,
+                  void* workspace, size_t workspace_bytes, void* stream) {
+  int nt[3];
+  int st = check_tile_args(tile_ranges, num_tiles, n, tile, d, h, w, nt);
+  if (st != GSTV_OK) return st;
+  if (n == 0) return GSTV_OK;
+  if (center_vox == nullptr || cov_inv == nullptr || rho_r == nullptr || rho_i == nullptr || pair_ids == nullptr ||
+      half_shape == nullptr || grad_r == nullptr || grad_i == nullptr || grad_center == nullptr ||
+      grad_cov_inv == nullptr || grad_rho_r == nullptr || grad_rho_i == nullptr)
+    return GSTV_ERR_INVALID_ARGUMENT;
+  if (workspace == nullptr || workspace_bytes < gstv_backward_workspace_bytes(n))
+    return GSTV_ERR_WORKSPACE_TOO_SMALL;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Carve cv(workspace);
+  GRec* rec = cv.take<GRec>((size_t)n);
+  float* gacc = cv.take<float>((size_t)n * kAccStride);
+  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, half_shape, n, rec, s);
+  if (st != GSTV_OK) return st;
+  const int64_t nacc = (int64_t)n * kAccStride;
+  hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)ceil_div_l(nacc, 256)), dim3(256), 0, s, gacc, nacc, 0.0f);
+  GSTV_LAUNCH_CHECK("grad_clear");
+  const VoxGeom geo = make_vox_geom(d, h, w, tile, nt, num_tiles);
+  const int slot = prof_start(1, s);
+  hipLaunchKernelGGL(k_backward, dim3(tile_grid(geo)), dim3(kTileThreads), 0, s, (const GRec*)rec, tile_ranges,
+                     pair_ids, geo, grad_r, grad_i, gacc);
+  GSTV_LAUNCH_CHECK("backward");
+  prof_stop(1, slot, s);
+  hipLaunchKernelGGL(k_finalize, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, (const float*)gacc, cov_inv,
+                     half_shape, n, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i);
+  GSTV_LAUNCH_CHECK("finalize");
+  return GSTV_OK;
+}
+
+int gstv_count_evaluations(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
+                           const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
+                           const float* half_shape, int32_t n, int32_t tile, int32_t d, int32_t h, int32_t w,
+                           int64_t* counts, void* workspace, size_t workspace_bytes, void* stream) {
+  int nt[3];
+  int st = check_tile_args(tile_ranges, num_tiles, n, tile, d, h, w, nt);
+  if (st != GSTV_OK) return st;
+  if (counts == nullptr || half_shape == nullptr) return GSTV_ERR_INVALID_ARGUMENT;
+  if (n > 0 && (center_vox == nullptr || cov_inv == nullptr || rho_r == nullptr || rho_i == nullptr ||
+                pair_ids == nullptr))
+    return GSTV_ERR_INVALID_ARGUMENT;
+  if (n > 0 && (workspace == nullptr || workspace_bytes < gstv_forward_workspace_bytes(n)))
+    return GSTV_ERR_WORKSPACE_TOO_SMALL;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, reinterpret_cast<int32_t*>(counts), (int64_t)6, 0);
+  GSTV_LAUNCH_CHECK("count_clear");
+  GRec* rec = static_cast<GRec*>(workspace);
+  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, half_shape, n, rec, s);
+  if (st != GSTV_OK) return st;
+  const VoxGeom geo = make_vox_geom(d, h, w, tile, nt, num_tiles);
+  hipLaunchKernelGGL(k_count, dim3(tile_grid(geo)), dim3(kTileThreads), 0, s, (const GRec*)rec, tile_ranges, pair_ids,
+                     geo, reinterpret_cast<unsigned long long*>(counts));
+  GSTV_LAUNCH_CHECK("count");
+  return GSTV_OK;
+}
+
+int gstv_profile_begin(void) {
+  g_prof.on = true;
+  g_prof.used[0] = g_prof.used[1] = 0;
+  return GSTV_OK;
+}
+
+int gstv_profile_end(double* forward_ms, int32_t* forward_launches, double* backward_ms,
+                     int32_t* backward_launches) {
+  double total[2] = {0.0, 0.0};
+  int status = GSTV_OK;
+  for (int kind = 0; kind < 2; ++kind) {
+    for (int i = 0; i < g_prof.used[kind]; ++i) {
+      float ms = 0.0f;
+      int st = hip_status(hipEventSynchronize(g_prof.stop[kind][i]), "profile_sync");
+      if (st == GSTV_OK) st = hip_status(hipEventElapsedTime(&ms, g_prof.start[kind][i], g_prof.stop[kind][i]),
+                                         "profile_elapsed");
+      if (st != GSTV_OK) status = st;
+      total[kind] += ms;
+    }
+  }
+  if (forward_ms) *forward_ms = total[0];
+  if (forward_launches) *forward_launches = g_prof.used[0];
+  if (backward_ms) *backward_ms = total[1];
+  if (backward_launches) *backward_launches = g_prof.used[1];
+  g_prof.on = false;
+  g_prof.used[0] = g_prof.used[1] = 0;
+  return status;
+}
+
+}  // extern "C"

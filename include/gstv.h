@@ -1,0 +1,175 @@
+/*
+ * gstv.h - C ABI of the MI355X (gfx950) tile voxelizer.
+ *
+ * The library renders N anisotropic complex 3D Gaussians into a planar complex
+ * volume through 3D voxel tiles, and computes the adjoint (backward) of that
+ * rendering.  It replaces the pybind11 extension and the torch-side binning of
+ * 3DGSMR's tile voxelizer:
+ *
+ *   gaussian/tile_voxelizer.py:87-162          build_tile_pairs (cull, expand, sort, ranges)
+ *   gaussian/cuda_ext/tile_voxelize_cuda.cpp:33-36   module forward / backward
+ *   gaussian/cuda_ext/tile_voxelize_cuda_kernel.cu:262-349  host wrappers
+ *
+ * Conventions shared by every entry point
+ *   - All pointers are device pointers (hipMalloc / torch CUDA tensors), except
+ *     where a comment says "host".
+ *   - The caller allocates every buffer, including workspaces; the library never
+ *     allocates device memory.  Query workspace sizes with the *_workspace_bytes
+ *     functions; pass at least that many bytes, aligned to 256 bytes.
+ *   - Work is enqueued on `stream` (a hipStream_t, may be NULL for the null
+ *     stream) and returns without synchronising.  The device is the caller's
+ *     current HIP device.
+ *   - The return value is GSTV_OK (0) or a status code; gstv_status_string()
+ *     gives the message.  Kernel launch errors are reported as GSTV_ERR_HIP.
+ *   - Volumes are row-major [D][H][W]; flat index = d*H*W + h*W + w.
+ *   - Tiles are cubes of `tile` voxels; nT* = ceil(size / tile); tile id =
+ *     td*(nTH*nTW) + th*nTW + tw.  `tile` must be in [1, 32].
+ *   - Gaussian parameters are in voxel space: center_vox [N][3] (axis order
+ *     D, H, W), cov_inv [N][3][3] row-major (the inverse covariance in units
+ *     normalised by half_shape), density planes rho_r [N], rho_i [N].
+ *   - half_shape is a device array of 3 floats (D/2, H/2, W/2 for the reference
+ *     caller); the kernels read it on the device as the reference kernel does.
+ */
+#ifndef GSTV_H
+#define GSTV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSTV_OK 0
+#define GSTV_ERR_INVALID_ARGUMENT 1
+#define GSTV_ERR_WORKSPACE_TOO_SMALL 2
+#define GSTV_ERR_TOO_MANY_PAIRS 3
+#define GSTV_ERR_HIP 4
+
+/* Human-readable message for a status code (static storage, never NULL). */
+const char* gstv_status_string(int status);
+
+/* Library ABI version (major * 100 + minor). */
+int gstv_version(void);
+
+/* ---------------------------------------------------------------- binning --
+ * Two-phase binning, as in gaussian/tile_voxelizer.py:87-162: phase one counts
+ * the tiles of every Gaussian's 3-sigma box and scans the counts; the caller
+ * reads the total (one device-to-host copy, the same sync the reference makes
+ * at gaussian/tile_voxelizer.py:118), allocates pair_ids, and calls phase two.
+ *
+ * Box rule (bit-exact with gaussian/tile_voxelizer.py:97-117): per axis
+ *   lo = int(floor(c - r)), hi = int(ceil(c + r)), clamped to [0, size-1];
+ *   tile lo/hi = lo / tile, hi / tile, clamped to [0, nT-1].
+ * float->int conversion truncates with saturation and maps NaN to 0.
+ */
+
+/* Workspace for gstv_bin_count. */
+size_t gstv_bin_count_workspace_bytes(int32_t n);
+
+/* Phase one.  Replaces gaussian/tile_voxelizer.py:97-118.
+ *   center_vox [n][3], radius_vox [n]           fp32 inputs
+ *   gauss_offsets [n+1] int32 out: exclusive scan of per-Gaussian tile counts,
+ *                       gauss_offsets[n] = total (valid only if total < 2^31)
+ *   total_pairs [1] int64 out (device): the total number of (tile, Gaussian) pairs.
+ */
+int gstv_bin_count(const float* center_vox, const float* radius_vox, int32_t n,
+                   int32_t d, int32_t h, int32_t w, int32_t tile,
+                   int32_t* gauss_offsets, int64_t* total_pairs,
+                   void* workspace, size_t workspace_bytes, void* stream);
+
+/* Workspace for gstv_bin_fill. */
+size_t gstv_bin_fill_workspace_bytes(int32_t n, int64_t total_pairs, int32_t num_tiles);
+
+/* Phase two.  Replaces gaussian/tile_voxelizer.py:120-162.
+ *   total_pairs   host value read back from phase one; must be < 2^31
+ *                 (GSTV_ERR_TOO_MANY_PAIRS otherwise, as pair offsets are int32
+ *                 in the reference, gaussian/tile_voxelizer.py:160-161).
+ *   tile_ranges [num_tiles][2] int32 out: [start, end) into pair_ids, (0, 0) for
+ *                 tiles without pairs.
+ *   pair_ids [total_pairs] int32 out: Gaussian ids sorted by tile id; within a
+ *                 tile in ascending Gaussian id (a stable sort of the g-major
+ *                 pair list, the canonical order of the reference's argsort).
+ */
+int gstv_bin_fill(const float* center_vox, const float* radius_vox, int32_t n,
+                  int32_t d, int32_t h, int32_t w, int32_t tile,
+                  const int32_t* gauss_offsets, int64_t total_pairs,
+                  int32_t* tile_ranges, int32_t* pair_ids,
+                  void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------- forward --
+ * Replaces tile_voxelize_forward_cuda (gaussian/cuda_ext/tile_voxelize_cuda_kernel.cu:262-300)
+ * and its kernel (:48-143):
+ *   vol[x] = sum over g in tile(x) of rho_g * exp(-q/2) * [q <= 9],
+ *   q = d^T M_g d,  d = (x - c_g) / half_shape.
+ * Every voxel of vol_r / vol_i [d*h*w] is written (zero where no Gaussian
+ * contributes), so the outputs need no prior clearing.  Deterministic.
+ *   n           number of Gaussians (rows of center_vox / cov_inv / rho)
+ *   num_tiles   rows of tile_ranges; must equal nTD*nTH*nTW for (d, h, w, tile)
+ */
+size_t gstv_forward_workspace_bytes(int32_t n);
+
+int gstv_forward(const float* center_vox, const float* cov_inv,
+                 const float* rho_r, const float* rho_i,
+                 const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
+                 const float* half_shape, int32_t n,
+                 int32_t tile, int32_t d, int32_t h, int32_t w,
+                 float* vol_r, float* vol_i,
+                 void* workspace, size_t workspace_bytes, void* stream);
+
+/* --------------------------------------------------------------- backward --
+ * Replaces tile_voxelize_backward_cuda (gaussian/cuda_ext/tile_voxelize_cuda_kernel.cu:302-349)
+ * and its kernel (:145-258).  Given the upstream gradient planes grad_r,
+ * grad_i [d*h*w], writes (overwriting, no prior clearing needed):
+ *   grad_center [n][3], grad_cov_inv [n][3][3], grad_rho_r [n], grad_rho_i [n]
+ * with the reference's formulas: w = exp(-q/2), s = -w/2 (rho_r g_r + rho_i g_i),
+ *   grad_rho += w g;  grad_center_i += s * (-2 (M d)_i / hs_i);  grad_M_ij += s d_i d_j.
+ * Exponents are recomputed, not stored.  Summation order across tiles is not
+ * fixed (fp32 atomics), as in the reference.
+ */
+size_t gstv_backward_workspace_bytes(int32_t n);
+
+int gstv_backward(const float* center_vox, const float* cov_inv,
+                  const float* rho_r, const float* rho_i,
+                  const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
+                  const float* half_shape, int32_t n,
+                  const float* grad_r, const float* grad_i,
+                  int32_t tile, int32_t d, int32_t h, int32_t w,
+                  float* grad_center, float* grad_cov_inv,
+                  float* grad_rho_r, float* grad_rho_i,
+                  void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------ measurement --
+ * No counterpart in the reference, which has no profiling hooks on this
+ * path; these serve the benchmark's roofline accounting.
+ */
+
+/* Exact evaluation counts of the tile kernels for the given binning:
+ *   counts[0] = E_total   = sum over tiles of pairs(t) * voxels(t)
+ *   counts[1] = E_active  = evaluations with q <= 9 (the kernel's own test)
+ *   counts[2] = E_culled  = evaluations the kernels actually perform after
+ *               sub-brick culling (16 Gaussian slots x 64 voxels per group)
+ * counts is an int64 [3] device array (overwritten).  Arguments otherwise as
+ * for gstv_forward (same workspace size). */
+int gstv_count_evaluations(const float* center_vox, const float* cov_inv,
+                           const float* rho_r, const float* rho_i,
+                           const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
+                           const float* half_shape, int32_t n,
+                           int32_t tile, int32_t d, int32_t h, int32_t w,
+                           int64_t* counts, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Kernel timing with HIP events.  After gstv_profile_begin(), every launch of
+ * the forward and backward tile kernels is bracketed by events recorded on the
+ * launch stream (at most 1024 launches each).  gstv_profile_end() waits for
+ * the recorded events, returns the summed milliseconds and launch counts of
+ * each kernel (any pointer may be NULL), and stops recording.  Host-side
+ * state is process-wide and not thread-safe; use from one thread. */
+int gstv_profile_begin(void);
+int gstv_profile_end(double* forward_ms, int32_t* forward_launches,
+                     double* backward_ms, int32_t* backward_launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSTV_H */

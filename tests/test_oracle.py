@@ -1,0 +1,311 @@
+"""Pins of the CPU oracle (CPU tests).  Each check is independent of the
+oracle's own formula: brute force, closed forms, invariants, finite differences."""
+
+from __future__ import annotations
+
+import itertools
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import torch64
+from oracle import voxelizer_oracle as vo
+from tests import cases
+
+F32 = np.float32
+
+
+# ---------------------------------------------------------------- prologue --
+
+def test_prologue_center_radius_bit_exact_with_torch():
+    """The oracle's fp32 centre/radius equal torch's op-by-op fp32 evaluation."""
+    c = cases.microbench_case(n=20000, shape=(256, 256, 128), seed=3)
+    shape = c["shape"]
+    p = torch.from_numpy(c["pos"])
+    s = torch.from_numpy(c["scales"])
+    st = torch.tensor(shape, dtype=torch.float32)
+    center = (p + 1.0) * 0.5 * st - 0.5
+    radius = (s * (st * 0.5)).max(dim=-1)[0] * 3.0
+    c32, r32, _, _ = vo.prologue(c["pos"], c["scales"], c["rot"], shape)
+    np.testing.assert_array_equal(center.numpy(), c32)
+    np.testing.assert_array_equal(radius.numpy(), r32)
+
+
+def test_cov_inv_closed_form_and_quaternion_invariance():
+    rng = np.random.default_rng(0)
+    q = rng.normal(size=(50, 4))
+    s = np.exp(rng.normal(size=(50, 3)))
+    r = vo.quaternion_to_rotation(q)
+    # Orthonormal with det +1 (to the 1e-8 normalisation epsilon).
+    np.testing.assert_allclose(r @ np.swapaxes(r, 1, 2), np.broadcast_to(np.eye(3), (50, 3, 3)), atol=1e-7)
+    np.testing.assert_allclose(np.linalg.det(r), 1.0, atol=1e-7)
+    # q and -q, q and 3q describe the same rotation.
+    np.testing.assert_allclose(vo.quaternion_to_rotation(-q), r, atol=1e-12)
+    np.testing.assert_allclose(vo.quaternion_to_rotation(3.0 * q), r, atol=1e-7)
+    # Identity quaternion.
+    np.testing.assert_allclose(vo.quaternion_to_rotation(np.array([[1.0, 0, 0, 0]])), np.eye(3)[None], atol=1e-7)
+    # 90 degrees about the W axis: (cos 45, 0, 0, sin 45) maps axis 0 to axis 1.
+    r90 = vo.quaternion_to_rotation(np.array([[math.cos(math.pi / 4), 0, 0, math.sin(math.pi / 4)]]))[0]
+    np.testing.assert_allclose(r90 @ np.array([1.0, 0, 0]), [0, 1, 0], atol=1e-7)
+    _, _, cov, _ = vo.prologue(np.zeros((50, 3), F32), s.astype(F32), q.astype(F32), (16, 16, 16))
+    sf = s.astype(F32).astype(np.float64)
+    expect = vo.quaternion_to_rotation(q.astype(F32)) @ (np.eye(3)[None] / (sf[:, :, None] + 1e-8) ** 2) @ \
+        np.swapaxes(vo.quaternion_to_rotation(q.astype(F32)), 1, 2)
+    np.testing.assert_allclose(cov, expect, rtol=1e-9)
+
+
+# ----------------------------------------------------------------- binning --
+
+def _brute_force_tiles(center, radius, shape, tile):
+    """Tiles hit by every voxel of each Gaussian's clamped box, via v // tile."""
+    nt = vo.tile_grid(shape, tile)
+    out = []
+    for c, r in zip(center, radius):
+        lo = []
+        hi = []
+        for a in range(3):
+            x = np.float32(c[a]) - np.float32(r)
+            y = np.float32(c[a]) + np.float32(r)
+            lo_a = int(vo._sat_f2i(np.floor(np.float32(x))))
+            hi_a = int(vo._sat_f2i(np.ceil(np.float32(y))))
+            lo.append(min(max(lo_a, 0), shape[a] - 1))
+            hi.append(min(max(hi_a, 0), shape[a] - 1))
+        tiles = set()
+        if all(lo[a] <= hi[a] for a in range(3)):
+            for v in itertools.product(*[range(lo[a], hi[a] + 1) for a in range(3)]):
+                t = (v[0] // tile, v[1] // tile, v[2] // tile)
+                tiles.add(t[0] * nt[1] * nt[2] + t[1] * nt[2] + t[2])
+        out.append(tiles)
+    return out
+
+
+@pytest.mark.parametrize("tile", [1, 3, 4, 5, 8, 16])
+def test_binning_brute_force(tile):
+    rng = np.random.default_rng(tile)
+    shape = (13, 7, 19)
+    n = 60
+    center = rng.uniform(-4, np.array(shape) + 4, size=(n, 3)).astype(F32)
+    radius = np.abs(rng.normal(0, 2.5, size=n)).astype(F32)
+    radius[:3] = 0.0
+    radius[3] = 1e6
+    center[4] = np.nan
+    center[5, 1] = np.inf
+    center[6, 2] = -np.inf
+    ranges, pids = vo.bin_tiles(center, radius, shape, tile)
+    sets = _brute_force_tiles(center, radius, shape, tile)
+    num_tiles = math.prod(vo.tile_grid(shape, tile))
+    expect = [[] for _ in range(num_tiles)]
+    for g, ts in enumerate(sets):
+        for t in ts:
+            expect[t].append(g)
+    for t in range(num_tiles):
+        s, e = ranges[t]
+        assert list(pids[s:e]) == sorted(expect[t]), t  # ascending Gaussian id (stable order)
+        if not expect[t]:
+            assert s == 0 and e == 0
+    assert pids.size == sum(len(x) for x in sets)
+
+
+def test_binning_ranges_partition_pairs():
+    c = cases.microbench_case(n=5000, shape=(64, 48, 40), seed=1)
+    c32, r32, _, _ = vo.prologue(c["pos"], c["scales"], c["rot"], c["shape"])
+    ranges, pids = vo.bin_tiles(c32, r32, c["shape"], 8)
+    nonempty = ranges[ranges[:, 1] > ranges[:, 0]]
+    order = np.argsort(nonempty[:, 0])
+    spans = nonempty[order]
+    assert spans[0, 0] == 0 and spans[-1, 1] == pids.size
+    assert np.all(spans[1:, 0] == spans[:-1, 1])
+    for s, e in spans:
+        assert np.all(np.diff(pids[s:e]) > 0)
+
+
+def test_negative_radius_has_no_pairs():
+    center = np.array([[5.0, 5.0, 5.0], [2.0, 2.0, 2.0]], F32)
+    radius = np.array([-3.0, 1.0], F32)
+    ranges, pids = vo.bin_tiles(center, radius, (16, 16, 16), 4)
+    assert 0 not in set(pids.tolist())
+    assert pids.size > 0
+
+
+# ----------------------------------------------------------------- forward --
+
+def _single_gaussian(center, scales_vox, shape, rho=(1.0, 0.0), quat=(1.0, 0.0, 0.0, 0.0)):
+    hs = np.asarray(shape, np.float64) / 2
+    pos = ((np.asarray(center, np.float64) + 0.5) / hs - 1.0).astype(F32)[None]
+    sc = (np.asarray(scales_vox, np.float64) / hs).astype(F32)[None]
+    rot = np.asarray(quat, F32)[None]
+    c32, r32, cov, hs32 = vo.prologue(pos, sc, rot, shape)
+    return c32, r32, cov, hs32, sc
+
+
+def test_forward_closed_form_axis_aligned():
+    shape, tile = (24, 24, 24), 8
+    c32, r32, cov, hs, sc = _single_gaussian((11.0, 12.0, 9.0), (1.5, 2.0, 0.8), shape, rho=(1.0, 0.0))
+    ranges, pids = vo.bin_tiles(c32, r32, shape, tile)
+    vr, vi, _ = vo.forward(c32, cov, [0.7], [-0.3], ranges, pids, hs, shape, tile)
+    vr = vr.reshape(shape)
+    vi = vi.reshape(shape)
+    s_vox = sc[0].astype(np.float64) * hs  # the voxel sigma as the prologue sees it
+    assert vr[11, 12, 9] == pytest.approx(0.7, rel=1e-12)
+    assert vi[11, 12, 9] == pytest.approx(-0.3, rel=1e-12)
+    for off in [(1, 0, 0), (0, -2, 0), (0, 0, 1), (2, 1, -1), (3, 0, 0)]:
+        u = np.asarray(off, np.float64)
+        # d = u / hs, M = diag(1/(s+1e-8)^2) in normalised units.
+        q = float(np.sum((u / hs) ** 2 / (sc[0].astype(np.float64) + 1e-8) ** 2))
+        expect = 0.7 * math.exp(-0.5 * q) if q <= 9.0 else 0.0
+        assert vr[11 + off[0], 12 + off[1], 9 + off[2]] == pytest.approx(expect, rel=1e-10, abs=1e-300)
+        assert abs(q - float(np.sum((u / s_vox) ** 2))) < 1e-5
+    # Outside the 3-sigma support, and empty tiles, are exactly zero.
+    assert vr[11, 12, 9 + 4] == 0.0  # 4 / 0.8 = 5 sigma
+    assert vr[0, 0, 0] == 0.0 and vr[23, 23, 23] == 0.0
+
+
+def test_forward_linearity_permutation_translation():
+    c = cases.edge_case("anisotropic_volume", seed=2)
+    shape, tile = c["shape"], c["tile"]
+    c32, r32, cov, hs = vo.prologue(c["pos"], c["scales"], c["rot"], shape)
+    keep = np.arange(200)  # a subset keeps the test fast
+    c32, r32, cov = c32[keep], r32[keep], cov[keep]
+    rr, ri = c["rr"][keep].astype(np.float64), c["ri"][keep].astype(np.float64)
+    ranges, pids = vo.bin_tiles(c32, r32, shape, tile)
+    a_r, a_i, _ = vo.forward(c32, cov, rr, ri, ranges, pids, hs, shape, tile)
+    assert np.abs(a_r).sum() > 0
+    b_r, b_i, _ = vo.forward(c32, cov, 2.5 * rr - ri, ri + 0.5 * rr, ranges, pids, hs, shape, tile)
+    np.testing.assert_allclose(b_r, 2.5 * a_r - a_i, atol=1e-12)
+    np.testing.assert_allclose(b_i, a_i + 0.5 * a_r, atol=1e-12)
+    perm = np.random.default_rng(0).permutation(200)
+    rp, pp = vo.bin_tiles(c32[perm], r32[perm], shape, tile)
+    p_r, p_i, _ = vo.forward(c32[perm], cov[perm], rr[perm], ri[perm], rp, pp, hs, shape, tile)
+    np.testing.assert_allclose(p_r, a_r, atol=1e-12)
+    np.testing.assert_allclose(p_i, a_i, atol=1e-12)
+    # Translation by one tile along D of Gaussians whose box stays inside the volume.
+    inner = (c32[:, 0] - r32 > 1) & (c32[:, 0] + r32 < shape[0] - tile - 1)
+    assert inner.sum() > 50
+    ci, rgi, covi, rri, rii = c32[inner], r32[inner], cov[inner], rr[inner], ri[inner]
+    rg, pg = vo.bin_tiles(ci, rgi, shape, tile)
+    x_r, _, _ = vo.forward(ci, covi, rri, rii, rg, pg, hs, shape, tile)
+    ct = ci + np.array([tile, 0, 0], F32)
+    rg2, pg2 = vo.bin_tiles(ct, rgi, shape, tile)
+    y_r, _, _ = vo.forward(ct, covi, rri, rii, rg2, pg2, hs, shape, tile)
+    x3 = x_r.reshape(shape)
+    y3 = y_r.reshape(shape)
+    # ci + tile may round in fp32, so the shift is exact only to ~1e-6.
+    np.testing.assert_allclose(y3[tile:], x3[:-tile], rtol=1e-5, atol=2e-6)
+    assert np.abs(y3[:tile]).max() == 0.0
+
+
+# ---------------------------------------------------------------- backward --
+
+def _fd_case(seed=0):
+    c = cases.adjoint_check_case(seed, "voxel")
+    keep = np.arange(24)
+    shape = (16, 16, 16)
+    rng = np.random.default_rng(seed)
+    center = rng.uniform(2, 14, size=(24, 3)).astype(F32)
+    scales = (np.exp(rng.normal(0, 0.3, size=(24, 3))) / 8.0).astype(F32)
+    _, r32, cov, hs = vo.prologue(np.zeros((24, 3), F32), scales, c["rot"][keep], shape)
+    return shape, center, r32, cov, c["rr"][keep].astype(np.float64), c["ri"][keep].astype(np.float64), hs
+
+
+def test_backward_vs_central_finite_differences():
+    shape, tile = (16, 16, 16), 8
+    _, center, r32, cov, rr, ri, hs = _fd_case(0)
+    ranges, pids = vo.bin_tiles(center, r32, shape, tile)
+    rng = np.random.default_rng(5)
+    g_r = rng.normal(size=int(np.prod(shape)))
+    g_i = rng.normal(size=int(np.prod(shape)))
+    dc, dm, drr, dri, _ = vo.backward(center, cov, rr, ri, ranges, pids, hs, g_r, g_i, shape, tile)
+    c64 = center.astype(np.float64)
+    h = 1e-6
+
+    def loss(c_, m_, rr_, ri_):
+        vr, vi, _ = vo.forward(c_, m_, rr_, ri_, ranges, pids, hs, shape, tile)
+        return float(vr @ g_r + vi @ g_i)
+
+    def active(c_, m_):
+        out = []
+        for t in range(ranges.shape[0]):
+            terms = vo._tile_terms(t, c_, m_.reshape(-1, 3, 3), ranges, pids, hs, shape, tile)
+            if terms is not None:
+                out.append(terms[4] <= 9.0)
+        return out
+
+    base_active = active(c64, cov)
+    checked = skipped = 0
+    for g in range(center.shape[0]):
+        for kind, idx in [("c", i) for i in range(3)] + [("m", (i, j)) for i in range(3) for j in range(3)] + \
+                         [("rr", None), ("ri", None)]:
+            def perturbed(sign):
+                c_ = c64.copy()
+                m_ = cov.copy()
+                rr_ = rr.copy()
+                ri_ = ri.copy()
+                if kind == "c":
+                    c_[g, idx] += sign * h
+                elif kind == "m":
+                    m_[g, idx[0], idx[1]] += sign * h
+                elif kind == "rr":
+                    rr_[g] += sign * h
+                else:
+                    ri_[g] += sign * h
+                return c_, m_, rr_, ri_
+
+            p, m = perturbed(+1), perturbed(-1)
+            if kind in ("c", "m"):
+                if any((a != b).any() for a, b in zip(active(p[0], p[1]), base_active)) or \
+                        any((a != b).any() for a, b in zip(active(m[0], m[1]), base_active)):
+                    skipped += 1
+                    continue
+            fd = (loss(*p) - loss(*m)) / (2 * h)
+            an = {"c": lambda: dc[g, idx], "m": lambda: dm[g, idx[0], idx[1]], "rr": lambda: drr[g],
+                  "ri": lambda: dri[g]}[kind]()
+            assert fd == pytest.approx(an, rel=2e-5, abs=2e-6), (g, kind, idx, fd, an)
+            checked += 1
+    assert checked > 20 * 14 and skipped < checked // 20
+
+
+def test_adjoint_dot_product_identity():
+    """<J(rho), G> = <rho, J^T G> for the density (the forward is linear in rho)."""
+    c = cases.adjoint_check_case(0, "voxel")
+    shape, tile = c["shape"], c["tile"]
+    c32, r32, cov, hs = vo.prologue(c["pos"], c["scales"], c["rot"], shape)
+    ranges, pids = vo.bin_tiles(c32, r32, shape, tile)
+    rng = np.random.default_rng(9)
+    g_r = rng.normal(size=int(np.prod(shape)))
+    g_i = rng.normal(size=int(np.prod(shape)))
+    vr, vi, _ = vo.forward(c32, cov, c["rr"], c["ri"], ranges, pids, hs, shape, tile)
+    _, _, drr, dri, _ = vo.backward(c32, cov, c["rr"], c["ri"], ranges, pids, hs, g_r, g_i, shape, tile)
+    lhs = vr @ g_r + vi @ g_i
+    rhs = c["rr"].astype(np.float64) @ drr + c["ri"].astype(np.float64) @ dri
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+
+
+def test_torch64_matches_numpy_oracle():
+    """The differentiable FP64 restatement agrees with the NumPy forward and adjoint."""
+    c = cases.edge_case("partial_tiles", seed=1)
+    shape, tile = c["shape"], c["tile"]
+    c32, r32, cov, hs = vo.prologue(c["pos"], c["scales"], c["rot"], shape)
+    ranges, pids = vo.bin_tiles(c32, r32, shape, tile)
+    t = {k: torch.from_numpy(np.asarray(v, np.float64)).requires_grad_()
+         for k, v in dict(pos=c["pos"], ls=np.log(c["scales"]), rot=c["rot"], rr=c["rr"], ri=c["ri"]).items()}
+    er, ei = torch64.voxelize64(t["pos"], t["ls"], t["rot"], t["rr"], t["ri"], shape, tile, ranges, pids)
+    # The numpy forward on the fp64 (unrounded) centre and cov_inv.
+    c64 = ((c["pos"].astype(np.float64) + 1) * 0.5 * np.array(shape) - 0.5)
+    rot = vo.quaternion_to_rotation(c["rot"])
+    tt = rot * (1.0 / (np.exp(np.log(c["scales"]).astype(np.float64)) + 1e-8))[:, None, :]
+    cov64 = tt @ np.swapaxes(tt, 1, 2)
+    nr, ni, _ = vo.forward(c64, cov64, c["rr"], c["ri"], ranges, pids, np.array(shape) / 2, shape, tile)
+    np.testing.assert_allclose(er.detach().numpy(), nr, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(ei.detach().numpy(), ni, rtol=1e-10, atol=1e-12)
+    rng = np.random.default_rng(2)
+    g_r = rng.normal(size=er.shape[0])
+    g_i = rng.normal(size=er.shape[0])
+    ((er * torch.from_numpy(g_r)).sum() + (ei * torch.from_numpy(g_i)).sum()).backward()
+    dc, dm, drr, dri, _ = vo.backward(c64, cov64, c["rr"], c["ri"], ranges, pids, np.array(shape) / 2, g_r, g_i,
+                                      shape, tile)
+    np.testing.assert_allclose(t["rr"].grad.numpy(), drr, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(t["ri"].grad.numpy(), dri, rtol=1e-9, atol=1e-12)
+    # Chain rule of the centre: dL/dp = dL/dc * 0.5 * shape.
+    np.testing.assert_allclose(t["pos"].grad.numpy(), dc * 0.5 * np.array(shape), rtol=1e-8, atol=1e-10)
