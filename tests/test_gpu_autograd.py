@@ -43,9 +43,11 @@ def test_tile_voxelizer_param_gradients_vs_fp64(reading):
     torch.cuda.synchronize()
 
     # FP64 reference with the reference's binning of the fp32 prologue.
-    c32, r32, cov64, hs = vo.prologue(c["pos"], np.exp(np.log(c["scales"])), c["rot"], shape)
+    # The oracle bins the exact fp32 scales the GPU prologue saw (exp on the GPU).
+    s_gpu = torch.exp(log_s).detach().cpu().numpy()
+    c32, r32, cov64, hs = vo.prologue(c["pos"], s_gpu, c["rot"], shape)
     ranges, pids = vo.bin_tiles(c32, r32, shape, tile)
-    _, _, amb = vo.backward(c32, cov64, c["rr"], c["ri"], ranges, pids, hs, g_r, g_i, shape, tile, band=1e-4)
+    *_, amb = vo.backward(c32, cov64, c["rr"], c["ri"], ranges, pids, hs, g_r, g_i, shape, tile, band=1e-4)
     t = {k: torch.from_numpy(np.asarray(v, dtype=np.float64)).requires_grad_()
          for k, v in dict(pos=c["pos"], log_s=np.log(c["scales"]), rot=c["rot"], dr=c["rr"], di=c["ri"]).items()}
     er, ei = torch64.voxelize64(t["pos"], t["log_s"], t["rot"], t["dr"], t["di"], shape, tile, ranges, pids)
