@@ -725,131 +725,268 @@ __global__ __launch_bounds__(kTileThreads) void k_forward(const GRec* __restrict
   }
 }
 
+// Cross-row sums with the gfx950 lane-swap instructions.  A wave is four
+// rows of 16 lanes.  v_permlane32_swap(a, b) exchanges the upper half of a with
+// the lower half of b, so a' + b' holds, in the lower half, a summed over rows
+// {r, r+2} and, in the upper half, b summed the same way.
+__device__ __forceinline__ float pair_sum_rows02(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// v_permlane16_swap(a, b) exchanges the odd rows of a with the even rows of b,
+// so a' + b' holds a summed over rows {0,1} in row 0, b over {0,1} in row 1,
+// a over {2,3} in row 2 and b over {2,3} in row 3.
+__device__ __forceinline__ float pair_sum_rows01(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
 constexpr int kMoments = 11;  // rho_r, rho_i, 3 first moments, 6 second moments
 constexpr int kAccStride = 12;
 
+// ---- backward ----
+//
+// The backward gives every Gaussian of a batch to one set of lanes for the
+// whole tile, so its 11 moments accumulate in registers and no LDS or global
+// atomics are needed inside the tile.  Gaussians are classed by the number of
+// 4^3 sub-bricks their exact box touches inside the tile; a group is 16
+// Gaussians of one class (lane = (Gaussian slot, W offset)), and every lane
+// walks the sub-bricks of its own Gaussian, so a group costs exactly
+// class x 16 x 64 evaluations.  Groups are dealt to the 8 waves in descending
+// class order for balance.
+
+constexpr int kMaxClass = 64;  // sub-bricks per tile for tile <= 16
+
+struct BwdStage {
+  float c0[kBatch], c1[kBatch], c2[kBatch], rr[kBatch], ri[kBatch];
+  float a00[kBatch], a11[kBatch], a22[kBatch], b01[kBatch], b02[kBatch], b12[kBatch];
+  float out[kAccStride][kBatch];  // per-pair moment totals for this tile
+  uint16_t sbox[kBatch];          // sub-brick box: lo0,lo1,lo2,hi0,hi1,hi2, 2 bits each
+  uint16_t order[kBatch];         // batch indices grouped by class
+  uint16_t slot[kBatch];          // rank inside the class
+  uint8_t cls[kBatch];            // sub-bricks touched (0: none)
+  int cls_cnt[kMaxClass + 1];
+  int cls_start[kMaxClass + 1];
+  int grp_start[kMaxClass];       // indexed by kMaxClass - class (descending classes)
+  int num_groups;
+};
+
+struct BwdAcc {
+  float dr, di, m, m0, m00, m1, m01, m11, m2, m22, m02, m12;
+};
+
+// One 4^3 sub-brick of one lane's Gaussian: 16 (D, H) positions of the lane's
+// W column.  TT is the tile edge when known at compile time (0: runtime).
+template <int TT, bool kPartial>
+__device__ __forceinline__ void bwd_subbrick(const float2* __restrict__ sh_g, int tp, const BlockCtx& b,
+                                             int s0, int s1, int s2, int xi, bool g_ok, float c0, float c1,
+                                             float c2, float rr, float ri, float a00, float a11, float a22,
+                                             float b01, float b02, float b12, BwdAcc& acc) {
+  const int T = TT > 0 ? TT : tp;  // row stride of sh_g
+  const float u2 = (float)(b.o2 + s2 + xi) - c2;
+  const float k0 = a22 * u2 * u2, k1 = b02 * u2, k2 = b12 * u2;
+  const float2* gp = sh_g + (s0 * T + s1) * T + s2 + xi;
+  bool ok_lane = g_ok;
+  if (kPartial) ok_lane = ok_lane && (s2 + xi < b.e2);
+  float m = 0.0f, m0 = 0.0f, m1 = 0.0f;
+  float u1v[kSub];
+#pragma unroll
+  for (int y = 0; y < kSub; ++y) u1v[y] = (float)(b.o1 + s1 + y) - c1;
+#pragma unroll
+  for (int z = 0; z < kSub; ++z) {
+    const float u0 = (float)(b.o0 + s0 + z) - c0;
+    float A = fmaf(fmaf(a00, u0, k1), u0, k0);
+    if (kPartial) A = (s0 + z < b.e0) ? A : -INFINITY;
+    const float B = fmaf(b01, u0, k2);
+    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+#pragma unroll
+    for (int y = 0; y < kSub; ++y) {
+      const float u1 = u1v[y];
+      float Ay = A;
+      if (kPartial) Ay = (s1 + y < b.e1) ? A : -INFINITY;
+      const float q = fmaf(fmaf(a11, u1, B), u1, Ay);
+      const bool ok = ok_lane && (q >= kThr);
+      const float w = ok ? __builtin_amdgcn_exp2f(q) : 0.0f;
+      const float2 g = gp[(z * T + y) * T];
+      acc.dr = fmaf(w, g.x, acc.dr);
+      acc.di = fmaf(w, g.y, acc.di);
+      const float wt = w * fmaf(ri, g.y, rr * g.x);
+      t0 += wt;
+      const float sy = wt * u1;
+      t1 += sy;
+      t2 = fmaf(sy, u1, t2);
+    }
+    m += t0;
+    const float tz = t0 * u0;
+    m0 += tz;
+    acc.m00 = fmaf(tz, u0, acc.m00);
+    m1 += t1;
+    acc.m01 = fmaf(t1, u0, acc.m01);
+    acc.m11 += t2;
+  }
+  acc.m += m;
+  acc.m0 += m0;
+  acc.m1 += m1;
+  acc.m2 = fmaf(u2, m, acc.m2);
+  acc.m22 = fmaf(u2 * u2, m, acc.m22);
+  acc.m02 = fmaf(u2, m0, acc.m02);
+  acc.m12 = fmaf(u2, m1, acc.m12);
+}
+
+template <int TT>
 __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restrict__ rec,
                                                            const int32_t* __restrict__ ranges,
                                                            const int32_t* __restrict__ pair_ids, VoxGeom geo,
                                                            const float* __restrict__ grad_r,
                                                            const float* __restrict__ grad_i,
                                                            float* __restrict__ gacc) {
-  __shared__ Stage sh;
-  __shared__ float sh_acc[kMoments][kBatch];
+  __shared__ BwdStage sh;
+  // Upstream gradient of the tile, [TP][TP][TP] with TP the tile edge rounded
+  // up to a multiple of 4, zero outside the tile's valid voxels.
+  extern __shared__ float2 sh_g[];
   const int logical = xcd_remap(blockIdx.x, gridDim.x);
   if (logical >= geo.blocks) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const int gl = lane & 15, xi = lane >> 4;
-  const BlockCtx b = make_ctx(logical, geo, ranges, wave);
-  const bool x_ok = xi < b.n2;
+  const int T = TT > 0 ? TT : geo.nsb * kSub;
+  // One block per tile: the context's wave-specific fields are unused here.
+  const BlockCtx b = make_ctx(logical, geo, ranges, 0);
+  const bool partial = b.e0 < T || b.e1 < T || b.e2 < T;
 
-  // Upstream gradient of this lane's W column (16 positions), zero outside.
-  float gr[kSub * kSub], gi[kSub * kSub];
-#pragma unroll
-  for (int z = 0; z < kSub; ++z) {
-#pragma unroll
-    for (int y = 0; y < kSub; ++y) {
-      float vr = 0.0f, vi = 0.0f;
-      if (b.active && x_ok && z < b.n0 && y < b.n1) {
-        const int64_t flat =
-            ((int64_t)(b.o0 + b.s0 + z) * geo.H + (b.o1 + b.s1 + y)) * geo.W + (b.o2 + b.s2 + xi);
-        vr = grad_r[flat];
-        vi = grad_i[flat];
-      }
-      gr[z * kSub + y] = vr;
-      gi[z * kSub + y] = vi;
+  for (int i = tid; i < T * T * T; i += kTileThreads) {
+    const int z = i / (T * T), y = (i / T) % T, x = i % T;
+    float2 g = make_float2(0.0f, 0.0f);
+    if (z < b.e0 && y < b.e1 && x < b.e2) {
+      const int64_t flat = ((int64_t)(b.o0 + z) * geo.H + (b.o1 + y)) * geo.W + (b.o2 + x);
+      g = make_float2(grad_r[flat], grad_i[flat]);
     }
+    sh_g[i] = g;
   }
+  if (tid <= kMaxClass) sh.cls_cnt[tid] = 0;
 
   for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
     const int cnt = min(kBatch, b.p_end - bs);
     __syncthreads();
-    if ((int)threadIdx.x < cnt) stage_pair(sh, threadIdx.x, rec[pair_ids[bs + threadIdx.x]], b);
+    // Stage: parameters, sub-brick box and class of every pair of the batch.
+    if (tid < cnt) {
+      const GRec r = rec[pair_ids[bs + tid]];
+      sh.c0[tid] = r.c0;
+      sh.c1[tid] = r.c1;
+      sh.c2[tid] = r.c2;
+      sh.rr[tid] = r.rr;
+      sh.ri[tid] = r.ri;
+      sh.a00[tid] = r.a00;
+      sh.a11[tid] = r.a11;
+      sh.a22[tid] = r.a22;
+      sh.b01[tid] = r.b01;
+      sh.b02[tid] = r.b02;
+      sh.b12[tid] = r.b12;
 #pragma unroll
-    for (int f = 0; f < kMoments; ++f) sh_acc[f][threadIdx.x] = 0.0f;
+      for (int f = 0; f < kAccStride; ++f) sh.out[f][tid] = 0.0f;
+      const float lo0 = fmaxf(ceilf(r.c0 - r.e0), (float)b.o0);
+      const float lo1 = fmaxf(ceilf(r.c1 - r.e1), (float)b.o1);
+      const float lo2 = fmaxf(ceilf(r.c2 - r.e2), (float)b.o2);
+      const float hi0 = fminf(floorf(r.c0 + r.e0), (float)(b.o0 + b.e0 - 1));
+      const float hi1 = fminf(floorf(r.c1 + r.e1), (float)(b.o1 + b.e1 - 1));
+      const float hi2 = fminf(floorf(r.c2 + r.e2), (float)(b.o2 + b.e2 - 1));
+      int cls = 0;
+      if (lo0 <= hi0 && lo1 <= hi1 && lo2 <= hi2) {
+        const int l0 = ((int)lo0 - b.o0) >> 2, l1 = ((int)lo1 - b.o1) >> 2, l2 = ((int)lo2 - b.o2) >> 2;
+        const int h0 = ((int)hi0 - b.o0) >> 2, h1 = ((int)hi1 - b.o1) >> 2, h2 = ((int)hi2 - b.o2) >> 2;
+        sh.sbox[tid] = (uint16_t)(l0 | (l1 << 2) | (l2 << 4) | (h0 << 6) | (h1 << 8) | (h2 << 10));
+        cls = (h0 - l0 + 1) * (h1 - l1 + 1) * (h2 - l2 + 1);
+        sh.slot[tid] = (uint16_t)atomicAdd(&sh.cls_cnt[cls], 1);
+      }
+      sh.cls[tid] = (uint8_t)cls;
+    }
     __syncthreads();
-    if (b.active) {
-      const int nhit = build_list(sh, cnt, b, wave, lane);
-      for (int gb = 0; gb < nhit; gb += 16) {
-        const int idx = gb + gl;
-        const bool slot_ok = idx < nhit;
-        const bool g_ok = slot_ok && x_ok;
-        const int k = sh.list[wave][slot_ok ? idx : gb];
-        const GaussLane g = load_gauss(sh, k, b, xi);
-        float dr = 0.0f, di = 0.0f;
-        float m = 0.0f, m0 = 0.0f, m00 = 0.0f, m1 = 0.0f, m01 = 0.0f, m11 = 0.0f;
+    // Class offsets and group offsets, classes in descending order.
+    if (wave == 0) {
+      const int c = kMaxClass - lane;  // lane 0 -> class 64 ... lane 63 -> class 1
+      const int n_c = sh.cls_cnt[c];
+      const int ng = (n_c + 15) >> 4;
+      int inc_n = n_c, inc_g = ng;
 #pragma unroll
-        for (int z = 0; z < kSub; ++z) {
-          if (z < b.n0) {
-            const float u0 = (float)(b.o0 + b.s0 + z) - g.c0;
-            const float A = fmaf(fmaf(g.a00, u0, g.k1), u0, g.k0);
-            const float B = fmaf(g.b01, u0, g.k2);
-            float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
-#pragma unroll
-            for (int y = 0; y < kSub; ++y) {
-              if (y < b.n1) {
-                const float u1 = g.u1[y];
-                const float q = fmaf(fmaf(g.a11, u1, B), u1, A);
-                const bool ok = g_ok && (q >= kThr);
-                const float w = ok ? __builtin_amdgcn_exp2f(q) : 0.0f;
-                const float vr = gr[z * kSub + y], vi = gi[z * kSub + y];
-                dr = fmaf(w, vr, dr);
-                di = fmaf(w, vi, di);
-                const float wt = w * fmaf(g.ri, vi, g.rr * vr);
-                t0 += wt;
-                const float sy = wt * u1;
-                t1 += sy;
-                t2 = fmaf(sy, u1, t2);
-              }
-            }
-            m += t0;
-            const float tz = t0 * u0;
-            m0 += tz;
-            m00 = fmaf(tz, u0, m00);
-            m1 += t1;
-            m01 = fmaf(t1, u0, m01);
-            m11 += t2;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int on = __shfl_up(inc_n, off, 64);
+        const int og = __shfl_up(inc_g, off, 64);
+        if (lane >= off) {
+          inc_n += on;
+          inc_g += og;
+        }
+      }
+      sh.cls_start[c] = inc_n - n_c;
+      sh.grp_start[lane] = inc_g - ng;
+      if (lane == 63) sh.num_groups = inc_g;
+    }
+    __syncthreads();
+    if (tid < cnt && sh.cls[tid] != 0) sh.order[sh.cls_start[sh.cls[tid]] + sh.slot[tid]] = (uint16_t)tid;
+    __syncthreads();
+
+    const int num_groups = sh.num_groups;
+    for (int gi = wave; gi < num_groups; gi += kWaves) {
+      int lo = 0, hi = kMaxClass - 1;  // last class rank whose groups start at or before gi
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sh.grp_start[mid] <= gi) lo = mid; else hi = mid - 1;
+      }
+      const int c = kMaxClass - lo;
+      const int within = (gi - sh.grp_start[lo]) << 4;
+      const int m_valid = min(16, sh.cls_cnt[c] - within);
+      const bool slot_ok = gl < m_valid;
+      const int k = sh.order[sh.cls_start[c] + within + (slot_ok ? gl : 0)];
+      const float c0 = sh.c0[k], c1 = sh.c1[k], c2 = sh.c2[k], rr = sh.rr[k], ri = sh.ri[k];
+      const float a00 = sh.a00[k], a11 = sh.a11[k], a22 = sh.a22[k];
+      const float b01 = sh.b01[k], b02 = sh.b02[k], b12 = sh.b12[k];
+      const int sb = sh.sbox[k];
+      const int l0 = sb & 3, l1 = (sb >> 2) & 3, l2 = (sb >> 4) & 3;
+      const int d1 = ((sb >> 8) & 3) - l1 + 1, d2 = ((sb >> 10) & 3) - l2 + 1;
+      BwdAcc acc = {};
+      int j0 = 0, j1 = 0, j2 = 0;
+      for (int j = 0; j < c; ++j) {
+        const int s0 = (l0 + j0) * kSub, s1 = (l1 + j1) * kSub, s2 = (l2 + j2) * kSub;
+        if (partial)
+          bwd_subbrick<TT, true>(sh_g, T, b, s0, s1, s2, xi, slot_ok, c0, c1, c2, rr, ri, a00, a11, a22, b01, b02,
+                                 b12, acc);
+        else
+          bwd_subbrick<TT, false>(sh_g, T, b, s0, s1, s2, xi, slot_ok, c0, c1, c2, rr, ri, a00, a11, a22, b01, b02,
+                                  b12, acc);
+        if (++j2 == d2) {
+          j2 = 0;
+          if (++j1 == d1) {
+            j1 = 0;
+            ++j0;
           }
         }
-        // Moments of w*t over the column: sum(w t u_a), sum(w t u_a u_b).
-        float v[kMoments];
-        v[0] = dr;
-        v[1] = di;
-        v[2] = m0;
-        v[3] = m1;
-        v[4] = g.u2 * m;
-        v[5] = m00;
-        v[6] = m11;
-        v[7] = g.u2 * g.u2 * m;
-        v[8] = m01;
-        v[9] = g.u2 * m0;
-        v[10] = g.u2 * m1;
+      }
+      float v[kMoments] = {acc.dr, acc.di, acc.m0, acc.m1, acc.m2, acc.m00, acc.m11, acc.m22, acc.m01, acc.m02,
+                           acc.m12};
+      // Reduce-scatter over the 4 W lanes of each Gaussian: afterwards the
+      // lane in row xi holds the totals of moments 3*xi .. 3*xi+2.
+      float s6[6], t3[3];
 #pragma unroll
-        for (int f = 0; f < kMoments; ++f) {
-          v[f] += __shfl_xor(v[f], 16, 64);
-          v[f] += __shfl_xor(v[f], 32, 64);
-        }
-        if (xi == 0 && slot_ok) {
+      for (int i = 0; i < 6; ++i) s6[i] = pair_sum_rows02(v[i], i + 6 < kMoments ? v[i + 6] : 0.0f);
 #pragma unroll
-          for (int f = 0; f < kMoments; ++f) atomicAdd(&sh_acc[f][k], v[f]);
-        }
+      for (int i = 0; i < 3; ++i) t3[i] = pair_sum_rows01(s6[i], s6[i + 3]);
+      if (slot_ok) {
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) sh.out[3 * xi + jj][k] = t3[jj];
       }
     }
     __syncthreads();
-    if ((int)threadIdx.x < cnt) {
-      float v[kMoments];
-      bool any = false;
-#pragma unroll
-      for (int f = 0; f < kMoments; ++f) {
-        v[f] = sh_acc[f][threadIdx.x];
-        any |= (v[f] != 0.0f);
-      }
-      if (any) {
-        float* dst = gacc + (int64_t)pair_ids[bs + threadIdx.x] * kAccStride;
-#pragma unroll
-        for (int f = 0; f < kMoments; ++f) unsafeAtomicAdd(dst + f, v[f]);
+    // Flush: adjacent lanes add the moments of one pair, so each atomic
+    // instruction touches a few 48-byte rows instead of 64 scattered lines.
+    for (int idx = tid; idx < cnt * kAccStride; idx += kTileThreads) {
+      const int p = idx / kAccStride;
+      const int f = idx - p * kAccStride;
+      if (f < kMoments) {
+        const float v = sh.out[f][p];
+        if (v != 0.0f) unsafeAtomicAdd(gacc + (int64_t)pair_ids[bs + p] * kAccStride + f, v);
       }
     }
+    if (tid <= kMaxClass) sh.cls_cnt[tid] = 0;
   }
 }
 
@@ -945,7 +1082,7 @@ __global__ __launch_bounds__(kTileThreads) void k_count(const GRec* __restrict__
 // ------------------------------------------------------------- host helpers --
 
 bool tile_geometry(int32_t d, int32_t h, int32_t w, int32_t tile, int nt[3], int64_t* num_tiles) {
-  if (d <= 0 || h <= 0 || w <= 0 || tile <= 0 || tile > 32) return false;
+  if (d <= 0 || h <= 0 || w <= 0 || tile <= 0 || tile > 16) return false;
   nt[0] = ceil_div_i(d, tile);
   nt[1] = ceil_div_i(h, tile);
   nt[2] = ceil_div_i(w, tile);
@@ -1234,8 +1371,17 @@ int gstv_backward(const float* center_vox, const float* cov_inv, const float* rh
   GSTV_LAUNCH_CHECK("grad_clear");
   const VoxGeom geo = make_vox_geom(d, h, w, tile, nt, num_tiles);
   const int slot = prof_start(1, s);
-  hipLaunchKernelGGL(k_backward, dim3(tile_grid(geo)), dim3(kTileThreads), 0, s, (const GRec*)rec, tile_ranges,
-                     pair_ids, geo, grad_r, grad_i, gacc);
+  VoxGeom geo_b = geo;  // the backward runs one block per tile
+  geo_b.chunks = 1;
+  geo_b.blocks = num_tiles;
+  const int tp = geo.nsb * kSub;
+  const size_t lds_grad = (size_t)tp * tp * tp * sizeof(float2);
+  if (tile == 8)
+    hipLaunchKernelGGL(k_backward<8>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, (const GRec*)rec,
+                       tile_ranges, pair_ids, geo_b, grad_r, grad_i, gacc);
+  else
+    hipLaunchKernelGGL(k_backward<0>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, (const GRec*)rec,
+                       tile_ranges, pair_ids, geo_b, grad_r, grad_i, gacc);
   GSTV_LAUNCH_CHECK("backward");
   prof_stop(1, slot, s);
   hipLaunchKernelGGL(k_finalize, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, (const float*)gacc, cov_inv,

@@ -23,7 +23,7 @@
  *     gives the message.  Kernel launch errors are reported as GSTV_ERR_HIP.
  *   - Volumes are row-major [D][H][W]; flat index = d*H*W + h*W + w.
  *   - Tiles are cubes of `tile` voxels; nT* = ceil(size / tile); tile id =
- *     td*(nTH*nTW) + th*nTW + tw.  `tile` must be in [1, 32].
+ *     td*(nTH*nTW) + th*nTW + tw.  `tile` must be in [1, 16].
  *   - Gaussian parameters are in voxel space: center_vox [N][3] (axis order
  *     D, H, W), cov_inv [N][3][3] row-major (the inverse covariance in units
  *     normalised by half_shape), density planes rho_r [N], rho_i [N].

@@ -79,7 +79,7 @@ def edge_case(kind, seed=0):
     return dict(shape=shape, tile=tile, pos=pos, scales=scales, rot=rot, rr=rr, ri=ri)
 
 
-EDGE_KINDS = ["partial_tiles", "outside", "tile1", "tile3", "tile5", "tile16", "tile32", "needles", "large_sigma",
+EDGE_KINDS = ["partial_tiles", "outside", "tile1", "tile3", "tile5", "tile12", "tile16", "needles", "large_sigma",
               "tiny_sigma", "anisotropic_volume", "single"]
 
 
