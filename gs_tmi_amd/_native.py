@@ -46,6 +46,8 @@ PROTOTYPES = {
     "gstv_count_evaluations": (ctypes.c_int,
                                [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32,
                                 _c_p, _c_p, _sz, _c_p]),
+    "gstv_prologue": (ctypes.c_int, [_c_p, _c_p, _c_p, _i32, _i32, _i32, _i32, _c_p, _c_p, _c_p, _c_p, _c_p]),
+    "gstv_epilogue": (ctypes.c_int, [_c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i32, _c_p, _c_p, _c_p, _c_p]),
     "gstv_profile_begin": (ctypes.c_int, []),
     "gstv_profile_end": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32),
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]),
@@ -227,3 +229,40 @@ def profile_end():
     _check(get_lib().gstv_profile_end(ctypes.byref(fms), ctypes.byref(fl), ctypes.byref(bms), ctypes.byref(bl)),
            "gstv_profile_end")
     return {"forward": (fms.value, fl.value), "backward": (bms.value, bl.value)}
+
+
+def prologue(positions, scales, rotations, shape):
+    """Fused prologue.  Returns (center_vox [n,3], radius_vox [n], cov_inv [n,3,3], half_shape [3])."""
+    lib = get_lib()
+    _require_gpu(positions, "positions")
+    dev = positions.device
+    p = positions.detach().contiguous().float()
+    s = scales.detach().contiguous().float()
+    q = rotations.detach().contiguous().float()
+    n = p.shape[0]
+    center = torch.empty(n, 3, dtype=torch.float32, device=dev)
+    radius = torch.empty(n, dtype=torch.float32, device=dev)
+    cov = torch.empty(n, 3, 3, dtype=torch.float32, device=dev)
+    hs = torch.empty(3, dtype=torch.float32, device=dev)
+    d, h, w = shape
+    with torch.cuda.device(dev):
+        st = lib.gstv_prologue(_ptr(p), _ptr(s), _ptr(q), n, d, h, w, _ptr(center), _ptr(radius), _ptr(cov),
+                               _ptr(hs), _stream(dev))
+    _check(st, "gstv_prologue")
+    return center, radius, cov, hs
+
+
+def epilogue(scales, rotations, grad_center, grad_cov, shape):
+    """Fused chain rule of the prologue.  Returns (grad_positions, grad_scales, grad_rotations)."""
+    lib = get_lib()
+    dev = scales.device
+    n = scales.shape[0]
+    gp = torch.empty(n, 3, dtype=torch.float32, device=dev)
+    gs = torch.empty(n, 3, dtype=torch.float32, device=dev)
+    gq = torch.empty(n, 4, dtype=torch.float32, device=dev)
+    d, h, w = shape
+    with torch.cuda.device(dev):
+        st = lib.gstv_epilogue(_ptr(scales), _ptr(rotations), _ptr(grad_center.contiguous()),
+                               _ptr(grad_cov.contiguous()), n, d, h, w, _ptr(gp), _ptr(gs), _ptr(gq), _stream(dev))
+    _check(st, "gstv_epilogue")
+    return gp, gs, gq

@@ -1019,6 +1019,117 @@ __global__ __launch_bounds__(256) void k_finalize(const float* __restrict__ gacc
   }
 }
 
+// ------------------------------------------------------ prologue / epilogue --
+
+// NaN-propagating maximum (torch.max semantics).
+__device__ __forceinline__ float max_nan(float a, float b) { return (a != a || a >= b) ? a : b; }
+
+// Fused prologue of TileVoxelizer.forward (gaussian/tile_voxelizer.py:281-290):
+//   hs = shape/2;  c = ((p + 1) * 0.5) * shape - 0.5;  r = max_a(s_a * hs_a) * 3;
+//   R = R(q / (|q| + 1e-8));  T = R * (1 / (s + 1e-8)) (column scaling);  M = T T^T.
+// Centre and radius round after every operation (no contraction) so binning
+// sees the same fp32 values the reference's torch ops produce.
+__global__ __launch_bounds__(256) void k_prologue(const float* __restrict__ pos, const float* __restrict__ scales,
+                                                  const float* __restrict__ rot, int n, float sh0, float sh1,
+                                                  float sh2, float* __restrict__ center, float* __restrict__ radius,
+                                                  float* __restrict__ cov, float* __restrict__ half_shape) {
+#pragma clang fp contract(off)
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  const float hs0 = sh0 * 0.5f, hs1 = sh1 * 0.5f, hs2 = sh2 * 0.5f;
+  if (g == 0) {
+    half_shape[0] = hs0;
+    half_shape[1] = hs1;
+    half_shape[2] = hs2;
+  }
+  if (g >= n) return;
+  const int64_t g3 = 3 * (int64_t)g;
+  center[g3 + 0] = ((pos[g3 + 0] + 1.0f) * 0.5f) * sh0 - 0.5f;
+  center[g3 + 1] = ((pos[g3 + 1] + 1.0f) * 0.5f) * sh1 - 0.5f;
+  center[g3 + 2] = ((pos[g3 + 2] + 1.0f) * 0.5f) * sh2 - 0.5f;
+  const float s0 = scales[g3 + 0], s1 = scales[g3 + 1], s2 = scales[g3 + 2];
+  radius[g] = max_nan(max_nan(s0 * hs0, s1 * hs1), s2 * hs2) * 3.0f;
+  const float4 q4 = *reinterpret_cast<const float4*>(rot + 4 * (int64_t)g);
+  const float nrm = sqrtf(q4.x * q4.x + q4.y * q4.y + q4.z * q4.z + q4.w * q4.w) + 1e-8f;
+  const float w = q4.x / nrm, x = q4.y / nrm, y = q4.z / nrm, z = q4.w / nrm;
+  const float R[3][3] = {{1.0f - 2.0f * y * y - 2.0f * z * z, 2.0f * x * y - 2.0f * z * w, 2.0f * x * z + 2.0f * y * w},
+                         {2.0f * x * y + 2.0f * z * w, 1.0f - 2.0f * x * x - 2.0f * z * z, 2.0f * y * z - 2.0f * x * w},
+                         {2.0f * x * z - 2.0f * y * w, 2.0f * y * z + 2.0f * x * w, 1.0f - 2.0f * x * x - 2.0f * y * y}};
+  const float si[3] = {1.0f / (s0 + 1e-8f), 1.0f / (s1 + 1e-8f), 1.0f / (s2 + 1e-8f)};
+  float T[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) T[i][k] = R[i][k] * si[k];
+  float* M = cov + 9 * (int64_t)g;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) M[3 * i + j] = T[i][0] * T[j][0] + T[i][1] * T[j][1] + T[i][2] * T[j][2];
+}
+
+// Chain rule of the prologue: from (dL/dc, dL/dM) to positions, linear scales
+// and quaternions.  dL/dT = (G + G^T) T;  dL/dR_ik = dL/dT_ik * si_k;
+// dL/dsi_k = sum_i dL/dT_ik R_ik;  dL/ds_k = -si_k^2 dL/dsi_k;  dL/dq from
+// dL/dR through R(q^) and the normalisation q^ = q / (|q| + 1e-8).  The
+// radius carries no gradient (the reference computes it under no_grad).
+__global__ __launch_bounds__(256) void k_epilogue(const float* __restrict__ scales, const float* __restrict__ rot,
+                                                  const float* __restrict__ g_center, const float* __restrict__ g_cov,
+                                                  int n, float sh0, float sh1, float sh2, float* __restrict__ g_pos,
+                                                  float* __restrict__ g_scales, float* __restrict__ g_rot) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= n) return;
+  const int64_t g3 = 3 * (int64_t)g;
+  g_pos[g3 + 0] = g_center[g3 + 0] * (0.5f * sh0);
+  g_pos[g3 + 1] = g_center[g3 + 1] * (0.5f * sh1);
+  g_pos[g3 + 2] = g_center[g3 + 2] * (0.5f * sh2);
+  const float s[3] = {scales[g3 + 0], scales[g3 + 1], scales[g3 + 2]};
+  const float4 q4 = *reinterpret_cast<const float4*>(rot + 4 * (int64_t)g);
+  const float nr = sqrtf(q4.x * q4.x + q4.y * q4.y + q4.z * q4.z + q4.w * q4.w);
+  const float nrm = nr + 1e-8f;
+  const float w = q4.x / nrm, x = q4.y / nrm, y = q4.z / nrm, z = q4.w / nrm;
+  const float R[3][3] = {{1.0f - 2.0f * y * y - 2.0f * z * z, 2.0f * x * y - 2.0f * z * w, 2.0f * x * z + 2.0f * y * w},
+                         {2.0f * x * y + 2.0f * z * w, 1.0f - 2.0f * x * x - 2.0f * z * z, 2.0f * y * z - 2.0f * x * w},
+                         {2.0f * x * z - 2.0f * y * w, 2.0f * y * z + 2.0f * x * w, 1.0f - 2.0f * x * x - 2.0f * y * y}};
+  const float si[3] = {1.0f / (s[0] + 1e-8f), 1.0f / (s[1] + 1e-8f), 1.0f / (s[2] + 1e-8f)};
+  float T[3][3], G[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      T[i][k] = R[i][k] * si[k];
+      G[i][k] = g_cov[9 * (int64_t)g + 3 * i + k];
+    }
+  float GR[3][3], gsi[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      float gt = 0.0f;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) gt += (G[a][j] + G[j][a]) * T[j][b];
+      GR[a][b] = gt * si[b];
+      gsi[b] += gt * R[a][b];
+    }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) g_scales[g3 + k] = -si[k] * si[k] * gsi[k];
+  const float gw = 2.0f * (x * (GR[2][1] - GR[1][2]) + y * (GR[0][2] - GR[2][0]) + z * (GR[1][0] - GR[0][1]));
+  const float gx = 2.0f * (y * (GR[0][1] + GR[1][0]) + z * (GR[0][2] + GR[2][0]) + w * (GR[2][1] - GR[1][2]) -
+                           2.0f * x * (GR[1][1] + GR[2][2]));
+  const float gy = 2.0f * (x * (GR[0][1] + GR[1][0]) + z * (GR[1][2] + GR[2][1]) + w * (GR[0][2] - GR[2][0]) -
+                           2.0f * y * (GR[0][0] + GR[2][2]));
+  const float gz = 2.0f * (x * (GR[0][2] + GR[2][0]) + y * (GR[1][2] + GR[2][1]) + w * (GR[1][0] - GR[0][1]) -
+                           2.0f * z * (GR[0][0] + GR[1][1]));
+  // Through q^ = q / (|q| + eps): dL/dq = g / (n+eps) - q (g . q) / (n (n+eps)^2); no second term at n = 0.
+  const float dot = gw * q4.x + gx * q4.y + gy * q4.z + gz * q4.w;
+  const float c2 = nr > 0.0f ? dot / (nr * nrm * nrm) : 0.0f;
+  float4 out;
+  out.x = gw / nrm - q4.x * c2;
+  out.y = gx / nrm - q4.y * c2;
+  out.z = gy / nrm - q4.z * c2;
+  out.w = gz / nrm - q4.w * c2;
+  *reinterpret_cast<float4*>(g_rot + 4 * (int64_t)g) = out;
+}
+
 // Exact evaluation counts for the roofline accounting: the forward kernel's
 // loop structure with the accumulation replaced by counting.
 __global__ __launch_bounds__(kTileThreads) void k_count(const GRec* __restrict__ rec,
@@ -1413,6 +1524,38 @@ int gstv_count_evaluations(const float* center_vox, const float* cov_inv, const 
   hipLaunchKernelGGL(k_count, dim3(tile_grid(geo)), dim3(kTileThreads), 0, s, (const GRec*)rec, tile_ranges, pair_ids,
                      geo, reinterpret_cast<unsigned long long*>(counts));
   GSTV_LAUNCH_CHECK("count");
+  return GSTV_OK;
+}
+
+int gstv_prologue(const float* positions, const float* scales, const float* rotations, int32_t n, int32_t d,
+                  int32_t h, int32_t w, float* center_vox, float* radius_vox, float* cov_inv, float* half_shape,
+                  void* stream) {
+  if (n < 0 || d <= 0 || h <= 0 || w <= 0 || half_shape == nullptr) return GSTV_ERR_INVALID_ARGUMENT;
+  if (n > 0 && (positions == nullptr || scales == nullptr || rotations == nullptr || center_vox == nullptr ||
+                radius_vox == nullptr || cov_inv == nullptr))
+    return GSTV_ERR_INVALID_ARGUMENT;
+  if (reinterpret_cast<uintptr_t>(rotations) % 16 != 0) return GSTV_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_prologue, dim3(ceil_div_i(std::max(n, 1), 256)), dim3(256), 0, s, positions, scales,
+                     rotations, n, (float)d, (float)h, (float)w, center_vox, radius_vox, cov_inv, half_shape);
+  GSTV_LAUNCH_CHECK("prologue");
+  return GSTV_OK;
+}
+
+int gstv_epilogue(const float* scales, const float* rotations, const float* grad_center, const float* grad_cov_inv,
+                  int32_t n, int32_t d, int32_t h, int32_t w, float* grad_positions, float* grad_scales,
+                  float* grad_rotations, void* stream) {
+  if (n < 0 || d <= 0 || h <= 0 || w <= 0) return GSTV_ERR_INVALID_ARGUMENT;
+  if (n == 0) return GSTV_OK;
+  if (scales == nullptr || rotations == nullptr || grad_center == nullptr || grad_cov_inv == nullptr ||
+      grad_positions == nullptr || grad_scales == nullptr || grad_rotations == nullptr)
+    return GSTV_ERR_INVALID_ARGUMENT;
+  if (reinterpret_cast<uintptr_t>(rotations) % 16 != 0 || reinterpret_cast<uintptr_t>(grad_rotations) % 16 != 0)
+    return GSTV_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_epilogue, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, scales, rotations, grad_center,
+                     grad_cov_inv, n, (float)d, (float)h, (float)w, grad_positions, grad_scales, grad_rotations);
+  GSTV_LAUNCH_CHECK("epilogue");
   return GSTV_OK;
 }
 

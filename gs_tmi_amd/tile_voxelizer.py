@@ -101,6 +101,48 @@ class _TileVoxelizeHIP(torch.autograd.Function):
         return g_c, g_m.view(-1, 3, 3), g_rr, g_ri, None, None, None, None, None, None, None
 
 
+class _TileVoxelizeFusedHIP(torch.autograd.Function):
+    """The whole TileVoxelizer call as one Function: fused HIP prologue
+    (gaussian/tile_voxelizer.py:281-290), GPU binning (:292-293), forward tile
+    kernel, and in backward the adjoint kernel plus the fused chain rule back to
+    positions, linear scales and quaternions.  Same math as composing
+    ``voxel_space_params`` with ``_TileVoxelizeHIP``; about thirty small torch
+    kernels fewer per step."""
+
+    @staticmethod
+    def forward(ctx, positions, scales, rotations, density_r, density_i, volume_shape, tile_size):
+        _load_hip_ext(strict=True)
+        d_size, h_size, w_size = volume_shape
+        scales_c = scales.detach().contiguous().float()
+        rot_c = rotations.detach().contiguous().float()
+        center, radius, cov, half_shape = _native.prologue(positions, scales_c, rot_c, volume_shape)
+        tile_ranges, pair_ids = build_tile_pairs(center, radius, volume_shape, tile_size)
+        rr = density_r.detach().contiguous().float()
+        ri = density_i.detach().contiguous().float()
+        vol_r, vol_i = _native.forward(center, cov, rr, ri, tile_ranges, pair_ids, half_shape, tile_size, d_size,
+                                       h_size, w_size)
+        ctx.save_for_backward(scales_c, rot_c, center, cov, rr, ri, tile_ranges, pair_ids, half_shape)
+        ctx.tile_size = tile_size
+        ctx.dims = (d_size, h_size, w_size)
+        return vol_r.view(d_size, h_size, w_size), vol_i.view(d_size, h_size, w_size)
+
+    @staticmethod
+    def backward(ctx, grad_vol_r, grad_vol_i):
+        scales_c, rot_c, center, cov, rr, ri, tile_ranges, pair_ids, half_shape = ctx.saved_tensors
+        d_size, h_size, w_size = ctx.dims
+        shape = (d_size, h_size, w_size)
+        if grad_vol_r is None:
+            grad_vol_r = torch.zeros(shape, dtype=torch.float32, device=center.device)
+        if grad_vol_i is None:
+            grad_vol_i = torch.zeros(shape, dtype=torch.float32, device=center.device)
+        g_c, g_m, g_rr, g_ri = _native.backward(
+            center, cov, rr, ri, tile_ranges, pair_ids, half_shape,
+            grad_vol_r.contiguous().float().view(-1), grad_vol_i.contiguous().float().view(-1),
+            ctx.tile_size, d_size, h_size, w_size)
+        g_p, g_s, g_q = _native.epilogue(scales_c, rot_c, g_c, g_m, shape)
+        return g_p, g_s, g_q, g_rr, g_ri, None, None
+
+
 def voxel_space_params(positions, scales, rotations, volume_shape):
     """The prologue of gaussian/tile_voxelizer.py:281-290, op for op.
 
@@ -150,6 +192,12 @@ class TileVoxelizer(nn.Module):
     def forward(self, positions: torch.Tensor, scales: torch.Tensor, rotations: torch.Tensor,
                 density: torch.Tensor, chunk_size: int = 4096):
         del chunk_size
+        return _TileVoxelizeFusedHIP.apply(positions, scales, rotations, density.real, density.imag,
+                                           self.volume_shape, self.tile_size)
+
+    def forward_unfused(self, positions, scales, rotations, density):
+        """The same call composed as the reference composes it: torch prologue,
+        binning, and the 11-argument Function (kept for Function-level parity)."""
         d_size, h_size, w_size = self.volume_shape
         center_vox, radius_vox, cov_inv, half_shape = voxel_space_params(positions, scales, rotations,
                                                                          self.volume_shape)

@@ -139,6 +139,28 @@ int gstv_backward(const float* center_vox, const float* cov_inv,
                   float* grad_rho_r, float* grad_rho_i,
                   void* workspace, size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------ prologue / epilogue --
+ * Replaces the torch prologue of TileVoxelizer.forward (gaussian/tile_voxelizer.py:281-290,
+ * with _quaternion_to_rotation_matrix :77-84) and torch autograd through it.
+ *   positions [n][3] in [-1, 1], scales [n][3] (linear, the caller's exp(log-scale)),
+ *   rotations [n][4] (w, x, y, z; 16-byte aligned).
+ * gstv_prologue writes center_vox [n][3] and radius_vox [n] bit-exact with the
+ * reference's fp32 operation order, cov_inv [n][3][3] = T T^T with
+ * T = R(q / (|q| + 1e-8)) * (1 / (s + 1e-8)) in fp32, and half_shape [3].
+ * gstv_epilogue maps grad_center [n][3] and grad_cov_inv [n][3][3] to
+ * grad_positions [n][3], grad_scales [n][3] and grad_rotations [n][4]
+ * (16-byte aligned); the radius carries no gradient, as in the reference. */
+int gstv_prologue(const float* positions, const float* scales, const float* rotations, int32_t n,
+                  int32_t d, int32_t h, int32_t w,
+                  float* center_vox, float* radius_vox, float* cov_inv, float* half_shape,
+                  void* stream);
+
+int gstv_epilogue(const float* scales, const float* rotations,
+                  const float* grad_center, const float* grad_cov_inv, int32_t n,
+                  int32_t d, int32_t h, int32_t w,
+                  float* grad_positions, float* grad_scales, float* grad_rotations,
+                  void* stream);
+
 /* ------------------------------------------------------------ measurement --
  * No counterpart in the reference, which has no profiling hooks on this
  * path; these serve the benchmark's roofline accounting.

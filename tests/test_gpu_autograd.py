@@ -134,3 +134,32 @@ def test_full_size_properties():
     rel = abs(float(lhs - rhs)) / max(abs(float(lhs)), 1e-30)
     print(f"  <J rho, G> = {float(lhs):.6e}, <rho, J^T G> = {float(rhs):.6e}, rel diff {rel:.2e}")
     assert rel < 1e-5
+
+
+@pytest.mark.parametrize("name", ["adjoint_voxel", "partial_tiles", "tile5", "outside"])
+def test_fused_call_matches_unfused_composition(name):
+    """TileVoxelizer.forward (fused HIP prologue and epilogue) against the
+    reference's composition: torch prologue + binning + 11-argument Function."""
+    from gs_tmi_amd import TileVoxelizer
+
+    c = cases.adjoint_check_case(0, "voxel") if name == "adjoint_voxel" else cases.edge_case(name)
+    shape, tile = c["shape"], c["tile"]
+    _announce(f"fused vs unfused {name} shape={shape} tile={tile}")
+    vox = TileVoxelizer(shape, tile_size=tile, device=DEV)
+    rng = np.random.default_rng(4)
+    g_r = torch.from_numpy(rng.normal(size=shape).astype(np.float32)).to(DEV)
+    g_i = torch.from_numpy(rng.normal(size=shape).astype(np.float32)).to(DEV)
+    out = {}
+    for mode in ("fused", "unfused"):
+        leaves = [torch.from_numpy(np.ascontiguousarray(a)).to(DEV).requires_grad_()
+                  for a in (c["pos"], np.log(c["scales"]), c["rot"], c["rr"], c["ri"])]
+        pos, log_s, rot, dr, di = leaves
+        fn = vox.forward if mode == "fused" else vox.forward_unfused
+        vr, vi = fn(pos, torch.exp(log_s), rot, torch.complex(dr, di))
+        ((vr * g_r).sum() + (vi * g_i).sum()).backward()
+        out[mode] = [vr.detach(), vi.detach()] + [t.grad for t in leaves]
+    names = ["vol_r", "vol_i", "pos", "log_s", "rot", "dr", "di"]
+    for nm, a, b in zip(names, out["fused"], out["unfused"]):
+        e = cases.rel_l2(a.cpu().numpy(), b.cpu().numpy())
+        print(f"  {nm}: fused vs unfused rel L2 {e:.2e}")
+        assert e < 2e-6, (nm, e)

@@ -62,6 +62,17 @@ def test_prologue_and_binning_bit_exact(name, make):
     r_exp, p_exp = vo.bin_tiles(c_ref, r_ref, c["shape"], c["tile"])
     np.testing.assert_array_equal(ranges, r_exp)
     np.testing.assert_array_equal(pids, p_exp)
+    # The fused HIP prologue gives the same fp32 centre and radius, and cov_inv
+    # within fp32 rounding of the FP64 closed form.
+    from gs_tmi_amd import _native
+
+    t = lambda a: torch.from_numpy(a).to(DEV)  # noqa: E731
+    fc, fr, fcov, fhs = _native.prologue(t(c["pos"]), t(c["scales"]), t(c["rot"]), c["shape"])
+    np.testing.assert_array_equal(fc.cpu().numpy(), c_ref)
+    np.testing.assert_array_equal(fr.cpu().numpy(), r_ref)
+    np.testing.assert_array_equal(fhs.cpu().numpy(), np.asarray(c["shape"], np.float32) * np.float32(0.5))
+    _, _, cov64, _ = vo.prologue(c["pos"], c["scales"], c["rot"], c["shape"])
+    assert cases.rel_l2(fcov.cpu().numpy(), cov64) < 2e-6
 
 
 def _function_inputs(c):
