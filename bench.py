@@ -21,6 +21,7 @@ by the sample's share of the evaluations.
 from __future__ import annotations
 
 import argparse
+import contextlib
 import json
 import os
 import statistics
@@ -141,7 +142,8 @@ def main():
     dev = torch.device("cuda:0")
     log("[bench] building inputs")
     params, grad_r, grad_i = make_inputs(dev)
-    vox = TileVoxelizer(SHAPE, tile_size=TILE, device=str(dev))
+    with contextlib.redirect_stdout(sys.stderr):  # keep stdout to the one JSON line
+        vox = TileVoxelizer(SHAPE, tile_size=TILE, device=str(dev))
 
     def step():
         for p in params.values():

@@ -362,28 +362,41 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_hist(const int32_t* __r
   if (threadIdx.x < (1 << bits)) hist[threadIdx.x * nblocks + blockIdx.x] = sh_h[threadIdx.x];
 }
 
-// Radix downsweep: stable scatter.  Keys of a block are visited in 16 rounds
-// of 256 in index order; inside a round a wave ranks equal digits with a
-// ballot multisplit, and per-wave digit counts give the order across waves.
+// Radix downsweep: stable scatter with coalesced writes.  Each wave owns a
+// contiguous quarter of the block's keys and ranks equal digits with a ballot
+// multisplit against a per-wave running count (no block barriers per round);
+// the per-digit wave prefixes then give every key its position in a
+// block-local, digit-sorted copy in LDS, which is written out so that
+// consecutive threads store consecutive addresses of a digit's run.
 __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
-    const int32_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in, int total, int shift,
-    int bits, int nblocks, const int32_t* __restrict__ hist_scanned, int32_t* __restrict__ keys_out,
+    const int32_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in, int total, int shift, int bits,
+    int nblocks, const int32_t* __restrict__ hist_scanned, int32_t* __restrict__ keys_out,
     int32_t* __restrict__ vals_out) {
   constexpr int kDigits = 1 << kRadixBits;
   constexpr int kW = kRadixThreads / 64;
-  __shared__ int32_t sh_base[kDigits];
+  constexpr int kPerWave = kRadixTile / kW;
+  constexpr int kRounds = kPerWave / 64;
+  __shared__ int32_t sh_key[kRadixTile];
+  __shared__ int32_t sh_val[kRadixTile];
   __shared__ int32_t sh_wcnt[kW][kDigits];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __shared__ int32_t sh_local[kDigits];
+  __shared__ int32_t sh_base[kDigits];
+  __shared__ int64_t sh_scan[kRadixThreads / 64];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wid = tid >> 6;
   const int ndig = 1 << bits;
   const int mask = ndig - 1;
-  sh_base[threadIdx.x] = threadIdx.x < ndig ? hist_scanned[threadIdx.x * nblocks + blockIdx.x] : 0;
 #pragma unroll
-  for (int w = 0; w < kW; ++w) sh_wcnt[w][threadIdx.x] = 0;
+  for (int w = 0; w < kW; ++w) sh_wcnt[w][tid] = 0;
+  sh_base[tid] = tid < ndig ? hist_scanned[tid * nblocks + blockIdx.x] : 0;
   __syncthreads();
+
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int base = blockIdx.x * kRadixTile;
-  for (int r = 0; r < kRadixRounds; ++r) {
-    const int idx = base + r * kRadixThreads + threadIdx.x;
+  const int wbase = blockIdx.x * kRadixTile + wid * kPerWave;
+  int32_t kk[kRounds], vv[kRounds], rk[kRounds];
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int idx = wbase + r * 64 + lane;
     const bool valid = idx < total;
     const int32_t key = valid ? keys_in[idx] : 0;
     const int32_t val = valid ? vals_in[idx] : 0;
@@ -395,26 +408,44 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
       peers &= bit ? bb : ~bb;
     }
     const int wrank = __popcll(peers & lt_mask);
-    if (valid && wrank == 0) sh_wcnt[wid][dig] = __popcll(peers);
-    __syncthreads();
-    if (valid) {
-      int pos = sh_base[dig] + wrank;
+    const int prev = sh_wcnt[wid][dig];  // read by the whole wave before its leader updates it
+    if (valid && wrank == 0) sh_wcnt[wid][dig] = prev + __popcll(peers);
+    kk[r] = key;
+    vv[r] = val;
+    rk[r] = valid ? prev + wrank : -1;
+  }
+  __syncthreads();
+  {
+    // Per digit (thread tid = digit): wave prefixes and the block total, then
+    // the block-local exclusive offsets across digits.
+    int run = 0;
 #pragma unroll
-      for (int w = 0; w < kW; ++w) pos += (w < wid) ? sh_wcnt[w][dig] : 0;
-      keys_out[pos] = key;
-      vals_out[pos] = val;
+    for (int w = 0; w < kW; ++w) {
+      const int c = sh_wcnt[w][tid];
+      sh_wcnt[w][tid] = run;
+      run += c;
     }
-    __syncthreads();
-    {
-      int add = 0;
+    int64_t tot;
+    sh_local[tid] = (int32_t)block_exclusive_scan(run, sh_scan, &tot);
+  }
+  __syncthreads();
 #pragma unroll
-      for (int w = 0; w < kW; ++w) {
-        add += sh_wcnt[w][threadIdx.x];
-        sh_wcnt[w][threadIdx.x] = 0;
-      }
-      sh_base[threadIdx.x] += add;
+  for (int r = 0; r < kRounds; ++r) {
+    if (rk[r] >= 0) {
+      const int dig = (kk[r] >> shift) & mask;
+      const int lpos = sh_local[dig] + sh_wcnt[wid][dig] + rk[r];
+      sh_key[lpos] = kk[r];
+      sh_val[lpos] = vv[r];
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  const int nblk = min(kRadixTile, total - (int)blockIdx.x * kRadixTile);
+  for (int i = tid; i < nblk; i += kRadixThreads) {
+    const int32_t key = sh_key[i];
+    const int dig = (key >> shift) & mask;
+    const int out = sh_base[dig] + (i - sh_local[dig]);
+    keys_out[out] = key;
+    vals_out[out] = sh_val[i];
   }
 }
 
@@ -1366,6 +1397,7 @@ int gstv_bin_fill(const float* center_vox, const float* radius_vox, int32_t n, i
 
   const int passes = radix_passes(num_tiles);
   const int bits_total = key_bits(num_tiles);
+  const int width = passes > 0 ? (bits_total + passes - 1) / passes : 0;  // balanced digit widths
   // With no radix pass (a single tile) the expansion order is already final.
   int32_t* first_vals = passes == 0 ? pair_ids : va;
   hipLaunchKernelGGL(k_bin_expand, dim3(ceil_div_i(total, kExpandTile)), dim3(kExpandThreads), 0, s, center_vox,
@@ -1376,8 +1408,8 @@ int gstv_bin_fill(const float* center_vox, const float* radius_vox, int32_t n, i
   int32_t* kin = ka;
   int32_t* vin = va;
   for (int p = 0; p < passes; ++p) {
-    const int shift = p * kRadixBits;
-    const int bits = std::min(kRadixBits, bits_total - shift);
+    const int shift = p * width;
+    const int bits = std::min(width, bits_total - shift);
     int32_t* kout = (kin == ka) ? kb : ka;
     int32_t* vout = (p == passes - 1) ? pair_ids : ((vin == va) ? vb : va);
     hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kRadixThreads), 0, s, kin, total, shift, bits, nblk, hist);

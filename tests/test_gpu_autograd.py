@@ -125,15 +125,27 @@ def test_full_size_properties():
     print(f"  sampled-tile forward rel L2 {e:.3e}")
     assert e <= 1e-5
 
-    # Adjoint dot product: the forward is linear in (rho_r, rho_i).
-    g_r = torch.randn(d * h * w, device=DEV)
-    g_i = torch.randn(d * h * w, device=DEV)
-    _, _, grr, gri = _native.backward(center, cov, rr, ri, ranges, pids, hs, g_r, g_i, 8, d, h, w)
+    # Adjoint dot product: the forward is linear in (rho_r, rho_i).  The
+    # difference is measured against |vol| |G| (Cauchy-Schwarz), which does not
+    # blow up when <vol, G> happens to be small.
+    gen = torch.Generator(device="cpu").manual_seed(21)
+    g_r = torch.randn(d * h * w, generator=gen).to(DEV)
+    g_i = torch.randn(d * h * w, generator=gen).to(DEV)
+    b1 = _native.backward(center, cov, rr, ri, ranges, pids, hs, g_r, g_i, 8, d, h, w)
+    _, _, grr, gri = b1
     lhs = (v1[0].double() * g_r.double()).sum() + (v1[1].double() * g_i.double()).sum()
     rhs = (rr.double() * grr.double()).sum() + (ri.double() * gri.double()).sum()
-    rel = abs(float(lhs - rhs)) / max(abs(float(lhs)), 1e-30)
-    print(f"  <J rho, G> = {float(lhs):.6e}, <rho, J^T G> = {float(rhs):.6e}, rel diff {rel:.2e}")
-    assert rel < 1e-5
+    scale = float(torch.sqrt((v1[0].double() ** 2).sum() + (v1[1].double() ** 2).sum()) *
+                  torch.sqrt((g_r.double() ** 2).sum() + (g_i.double() ** 2).sum()))
+    rel = abs(float(lhs - rhs)) / scale
+    print(f"  <J rho, G> = {float(lhs):.6e}, <rho, J^T G> = {float(rhs):.6e}, diff / (|vol||G|) {rel:.2e}")
+    assert rel < 1e-7
+    # Run to run, the backward differs only by the order of fp32 atomic adds.
+    b2 = _native.backward(center, cov, rr, ri, ranges, pids, hs, g_r, g_i, 8, d, h, w)
+    for name, x, y in zip(["center", "cov_inv", "rho_r", "rho_i"], b1, b2):
+        e = cases.rel_l2(x.cpu().numpy(), y.cpu().numpy())
+        print(f"  backward run-to-run {name}: rel L2 {e:.2e}")
+        assert e < 1e-6, (name, e)
 
 
 @pytest.mark.parametrize("name", ["adjoint_voxel", "partial_tiles", "tile5", "outside"])
