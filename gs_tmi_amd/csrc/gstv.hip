@@ -537,7 +537,7 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ center, 
 
 struct VoxGeom {
   int D, H, W, tile;
-  int nTH, nTW, num_tiles;
+  int nTD, nTH, nTW, num_tiles;
   int nsb;     // sub-bricks per tile axis, ceil(tile / 4)
   int chunks;  // blocks per tile, ceil(nsb^3 / 8)
   int blocks;  // num_tiles * chunks
@@ -556,6 +556,7 @@ struct BlockCtx {
   int e0, e1, e2;  // valid tile extent per axis
   int s0, s1, s2;  // this wave's sub-brick origin, tile-local
   int n0, n1, n2;  // valid voxels of the sub-brick per axis (<= 4)
+  bool valid;      // the block maps to a tile
   bool active;     // the wave owns at least one valid voxel
   int p_begin, p_end;
 };
@@ -565,12 +566,12 @@ __device__ __forceinline__ BlockCtx make_ctx(int logical, const VoxGeom& geo, co
   BlockCtx b;
   const int tile = logical / geo.chunks;
   const int chunk = logical - tile * geo.chunks;
-  b.tile = tile;
   const int nthw = geo.nTH * geo.nTW;
   const int td = tile / nthw;
-  const int rem = tile - td * nthw;
-  const int th = rem / geo.nTW;
-  const int tw = rem - th * geo.nTW;
+  const int th = (tile - td * nthw) / geo.nTW;
+  const int tw = tile - td * nthw - th * geo.nTW;
+  b.valid = tile < geo.num_tiles;
+  b.tile = tile;
   b.o0 = td * geo.tile;
   b.o1 = th * geo.tile;
   b.o2 = tw * geo.tile;
@@ -589,8 +590,8 @@ __device__ __forceinline__ BlockCtx make_ctx(int logical, const VoxGeom& geo, co
   b.n1 = min(kSub, b.e1 - b.s1);
   b.n2 = min(kSub, b.e2 - b.s2);
   b.active = (sb < nsb * nsb * nsb) && b.n0 > 0 && b.n1 > 0 && b.n2 > 0;
-  b.p_begin = ranges[2 * tile];
-  b.p_end = ranges[2 * tile + 1];
+  b.p_begin = b.valid ? ranges[2 * tile] : 0;
+  b.p_end = b.valid ? ranges[2 * tile + 1] : 0;
   return b;
 }
 
@@ -679,6 +680,39 @@ __device__ __forceinline__ GaussLane load_gauss(const Stage& sh, int k, const Bl
   return g;
 }
 
+// Evaluates the wave's compacted Gaussians 16 at a time over its sub-brick.
+// kPartial: the sub-brick may extend past the tile's valid voxels (edge or
+// non-multiple-of-4 tiles); full tiles take the branch-free path.
+template <bool kPartial>
+__device__ __forceinline__ void fwd_groups(const Stage& sh, int nhit, const BlockCtx& b, int wave, int gl, int xi,
+                                           bool x_ok, float (&acc_r)[kSub * kSub], float (&acc_i)[kSub * kSub]) {
+  for (int gb = 0; gb < nhit; gb += 16) {
+    const int idx = gb + gl;
+    const bool g_ok = (idx < nhit) && x_ok;
+    const int k = sh.list[wave][idx < nhit ? idx : gb];
+    const GaussLane g = load_gauss(sh, k, b, xi);
+#pragma unroll
+    for (int z = 0; z < kSub; ++z) {
+      if (!kPartial || z < b.n0) {
+        const float u0 = (float)(b.o0 + b.s0 + z) - g.c0;
+        const float A = fmaf(fmaf(g.a00, u0, g.k1), u0, g.k0);
+        const float B = fmaf(g.b01, u0, g.k2);
+#pragma unroll
+        for (int y = 0; y < kSub; ++y) {
+          if (!kPartial || y < b.n1) {
+            const float u1 = g.u1[y];
+            const float q = fmaf(fmaf(g.a11, u1, B), u1, A);
+            const bool ok = g_ok && (q >= kThr);
+            const float w = ok ? __builtin_amdgcn_exp2f(q) : 0.0f;
+            acc_r[z * kSub + y] = fmaf(w, g.rr, acc_r[z * kSub + y]);
+            acc_i[z * kSub + y] = fmaf(w, g.ri, acc_i[z * kSub + y]);
+          }
+        }
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(kTileThreads) void k_forward(const GRec* __restrict__ rec,
                                                           const int32_t* __restrict__ ranges,
                                                           const int32_t* __restrict__ pair_ids, VoxGeom geo,
@@ -686,10 +720,13 @@ __global__ __launch_bounds__(kTileThreads) void k_forward(const GRec* __restrict
   __shared__ Stage sh;
   const int logical = xcd_remap(blockIdx.x, gridDim.x);
   if (logical >= geo.blocks) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int gl = lane & 15, xi = lane >> 4;
   const BlockCtx b = make_ctx(logical, geo, ranges, wave);
+  if (!b.valid) return;
   const bool x_ok = xi < b.n2;
+  const bool full = b.e0 == geo.tile && b.e1 == geo.tile && b.e2 == geo.tile && (geo.tile & 3) == 0;
 
   float acc_r[kSub * kSub], acc_i[kSub * kSub];
 #pragma unroll
@@ -705,31 +742,10 @@ __global__ __launch_bounds__(kTileThreads) void k_forward(const GRec* __restrict
     __syncthreads();
     if (!b.active) continue;
     const int nhit = build_list(sh, cnt, b, wave, lane);
-    for (int gb = 0; gb < nhit; gb += 16) {
-      const int idx = gb + gl;
-      const bool g_ok = (idx < nhit) && x_ok;
-      const int k = sh.list[wave][idx < nhit ? idx : gb];
-      const GaussLane g = load_gauss(sh, k, b, xi);
-#pragma unroll
-      for (int z = 0; z < kSub; ++z) {
-        if (z < b.n0) {
-          const float u0 = (float)(b.o0 + b.s0 + z) - g.c0;
-          const float A = fmaf(fmaf(g.a00, u0, g.k1), u0, g.k0);
-          const float B = fmaf(g.b01, u0, g.k2);
-#pragma unroll
-          for (int y = 0; y < kSub; ++y) {
-            if (y < b.n1) {
-              const float u1 = g.u1[y];
-              const float q = fmaf(fmaf(g.a11, u1, B), u1, A);
-              const bool ok = g_ok && (q >= kThr);
-              const float w = ok ? __builtin_amdgcn_exp2f(q) : 0.0f;
-              acc_r[z * kSub + y] = fmaf(w, g.rr, acc_r[z * kSub + y]);
-              acc_i[z * kSub + y] = fmaf(w, g.ri, acc_i[z * kSub + y]);
-            }
-          }
-        }
-      }
-    }
+    if (full)
+      fwd_groups<false>(sh, nhit, b, wave, gl, xi, x_ok, acc_r, acc_i);
+    else
+      fwd_groups<true>(sh, nhit, b, wave, gl, xi, x_ok, acc_r, acc_i);
   }
 
   // Sum the 16 Gaussian slots of each W column (a row of 16 lanes).
@@ -884,6 +900,7 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
   const int T = TT > 0 ? TT : geo.nsb * kSub;
   // One block per tile: the context's wave-specific fields are unused here.
   const BlockCtx b = make_ctx(logical, geo, ranges, 0);
+  if (!b.valid) return;
   const bool partial = b.e0 < T || b.e1 < T || b.e2 < T;
 
   for (int i = tid; i < T * T * T; i += kTileThreads) {
@@ -1173,6 +1190,7 @@ __global__ __launch_bounds__(kTileThreads) void k_count(const GRec* __restrict__
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int gl = lane & 15, xi = lane >> 4;
   const BlockCtx b = make_ctx(logical, geo, ranges, wave);
+  if (!b.valid) return;
   const bool x_ok = xi < b.n2;
   if (threadIdx.x == 0 && logical % geo.chunks == 0) {
     const unsigned long long pairs = (unsigned long long)(b.p_end - b.p_begin);
@@ -1250,6 +1268,7 @@ VoxGeom make_vox_geom(int d, int h, int w, int tile, const int nt[3], int num_ti
   g.H = h;
   g.W = w;
   g.tile = tile;
+  g.nTD = nt[0];
   g.nTH = nt[1];
   g.nTW = nt[2];
   g.num_tiles = num_tiles;
