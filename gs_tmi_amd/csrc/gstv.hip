@@ -735,10 +735,16 @@ __global__ __launch_bounds__(kTileThreads) void k_forward(const GRec* __restrict
     acc_i[i] = 0.0f;
   }
 
+  // Records of the next batch are loaded into registers while the current
+  // batch is evaluated (loads stay in flight across the LDS barriers).
+  GRec nxt = {};
+  if ((int)threadIdx.x < min(kBatch, b.p_end - b.p_begin)) nxt = rec[pair_ids[b.p_begin + threadIdx.x]];
   for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
     const int cnt = min(kBatch, b.p_end - bs);
     __syncthreads();
-    if ((int)threadIdx.x < cnt) stage_pair(sh, threadIdx.x, rec[pair_ids[bs + threadIdx.x]], b);
+    if ((int)threadIdx.x < cnt) stage_pair(sh, threadIdx.x, nxt, b);
+    const int bn = bs + kBatch;
+    if (bn < b.p_end && (int)threadIdx.x < min(kBatch, b.p_end - bn)) nxt = rec[pair_ids[bn + threadIdx.x]];
     __syncthreads();
     if (!b.active) continue;
     const int nhit = build_list(sh, cnt, b, wave, lane);
@@ -914,12 +920,16 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
   }
   if (tid <= kMaxClass) sh.cls_cnt[tid] = 0;
 
+  // Records of the next batch are loaded into registers while the current
+  // batch is evaluated (loads stay in flight across the LDS barriers).
+  GRec nxt = {};
+  if (tid < min(kBatch, b.p_end - b.p_begin)) nxt = rec[pair_ids[b.p_begin + tid]];
   for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
     const int cnt = min(kBatch, b.p_end - bs);
     __syncthreads();
     // Stage: parameters, sub-brick box and class of every pair of the batch.
     if (tid < cnt) {
-      const GRec r = rec[pair_ids[bs + tid]];
+      const GRec r = nxt;
       sh.c0[tid] = r.c0;
       sh.c1[tid] = r.c1;
       sh.c2[tid] = r.c2;
@@ -948,6 +958,10 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
         sh.slot[tid] = (uint16_t)atomicAdd(&sh.cls_cnt[cls], 1);
       }
       sh.cls[tid] = (uint8_t)cls;
+    }
+    {
+      const int bn = bs + kBatch;
+      if (bn < b.p_end && tid < min(kBatch, b.p_end - bn)) nxt = rec[pair_ids[bn + tid]];
     }
     __syncthreads();
     // Class offsets and group offsets, classes in descending order.
