@@ -129,6 +129,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after timing, copy a buffer of this many MiB once (a known-size kernel that "
+                         "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the multi-GPU design is "
@@ -191,6 +194,15 @@ def main():
     t_total = time.perf_counter() - t_start
     prof = _native.profile_end()
     ms_per_step = t_total * 1e3 / args.steps
+
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     fwd_ms = prof["forward"][0] / max(prof["forward"][1], 1)
     bwd_ms = prof["backward"][0] / max(prof["backward"][1], 1)
