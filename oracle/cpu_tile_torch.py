@@ -75,3 +75,37 @@ def tile_forward_backward(center_vox, cov_inv, density_r, density_i, half_shape,
         loss = (val.real * g_r[flat]).sum() + (val.imag * g_i[flat]).sum()
         loss.backward()
     return c.grad, m.grad, rr.grad, ri.grad
+
+
+class CpuTileVoxelizer:
+    """The reference's CPU voxelizer call (gaussian/tile_voxelizer.py:279-322 with
+    `_tile_forward_pytorch`), restated: torch prologue, NumPy binning of the
+    oracle, per-tile fp32 forward with torch autograd.  Returns a complex64
+    [D, H, W] volume like the reference's CPU path.  Test infrastructure: it lets
+    the reconstruction harness run the reference's CPU path for parity runs."""
+
+    def __init__(self, volume_shape, tile_size=8):
+        self.volume_shape = tuple(volume_shape)
+        self.tile_size = tile_size
+
+    def __call__(self, positions, scales, rotations, density):
+        from oracle import voxelizer_oracle as vo
+
+        d, h, w = self.volume_shape
+        shape_t = torch.tensor([d, h, w], dtype=torch.float32)
+        half_shape = shape_t * 0.5
+        center = (positions + 1.0) * 0.5 * shape_t - 0.5
+        radius = (scales * half_shape).max(dim=-1)[0] * 3.0
+        q = rotations / (rotations.norm(dim=-1, keepdim=True) + 1e-8)
+        qw, qx, qy, qz = q.unbind(-1)
+        rot = torch.stack([
+            torch.stack([1 - 2 * qy * qy - 2 * qz * qz, 2 * qx * qy - 2 * qz * qw, 2 * qx * qz + 2 * qy * qw], -1),
+            torch.stack([2 * qx * qy + 2 * qz * qw, 1 - 2 * qx * qx - 2 * qz * qz, 2 * qy * qz - 2 * qx * qw], -1),
+            torch.stack([2 * qx * qz - 2 * qy * qw, 2 * qy * qz + 2 * qx * qw, 1 - 2 * qx * qx - 2 * qy * qy], -1),
+        ], -2)
+        t = rot * torch.reciprocal(scales + 1e-8).unsqueeze(1)
+        cov = (t.unsqueeze(2) * t.unsqueeze(1)).sum(-1)
+        ranges, pids = vo.bin_tiles(center.detach().numpy(), radius.detach().numpy(), self.volume_shape,
+                                    self.tile_size)
+        return tile_forward(center, cov, density, half_shape, torch.from_numpy(ranges), torch.from_numpy(pids),
+                            self.volume_shape, self.tile_size)
