@@ -16,6 +16,11 @@ duration, timed with HIP events recorded by the library on the launch stream
 during the timed steps.  The CPU baseline is a PyTorch CPU restatement of the
 reference's per-tile voxelizer, timed on a bounded sample of tiles and scaled
 by the sample's share of the evaluations.
+
+The second half of the metric, time to NMSE 1e-2 of the learned- against the
+plain-initialised reconstruction, is the "learned_init" object: both runs of
+the config-4 reconstruction (256x256x128, 16 coils, R=8) with the same Adam
+settings, each stopped at the target.
 """
 
 from __future__ import annotations
@@ -32,7 +37,7 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-METRIC = "voxelizer fwd+bwd ms (1M Gauss,256²×128)"
+METRIC = "voxelizer fwd+bwd ms (1M Gauss,256²×128); time-to-NMSE 1e-2 learned vs plain"
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector, published (packed FP32)
 SHAPE = (256, 256, 128)
 N_GAUSS = 1_000_000
@@ -112,6 +117,47 @@ def cpu_baseline(params, budget_s=15.0):
     return ms, note, cores
 
 
+def learned_init_leg(dev, max_steps=1500, eval_every=25, target=1e-2):
+    """Time to NMSE `target` of the config-4 reconstruction (256x256x128, 60
+    ellipsoids, 16 coils, Poisson-disc R=8 with a 24x24 calibration centre,
+    noise 1e-3) from the learned and from the plain initialisation, same Adam
+    settings (3x default learning rates, no densification; plain = 600k
+    Gaussians at above-threshold voxels).  Wall time = initialisation +
+    training, evaluations excluded; each run stops at the target."""
+    import torch
+
+    from gs_tmi_amd import TileVoxelizer
+    from gs_tmi_amd.learned.init import LearnedInitialiser, initialise
+    from gs_tmi_amd.learned.pipeline import schedules
+    from gs_tmi_amd.recon.data import make_problem, poisson_disc_mask
+    from gs_tmi_amd.recon.trainer import Reconstructor
+
+    shape = (256, 256, 128)
+    problem = make_problem(shape, 60, 16, poisson_disc_mask((256, 128), 8, 24, 0), 0, noise_sigma=1e-3, device=dev,
+                           phantom_on_device=True)
+    with contextlib.redirect_stdout(sys.stderr):
+        vox = TileVoxelizer(shape, tile_size=8, device=str(dev))
+    n_plain, cfg = schedules()["tuned"]
+    out = {"workload": "config-4 reconstruction 256x256x128, 16 coils, R=8 Poisson-disc, noise 1e-3",
+           "schedule": "Adam, 3x default learning rates, no densification", "target_nmse": target,
+           "max_steps": max_steps}
+    for kind, init in (("learned", LearnedInitialiser(device=dev)), ("plain", None)):
+        log(f"[bench] learned-init leg: {kind}")
+        g, info = initialise(problem, vox, init, plain_points=n_plain)
+        hist = Reconstructor(problem, g, vox, cfg).run(max_steps, eval_every=eval_every, target_nmse=target)
+        t = hist.time_to("nmse", target)
+        last = hist.rows[-1]
+        out[kind] = {"used": info["used"], "gaussians": g.num_points, "init_s": round(info["init_s"], 3),
+                     "time_to_target_s": None if t is None else round(info["init_s"] + t, 3),
+                     "steps": last["step"], "nmse": last["nmse"], "psnr": last["psnr"],
+                     "init_nmse": hist.rows[0]["nmse"]}
+        del g, hist
+        torch.cuda.empty_cache()
+    a, b = out["learned"]["time_to_target_s"], out["plain"]["time_to_target_s"]
+    out["ratio"] = None if (a is None or b is None) else round(a / b, 4)
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -129,6 +175,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain time-to-NMSE leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after timing, copy a buffer of this many MiB once (a known-size kernel that "
                          "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -218,6 +265,12 @@ def main():
         cpu_ms, note, cores = cpu_baseline(params, args.cpu_budget)
         cpu = {"value": round(cpu_ms, 1), "unit": "ms", "cores": cores, "kind": "port", "sample": note}
 
+    learned = None
+    if not args.no_learned:
+        del params, grad_r, grad_i
+        torch.cuda.empty_cache()
+        learned = learned_init_leg(dev)
+
     out = {
         "metric": METRIC,
         "value": round(ms_per_step, 4),
@@ -238,6 +291,7 @@ def main():
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": None},
         "cpu_baseline": cpu,
+        "learned_init": learned,
         "detail": {
             "median_step_ms": round(statistics.median(per_step), 4),
             "forward_kernel_ms": round(fwd_ms, 4),

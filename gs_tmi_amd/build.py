@@ -35,6 +35,7 @@ HIPCC_FLAGS = [
 # (library name, sources) pairs.
 LIBRARIES = [
     ("libgstv.so", ["gstv.hip"]),
+    ("libgstv_mlp.so", ["mlp.hip"]),
 ]
 
 

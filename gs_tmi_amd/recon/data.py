@@ -54,6 +54,33 @@ def ellipsoid_phantom(shape, n_ellipsoids, seed, mag_range=(0.2, 1.0)) -> np.nda
     return (mag * np.exp(1j * phase)).astype(np.complex64)
 
 
+def ellipsoid_phantom_torch(shape, n_ellipsoids, seed, mag_range=(0.2, 1.0), device="cpu") -> torch.Tensor:
+    """``ellipsoid_phantom`` evaluated with torch (float64) on `device` from the
+    same random draws; the inside test can differ from the NumPy path only for
+    points within round-off of an ellipsoid surface.  Returns complex64 [D, H, W]."""
+    rng = np.random.default_rng(seed)
+    axes1d = [torch.linspace(-1.0, 1.0, s, dtype=torch.float64, device=device) for s in shape]
+    z, y, x = torch.meshgrid(*axes1d, indexing="ij")
+    pts = torch.stack([z, y, x], dim=-1).reshape(-1, 3)
+    mag = torch.zeros(pts.shape[0], dtype=torch.float64, device=device)
+    for _ in range(n_ellipsoids):
+        centre = rng.uniform(-0.55, 0.55, size=3)
+        axes = rng.uniform(0.06, 0.35, size=3)
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        w, a, b, c = q
+        rot = np.array([[1 - 2 * (b * b + c * c), 2 * (a * b - c * w), 2 * (a * c + b * w)],
+                        [2 * (a * b + c * w), 1 - 2 * (a * a + c * c), 2 * (b * c - a * w)],
+                        [2 * (a * c - b * w), 2 * (b * c + a * w), 1 - 2 * (a * a + b * b)]])
+        local = (pts - torch.from_numpy(centre).to(device)) @ torch.from_numpy(rot).to(device)
+        inside = torch.sum((local / torch.from_numpy(axes).to(device)) ** 2, dim=1) <= 1.0
+        mag[inside] += rng.uniform(*mag_range)
+    mag = torch.clamp(mag, max=1.5).reshape(shape)
+    g = rng.uniform(-math.pi / 2, math.pi / 2, size=3)
+    phase = g[0] * z + g[1] * y + g[2] * x + rng.uniform(-math.pi, math.pi)
+    return torch.polar(mag, phase).to(torch.complex64)
+
+
 def gaussian_coils(shape, n_coils, radius=1.3, width=0.8) -> np.ndarray:
     """Coil maps with Gaussian magnitude profiles centred on a sphere of the
     given radius (Fibonacci points), a smooth per-coil phase, normalised to
@@ -167,14 +194,18 @@ class MRIProblem:
         return tuple(self.target.shape)
 
 
-def make_problem(shape, n_ellipsoids, n_coils, mask2d, seed, noise_sigma=0.0, device="cpu") -> MRIProblem:
+def make_problem(shape, n_ellipsoids, n_coils, mask2d, seed, noise_sigma=0.0, device="cpu",
+                 phantom_on_device=False, coils=None) -> MRIProblem:
     """Simulates y_c = M * (fft3c(S_c x) + n_c) and the zero-filled SENSE adjoint
     (data/dataset.py:232-250, 527-537), with the known coil maps used for
-    reconstruction."""
-    x = ellipsoid_phantom(shape, n_ellipsoids, seed)
-    s = gaussian_coils(shape, n_coils)
-    target = torch.from_numpy(x).to(device)
-    coils = torch.from_numpy(s).to(device)
+    reconstruction.  `phantom_on_device` evaluates the phantom with torch on
+    `device`; `coils` reuses precomputed maps of this shape."""
+    if phantom_on_device:
+        target = ellipsoid_phantom_torch(shape, n_ellipsoids, seed, device=device)
+    else:
+        target = torch.from_numpy(ellipsoid_phantom(shape, n_ellipsoids, seed)).to(device)
+    if coils is None:
+        coils = torch.from_numpy(gaussian_coils(shape, n_coils)).to(device)
     mask = torch.from_numpy(expand_phase_mask(mask2d, shape)).to(device)
     k_full = fft3c(coils * target.unsqueeze(0))
     if noise_sigma > 0:

@@ -39,3 +39,29 @@ def test_library_exports_every_declared_symbol():
     assert lib.gstv_forward_workspace_bytes(1000) >= 64 * 1000
     assert lib.gstv_backward_workspace_bytes(1000) >= 64 * 1000 + 48 * 1000
     assert lib.gstv_bin_fill_workspace_bytes(10, 12345, 64) >= 4 * 4 * 12345
+
+
+MLP_HEADER = ROOT / "include" / "gstv_mlp.h"
+
+
+def _declared_mlp():
+    text = re.sub(r"/\*.*?\*/", "", MLP_HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(gstv_mlp_\w+)\s*\(", text)))
+
+
+def test_mlp_library_exports_every_declared_symbol():
+    from gs_tmi_amd.learned import mfma
+
+    if not mfma.LIB_PATH.exists():
+        pytest.skip("library not built (run `python -m gs_tmi_amd.build`)")
+    lib = mfma.load_library()
+    names = _declared_mlp()
+    assert {"gstv_mlp_gemm_nt_bf16", "gstv_mlp_transpose_bf16", "gstv_mlp_cast_bf16"} <= set(names)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(mfma.PROTOTYPES), "ctypes prototypes out of sync with the header"
+    assert lib.gstv_mlp_status_string(0) == b"success"
+    # Argument checks run on the host before any launch.
+    assert lib.gstv_mlp_gemm_nt_bf16(None, 4, None, 4, None, 0, 2, 2, 4, None, 2, None, 2, None) == 1
+    assert lib.gstv_mlp_gemm_nt_bf16(None, 2, None, 4, None, 0, 2, 2, 4, None, 2, None, 2, None) == 1
+    assert lib.gstv_mlp_gemm_nt_bf16(None, 4, None, 4, None, 0, 0, 2, 4, 1, 2, None, 2, None) == 0
