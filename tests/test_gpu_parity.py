@@ -174,3 +174,43 @@ def test_strict_loader_contract(monkeypatch, tmp_path):
     finally:
         monkeypatch.undo()
         _native._reset_for_tests()
+
+
+@pytest.mark.parametrize("name,make", ALL_CASES[:2], ids=[n for n, _ in ALL_CASES[:2]])
+def test_gradients_vs_reference_cpu_tile_path(name, make):
+    """The HIP Function's gradients against the fp32 PyTorch CPU restatement of
+    the reference's own path (per-tile forward of gaussian/tile_voxelizer.py:210-244
+    with torch autograd), on the same fp32 inputs and tile lists."""
+    from gs_tmi_amd import _native
+    from oracle import cpu_tile_torch as cpu
+
+    c = make()
+    d, h, w = c["shape"]
+    _announce(f"vs reference CPU tile path {name} shape={c['shape']} n={len(c['rr'])}")
+    center, cov, rr, ri, ranges, pids, hs = _function_inputs(c)
+    rng = np.random.default_rng(11)
+    g_r = rng.normal(size=d * h * w).astype(np.float32)
+    g_i = rng.normal(size=d * h * w).astype(np.float32)
+    gc, gm, grr, gri = _native.backward(center, cov, rr, ri, ranges, pids, hs, torch.from_numpy(g_r).to(DEV),
+                                        torch.from_numpy(g_i).to(DEV), c["tile"], d, h, w)
+    vol_r, vol_i = _native.forward(center, cov, rr, ri, ranges, pids, hs, c["tile"], d, h, w)
+    torch.cuda.synchronize()
+    cc, cm, cr, ci, cranges, cpids, chs = (t.cpu() for t in (center, cov, rr, ri, ranges, pids, hs))
+    g = torch.complex(torch.from_numpy(g_r), torch.from_numpy(g_i)).view(d, h, w)
+    ref_vol = cpu.tile_forward(cc, cm, torch.complex(cr, ci), chs, cranges, cpids, c["shape"], c["tile"])
+    rc, rm, rrr, rri = cpu.tile_forward_backward(cc, cm, cr, ci, chs, cranges, cpids, g, c["shape"], c["tile"],
+                                                 range(cranges.shape[0]))
+    args = (cc.numpy(), cm.numpy(), c["rr"], c["ri"], cranges.numpy(), cpids.numpy(), chs.numpy())
+    _, _, amb = vo.forward(*args, c["shape"], c["tile"], band=BAND)
+    *_, gamb = vo.backward(*args, g_r, g_i, c["shape"], c["tile"], band=BAND)
+    ok, okg = ~amb, ~gamb
+    e_vol = cases.rel_l2(np.concatenate([vol_r.cpu().numpy()[ok], vol_i.cpu().numpy()[ok]]),
+                         np.concatenate([ref_vol.real.numpy().ravel()[ok], ref_vol.imag.numpy().ravel()[ok]]))
+    errs = {"volume": e_vol,
+            "center": cases.rel_l2(gc.cpu().numpy()[okg], rc.numpy()[okg]),
+            "cov_inv": cases.rel_l2(gm.cpu().numpy().reshape(-1, 3, 3)[okg], rm.numpy()[okg]),
+            "rho": cases.rel_l2(np.stack([grr.cpu().numpy(), gri.cpu().numpy()], 1)[okg],
+                                np.stack([rrr.numpy(), rri.numpy()], 1)[okg])}
+    print(f"  HIP vs reference CPU path rel L2 = {errs}")
+    for k, e in errs.items():
+        assert e <= TOL, (k, e)
