@@ -5,8 +5,9 @@
 // which is the natural operand form of the gfx950 v_mfma_f32_16x16x32_bf16
 // instruction (each lane feeds 8 consecutive k of one row).  The caller forms
 // dX = dY * W by passing W^T and dW = dY^T * X by passing transposed copies
-// (tiled LDS transposes below).  Block tile 128x128, k-step 32, 4 waves of
-// 64x64 (4x4 MFMA tiles each), operands staged through padded LDS rows.
+// (tiled LDS transposes below).  Block tiles 128x128 or 64x64 (chosen so the
+// grid fills the chip), 64-wide k stages double-buffered in padded LDS rows,
+// 4 waves each owning a quarter of the block tile.
 
 #include <hip/hip_runtime.h>
 
@@ -21,9 +22,10 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int kBM = 128, kBN = 128, kBK = 32;
-constexpr int kThreads = 256;
-constexpr int kPad = 8;  // bf16 elements of padding per LDS row (16 B)
+constexpr int kBK = 64;                 // k per LDS stage
+constexpr int kPad = 8;                 // bf16 elements of padding per LDS row (16 B): conflict-free b128 reads
+constexpr int kLdk = kBK + kPad;
+constexpr int kThreads = 256;           // 4 waves in a 2 x 2 grid
 
 thread_local char g_detail[256] = "";
 
@@ -50,76 +52,111 @@ __device__ __forceinline__ float apply_act(float v, int act) {
   return v;
 }
 
-// Loads a 128 x 32 bf16 tile (rows r0.., k0..) of a K-contiguous matrix into
-// LDS; out-of-range elements are zero.
-__device__ __forceinline__ void load_tile(const uint16_t* __restrict__ src, int64_t ld, int rows, int K, int r0,
-                                          int k0, uint16_t (*dst)[kBK + kPad], bool vec_ok) {
-  for (int c = threadIdx.x; c < kBM * (kBK / 8); c += kThreads) {
-    const int r = c >> 2;
-    const int kc = (c & 3) * 8;
-    const int gr = r0 + r, gk = k0 + kc;
+// Global -> registers: this thread's 16-byte chunks of a ROWS x 64 tile of a
+// K-contiguous bf16 matrix (chunk c: row c / 8, k 8 * (c % 8)); out-of-range
+// elements are zero.
+template <int ROWS>
+__device__ __forceinline__ void load_chunks(const uint16_t* __restrict__ src, int64_t ld, int rows, int K, int r0,
+                                            int k0, bool vec_ok, uint4 (&reg)[ROWS * kBK / 8 / kThreads]) {
+#pragma unroll
+  for (int i = 0; i < ROWS * kBK / 8 / kThreads; ++i) {
+    const int c = threadIdx.x + i * kThreads;
+    const int gr = r0 + (c >> 3), gk = k0 + (c & 7) * 8;
     uint4 v = make_uint4(0, 0, 0, 0);
     if (gr < rows) {
       const uint16_t* p = src + (int64_t)gr * ld + gk;
       if (vec_ok && gk + 8 <= K) {
         v = *reinterpret_cast<const uint4*>(p);
       } else {
-        uint16_t tmp[8];
+        uint16_t t[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) tmp[e] = (gk + e < K) ? p[e] : (uint16_t)0;
-        v = make_uint4(tmp[0] | (tmp[1] << 16), tmp[2] | (tmp[3] << 16), tmp[4] | (tmp[5] << 16),
-                       tmp[6] | (tmp[7] << 16));
+        for (int e = 0; e < 8; ++e) t[e] = (gk + e < K) ? p[e] : (uint16_t)0;
+        v = make_uint4(t[0] | (t[1] << 16), t[2] | (t[3] << 16), t[4] | (t[5] << 16), t[6] | (t[7] << 16));
       }
     }
-    *reinterpret_cast<uint4*>(&dst[r][kc]) = v;
+    reg[i] = v;
   }
 }
 
+template <int ROWS>
+__device__ __forceinline__ void store_chunks(uint16_t (*dst)[kLdk], const uint4 (&reg)[ROWS * kBK / 8 / kThreads]) {
+#pragma unroll
+  for (int i = 0; i < ROWS * kBK / 8 / kThreads; ++i) {
+    const int c = threadIdx.x + i * kThreads;
+    *reinterpret_cast<uint4*>(&dst[c >> 3][(c & 7) * 8]) = reg[i];
+  }
+}
+
+// C[M,N] = act(A[M,K] B[N,K]^T + bias).  Block tile BM x BN, four waves of
+// (BM/2) x (BN/2), each a grid of 16 x 16 MFMA tiles.  The k loop runs over
+// 64-wide stages held in two LDS buffers: the next stage's global loads are
+// issued into registers before the current stage's MFMAs and stored to the
+// other buffer after them, so one barrier per stage suffices.
+template <int BM, int BN>
 __global__ __launch_bounds__(kThreads) void k_gemm_nt(const uint16_t* __restrict__ A, int64_t lda,
                                                       const uint16_t* __restrict__ B, int64_t ldb,
                                                       const float* __restrict__ bias, int act, int M, int N, int K,
                                                       float* __restrict__ C, int64_t ldc, uint16_t* __restrict__ Cb,
                                                       int64_t ldcb, bool vec_a, bool vec_b) {
-  __shared__ __align__(16) uint16_t sA[kBM][kBK + kPad];
-  __shared__ __align__(16) uint16_t sB[kBN][kBK + kPad];
+  constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+  __shared__ __align__(16) uint16_t sA[2][BM][kLdk];
+  __shared__ __align__(16) uint16_t sB[2][BN][kLdk];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
-  floatx4 acc[4][4];
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  floatx4 acc[TM][TN];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int j = 0; j < TN; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 
-  const int row = lane & 15;      // operand row inside a 16-row MFMA tile
-  const int kq = (lane >> 4) * 8;  // this lane's 8 consecutive k inside the 32-wide step
-  for (int k0 = 0; k0 < K; k0 += kBK) {
-    load_tile(A, lda, M, K, m0, k0, sA, vec_a);
-    load_tile(B, ldb, N, K, n0, k0, sB, vec_b);
-    __syncthreads();
-    bf16x8 a[4], b[4];
+  uint4 ra[BM * kBK / 8 / kThreads], rb[BN * kBK / 8 / kThreads];
+  const int nk = (K + kBK - 1) / kBK;
+  load_chunks<BM>(A, lda, M, K, m0, 0, vec_a, ra);
+  load_chunks<BN>(B, ldb, N, K, n0, 0, vec_b, rb);
+  store_chunks<BM>(sA[0], ra);
+  store_chunks<BN>(sB[0], rb);
+  __syncthreads();
+
+  const int row = lane & 15;       // operand row inside a 16-row MFMA tile
+  const int kq = (lane >> 4) * 8;  // this lane's 8 consecutive k inside a 32-wide MFMA step
+  for (int kt = 0; kt < nk; ++kt) {
+    const int buf = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) {
+      load_chunks<BM>(A, lda, M, K, m0, (kt + 1) * kBK, vec_a, ra);
+      load_chunks<BN>(B, ldb, N, K, n0, (kt + 1) * kBK, vec_b, rb);
+    }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&sA[wm * 64 + i * 16 + row][kq]);
+    for (int ks = 0; ks < kBK; ks += 32) {
+      bf16x8 a[TM], b[TN];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&sB[wn * 64 + j * 16 + row][kq]);
+      for (int i = 0; i < TM; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&sA[buf][wm * WM + i * 16 + row][ks + kq]);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < TN; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&sB[buf][wn * WN + j * 16 + row][ks + kq]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) {
+      store_chunks<BM>(sA[buf ^ 1], ra);
+      store_chunks<BN>(sB[buf ^ 1], rb);
+    }
     __syncthreads();
   }
   // D layout: lane holds rows 4*(lane/16) .. +3 of column lane%16 of each tile.
   const int rq = (lane >> 4) * 4, col = lane & 15;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int n = n0 + wn * 64 + j * 16 + col;
+    for (int j = 0; j < TN; ++j) {
+      const int n = n0 + wn * WN + j * 16 + col;
       if (n >= N) continue;
       const float bn = bias ? bias[n] : 0.0f;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm * 64 + i * 16 + rq + r;
+        const int m = m0 + wm * WM + i * 16 + rq + r;
         if (m >= M) continue;
         const float v = apply_act(acc[i][j][r] + bn, act);
         if (C) C[(int64_t)m * ldc + n] = v;
@@ -172,10 +209,21 @@ int gstv_mlp_gemm_nt_bf16(const void* a, int64_t lda, const void* b, int64_t ldb
   if (m == 0 || n == 0) return GSTV_MLP_OK;
   const bool vec_a = (reinterpret_cast<uintptr_t>(a) % 16 == 0) && (lda % 8 == 0);
   const bool vec_b = (reinterpret_cast<uintptr_t>(b) % 16 == 0) && (ldb % 8 == 0);
-  dim3 grid((n + kBN - 1) / kBN, (m + kBM - 1) / kBM);
-  hipLaunchKernelGGL(k_gemm_nt, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream),
-                     static_cast<const uint16_t*>(a), lda, static_cast<const uint16_t*>(b), ldb, bias, act, m, n, k,
-                     c, ldc, static_cast<uint16_t*>(c_bf16), ldcb, vec_a, vec_b);
+  // 128 x 128 tiles while they give at least one block per CU, else 64 x 64.
+  const int64_t big_blocks = (int64_t)((n + 127) / 128) * ((m + 127) / 128);
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  const uint16_t* pa = static_cast<const uint16_t*>(a);
+  const uint16_t* pb = static_cast<const uint16_t*>(b);
+  uint16_t* pcb = static_cast<uint16_t*>(c_bf16);
+  if (big_blocks >= 256) {
+    dim3 grid((n + 127) / 128, (m + 127) / 128);
+    hipLaunchKernelGGL((k_gemm_nt<128, 128>), grid, dim3(kThreads), 0, st, pa, lda, pb, ldb, bias, act, m, n, k, c,
+                       ldc, pcb, ldcb, vec_a, vec_b);
+  } else {
+    dim3 grid((n + 63) / 64, (m + 63) / 64);
+    hipLaunchKernelGGL((k_gemm_nt<64, 64>), grid, dim3(kThreads), 0, st, pa, lda, pb, ldb, bias, act, m, n, k, c,
+                       ldc, pcb, ldcb, vec_a, vec_b);
+  }
   return hip_status(hipGetLastError(), "gemm_nt_bf16");
 }
 

@@ -159,3 +159,43 @@ def test_init_mlp_gradients_match_fp64_oracle():
         cos = float(np.sum(gw * dw) / (np.linalg.norm(gw) * np.linalg.norm(dw)))
         assert cos > 0.99 and rel(gw, dw) < 0.1
         assert rel(l.bias.grad.cpu().double().numpy(), db) < 0.1
+
+
+def test_gemm_nt_k_split_invariance():
+    """A K-contiguous product equals the sum of the products of its two K halves
+    (sub-matrix views with the full leading dimension), to fp32 accumulation."""
+    mf = _mfma()
+    g = torch.Generator(device="cpu").manual_seed(14)
+    m, n, k, k1 = 300, 200, 768, 320
+    a = mf.to_bf16(torch.randn(m, k, generator=g).to(DEV))
+    b = mf.to_bf16(torch.randn(n, k, generator=g).to(DEV))
+    lib = mf.get_lib()
+    s = torch.cuda.current_stream().cuda_stream
+    full = mf.gemm_nt(a, b)
+    lo = torch.empty(m, n, device=DEV)
+    hi = torch.empty(m, n, device=DEV)
+    assert lib.gstv_mlp_gemm_nt_bf16(a.data_ptr(), k, b.data_ptr(), k, None, 0, m, n, k1, lo.data_ptr(), n, None, 0,
+                                     s) == 0
+    assert lib.gstv_mlp_gemm_nt_bf16(a.data_ptr() + 2 * k1, k, b.data_ptr() + 2 * k1, k, None, 0, m, n, k - k1,
+                                     hi.data_ptr(), n, None, 0, s) == 0
+    torch.cuda.synchronize()
+    a64, b64 = _bf16_np(a), _bf16_np(b)
+    bound = np.abs(a64) @ np.abs(b64).T * (k + 8) * 2.0 ** -23
+    diff = np.abs(full.cpu().double().numpy() - (lo + hi).cpu().double().numpy())
+    assert np.all(diff <= bound), float(np.max(diff - bound))
+
+
+@pytest.mark.parametrize("m,n,k", [(64, 64, 64), (65, 63, 130), (256, 128, 200), (4096, 512, 1024)])
+def test_gemm_nt_tile_variants(m, n, k):
+    """Both block-tile variants (64x64 below one block per CU, 128x128 above)
+    against FP64 of the rounded operands, with bias, ReLU and the bf16 copy."""
+    mf = _mfma()
+    print(f"[gemm variant] m={m} n={n} k={k}", flush=True)
+    g = torch.Generator(device="cpu").manual_seed(m + n + k)
+    ab = mf.to_bf16(torch.randn(m, k, generator=g).to(DEV))
+    bb = mf.to_bf16(torch.randn(n, k, generator=g).to(DEV))
+    bias = torch.randn(n, generator=g).to(DEV)
+    c, cb = mf.gemm_nt(ab, bb, bias, act="relu", want_bf16=True)
+    torch.cuda.synchronize()
+    _check_product(c.cpu().double().numpy(), _bf16_np(ab), _bf16_np(bb), bias.cpu().double().numpy(), "relu")
+    assert torch.equal(cb.cpu(), c.cpu().to(torch.bfloat16))
