@@ -811,6 +811,13 @@ constexpr int kAccStride = 12;
 
 constexpr int kMaxClass = 64;  // sub-bricks per tile for tile <= 16
 
+// Strides (float2 elements) of the tile's upstream-gradient copy in LDS.  With
+// rows padded by 2 and slabs by 4 elements, the sub-brick origins of an 8^3
+// tile fall on distinct groups of 8 banks, so a wave whose 16 Gaussians each
+// read their own sub-brick (4 W lanes, 64-bit loads) is conflict-free.
+__host__ __device__ constexpr int grad_ys(int t) { return t + 2; }
+__host__ __device__ constexpr int grad_zs(int t) { return t * (t + 2) + 4; }
+
 struct BwdStage {
   float c0[kBatch], c1[kBatch], c2[kBatch], rr[kBatch], ri[kBatch];
   float a00[kBatch], a11[kBatch], a22[kBatch], b01[kBatch], b02[kBatch], b12[kBatch];
@@ -836,23 +843,25 @@ __device__ __forceinline__ void bwd_subbrick(const float2* __restrict__ sh_g, in
                                              int s0, int s1, int s2, int xi, bool g_ok, float c0, float c1,
                                              float c2, float rr, float ri, float a00, float a11, float a22,
                                              float b01, float b02, float b12, BwdAcc& acc) {
-  const int T = TT > 0 ? TT : tp;  // row stride of sh_g
+  const int T = TT > 0 ? TT : tp;
+  const int YS = grad_ys(T), ZS = grad_zs(T);
   const float u2 = (float)(b.o2 + s2 + xi) - c2;
   const float k0 = a22 * u2 * u2, k1 = b02 * u2, k2 = b12 * u2;
-  const float2* gp = sh_g + (s0 * T + s1) * T + s2 + xi;
+  const float2* gp = sh_g + s0 * ZS + s1 * YS + s2 + xi;
   bool ok_lane = g_ok;
   if (kPartial) ok_lane = ok_lane && (s2 + xi < b.e2);
   float m = 0.0f, m0 = 0.0f, m1 = 0.0f;
   float u1v[kSub];
 #pragma unroll
   for (int y = 0; y < kSub; ++y) u1v[y] = (float)(b.o1 + s1 + y) - c1;
+  const float u10 = u1v[0];
 #pragma unroll
   for (int z = 0; z < kSub; ++z) {
     const float u0 = (float)(b.o0 + s0 + z) - c0;
     float A = fmaf(fmaf(a00, u0, k1), u0, k0);
     if (kPartial) A = (s0 + z < b.e0) ? A : -INFINITY;
     const float B = fmaf(b01, u0, k2);
-    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+    float wt[kSub];
 #pragma unroll
     for (int y = 0; y < kSub; ++y) {
       const float u1 = u1v[y];
@@ -861,15 +870,19 @@ __device__ __forceinline__ void bwd_subbrick(const float2* __restrict__ sh_g, in
       const float q = fmaf(fmaf(a11, u1, B), u1, Ay);
       const bool ok = ok_lane && (q >= kThr);
       const float w = ok ? __builtin_amdgcn_exp2f(q) : 0.0f;
-      const float2 g = gp[(z * T + y) * T];
+      const float2 g = gp[z * ZS + y * YS];
       acc.dr = fmaf(w, g.x, acc.dr);
       acc.di = fmaf(w, g.y, acc.di);
-      const float wt = w * fmaf(ri, g.y, rr * g.x);
-      t0 += wt;
-      const float sy = wt * u1;
-      t1 += sy;
-      t2 = fmaf(sy, u1, t2);
+      wt[y] = w * fmaf(ri, g.y, rr * g.x);
     }
+    // Row moments with u1 = u10 + y:  t0 = sum wt,  t1 = sum wt u1 = u10 t0 + s1,
+    // t2 = sum wt u1^2 = u10^2 t0 + 2 u10 s1 + s2 = u10 (t1 + s1) + s2, where
+    // s1 = sum y wt_y and s2 = sum y^2 wt_y.
+    const float t0 = (wt[0] + wt[1]) + (wt[2] + wt[3]);
+    const float s1y = fmaf(3.0f, wt[3], fmaf(2.0f, wt[2], wt[1]));
+    const float s2y = fmaf(9.0f, wt[3], fmaf(4.0f, wt[2], wt[1]));
+    const float t1 = fmaf(u10, t0, s1y);
+    const float t2 = fmaf(u10, t1 + s1y, s2y);
     m += t0;
     const float tz = t0 * u0;
     m0 += tz;
@@ -895,8 +908,9 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
                                                            const float* __restrict__ grad_i,
                                                            float* __restrict__ gacc) {
   __shared__ BwdStage sh;
-  // Upstream gradient of the tile, [TP][TP][TP] with TP the tile edge rounded
-  // up to a multiple of 4, zero outside the tile's valid voxels.
+  // Upstream gradient of the tile, TP^3 with TP the tile edge rounded up to a
+  // multiple of 4 (strides grad_zs, grad_ys, 1), zero outside the tile's valid
+  // voxels.
   extern __shared__ float2 sh_g[];
   const int logical = xcd_remap(blockIdx.x, gridDim.x);
   if (logical >= geo.blocks) return;
@@ -916,7 +930,7 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
       const int64_t flat = ((int64_t)(b.o0 + z) * geo.H + (b.o1 + y)) * geo.W + (b.o2 + x);
       g = make_float2(grad_r[flat], grad_i[flat]);
     }
-    sh_g[i] = g;
+    sh_g[z * grad_zs(T) + y * grad_ys(T) + x] = g;
   }
   if (tid <= kMaxClass) sh.cls_cnt[tid] = 0;
 
@@ -1551,7 +1565,7 @@ int gstv_backward(const float* center_vox, const float* cov_inv, const float* rh
   geo_b.chunks = 1;
   geo_b.blocks = num_tiles;
   const int tp = geo.nsb * kSub;
-  const size_t lds_grad = (size_t)tp * tp * tp * sizeof(float2);
+  const size_t lds_grad = (size_t)tp * grad_zs(tp) * sizeof(float2);
   if (tile == 8)
     hipLaunchKernelGGL(k_backward<8>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, (const GRec*)rec,
                        tile_ranges, pair_ids, geo_b, grad_r, grad_i, gacc);
