@@ -658,23 +658,28 @@ struct GaussLane {
   float u2;
 };
 
-__device__ __forceinline__ GaussLane load_gauss(const Stage& sh, int k, const BlockCtx& b, int xi) {
+struct GaussRaw {
+  float c0, c1, c2, rr, ri, a00, a11, a22, b01, b02, b12;
+};
+
+__device__ __forceinline__ GaussRaw load_raw(const Stage& sh, int k) {
+  return GaussRaw{sh.c0[k], sh.c1[k], sh.c2[k], sh.rr[k], sh.ri[k], sh.a00[k], sh.a11[k], sh.a22[k], sh.b01[k],
+                  sh.b02[k], sh.b12[k]};
+}
+
+__device__ __forceinline__ GaussLane lane_terms(const GaussRaw& r, const BlockCtx& b, int xi) {
   GaussLane g;
-  g.c0 = sh.c0[k];
-  g.c1 = sh.c1[k];
-  const float c2 = sh.c2[k];
-  g.rr = sh.rr[k];
-  g.ri = sh.ri[k];
-  g.a00 = sh.a00[k];
-  g.a11 = sh.a11[k];
-  const float a22 = sh.a22[k];
-  g.b01 = sh.b01[k];
-  const float b02 = sh.b02[k];
-  const float b12 = sh.b12[k];
-  g.u2 = (float)(b.o2 + b.s2 + xi) - c2;
-  g.k0 = a22 * g.u2 * g.u2;
-  g.k1 = b02 * g.u2;
-  g.k2 = b12 * g.u2;
+  g.c0 = r.c0;
+  g.c1 = r.c1;
+  g.rr = r.rr;
+  g.ri = r.ri;
+  g.a00 = r.a00;
+  g.a11 = r.a11;
+  g.b01 = r.b01;
+  g.u2 = (float)(b.o2 + b.s2 + xi) - r.c2;
+  g.k0 = r.a22 * g.u2 * g.u2;
+  g.k1 = r.b02 * g.u2;
+  g.k2 = r.b12 * g.u2;
 #pragma unroll
   for (int y = 0; y < kSub; ++y) g.u1[y] = (float)(b.o1 + b.s1 + y) - g.c1;
   return g;
@@ -690,7 +695,7 @@ __device__ __forceinline__ void fwd_groups(const Stage& sh, int nhit, const Bloc
     const int idx = gb + gl;
     const bool g_ok = (idx < nhit) && x_ok;
     const int k = sh.list[wave][idx < nhit ? idx : gb];
-    const GaussLane g = load_gauss(sh, k, b, xi);
+    const GaussLane g = lane_terms(load_raw(sh, k), b, xi);
 #pragma unroll
     for (int z = 0; z < kSub; ++z) {
       if (!kPartial || z < b.n0) {
@@ -802,19 +807,23 @@ constexpr int kAccStride = 12;
 //
 // The backward gives every Gaussian of a batch to one set of lanes for the
 // whole tile, so its 11 moments accumulate in registers and no LDS or global
-// atomics are needed inside the tile.  Gaussians are classed by the number of
-// 4^3 sub-bricks their exact box touches inside the tile; a group is 16
-// Gaussians of one class (lane = (Gaussian slot, W offset)), and every lane
-// walks the sub-bricks of its own Gaussian, so a group costs exactly
-// class x 16 x 64 evaluations.  Groups are dealt to the 8 waves in descending
-// class order for balance.
+// atomics are needed inside the tile.  A lane = (Gaussian slot, W offset)
+// walks the columns of its Gaussian's box -- 4 x 4 (H, W) voxel blocks -- and
+// in each column exactly the D rows of the box (no rounding of D to blocks).
+// Gaussians are keyed by (columns, D rows) and counting-sorted; a group is 16
+// Gaussians with the same column count and neighbouring row counts, so a group
+// costs columns x max(rows) x 64 evaluations.  Groups are dealt to the 8 waves
+// in descending cost order for balance.
 
-constexpr int kMaxClass = 64;  // sub-bricks per tile for tile <= 16
+constexpr int kMaxCols = 16;           // 4 x 4 column blocks for tile <= 16
+constexpr int kMaxRows = 16;           // D rows for tile <= 16
+constexpr int kKeys = kMaxCols * kMaxRows;
+constexpr uint16_t kNoKey = 0xffff;
 
 // Strides (float2 elements) of the tile's upstream-gradient copy in LDS.  With
-// rows padded by 2 and slabs by 4 elements, the sub-brick origins of an 8^3
-// tile fall on distinct groups of 8 banks, so a wave whose 16 Gaussians each
-// read their own sub-brick (4 W lanes, 64-bit loads) is conflict-free.
+// rows padded by 2 and slabs by 4 elements, the 4 x 4 column blocks of an 8^3
+// tile at different D origins fall on different groups of 8 banks, so the 64
+// lanes of a wave (64-bit loads) rarely conflict.
 __host__ __device__ constexpr int grad_ys(int t) { return t + 2; }
 __host__ __device__ constexpr int grad_zs(int t) { return t * (t + 2) + 4; }
 
@@ -822,32 +831,34 @@ struct BwdStage {
   float c0[kBatch], c1[kBatch], c2[kBatch], rr[kBatch], ri[kBatch];
   float a00[kBatch], a11[kBatch], a22[kBatch], b01[kBatch], b02[kBatch], b12[kBatch];
   float out[kAccStride][kBatch];  // per-pair moment totals for this tile
-  uint16_t sbox[kBatch];          // sub-brick box: lo0,lo1,lo2,hi0,hi1,hi2, 2 bits each
-  uint16_t order[kBatch];         // batch indices grouped by class
-  uint16_t slot[kBatch];          // rank inside the class
-  uint8_t cls[kBatch];            // sub-bricks touched (0: none)
-  int cls_cnt[kMaxClass + 1];
-  int cls_start[kMaxClass + 1];
-  int grp_start[kMaxClass];       // indexed by kMaxClass - class (descending classes)
-  int num_groups;
+  uint16_t box[kBatch];           // z0 (4 bits), rows-1 (4), H blocks lo, hi, W blocks lo, hi (2 each)
+  uint16_t key[kBatch];           // (columns - 1) * 16 + rows - 1, or kNoKey
+  uint16_t order[kBatch];         // batch indices sorted by key
+  uint16_t slot[kBatch];          // rank inside the key
+  int key_cnt[kKeys];
+  int key_start[kKeys];
+  int seg_begin[kMaxCols];        // per column count cc (index cc - 1): first sorted position
+  int seg_cnt[kMaxCols];          //   number of Gaussians
+  int grp_first[kMaxCols];        //   first group; groups run in descending column count
 };
 
 struct BwdAcc {
   float dr, di, m, m0, m00, m1, m01, m11, m2, m22, m02, m12;
 };
 
-// One 4^3 sub-brick of one lane's Gaussian: 16 (D, H) positions of the lane's
-// W column.  TT is the tile edge when known at compile time (0: runtime).
+// One column (4 x 4 block of H, W) of one lane's Gaussian: `rows` D rows from
+// tile-local z0 (rows beyond the lane's own `own_rows` contribute nothing).
+// TT is the tile edge when known at compile time (0: runtime).
 template <int TT, bool kPartial>
-__device__ __forceinline__ void bwd_subbrick(const float2* __restrict__ sh_g, int tp, const BlockCtx& b,
-                                             int s0, int s1, int s2, int xi, bool g_ok, float c0, float c1,
-                                             float c2, float rr, float ri, float a00, float a11, float a22,
-                                             float b01, float b02, float b12, BwdAcc& acc) {
+__device__ __forceinline__ void bwd_column(const float2* __restrict__ sh_g, int tp, const BlockCtx& b, int z0,
+                                           int rows, int own_rows, int s1, int s2, int xi, bool g_ok, float c0,
+                                           float c1, float c2, float rr, float ri, float a00, float a11, float a22,
+                                           float b01, float b02, float b12, BwdAcc& acc) {
   const int T = TT > 0 ? TT : tp;
   const int YS = grad_ys(T), ZS = grad_zs(T);
   const float u2 = (float)(b.o2 + s2 + xi) - c2;
   const float k0 = a22 * u2 * u2, k1 = b02 * u2, k2 = b12 * u2;
-  const float2* gp = sh_g + s0 * ZS + s1 * YS + s2 + xi;
+  const float2* gp = sh_g + s1 * YS + s2 + xi;
   bool ok_lane = g_ok;
   if (kPartial) ok_lane = ok_lane && (s2 + xi < b.e2);
   float m = 0.0f, m0 = 0.0f, m1 = 0.0f;
@@ -855,12 +866,27 @@ __device__ __forceinline__ void bwd_subbrick(const float2* __restrict__ sh_g, in
 #pragma unroll
   for (int y = 0; y < kSub; ++y) u1v[y] = (float)(b.o1 + s1 + y) - c1;
   const float u10 = u1v[0];
+  // The next row's upstream gradient is loaded from LDS while the current row
+  // is evaluated.
+  float2 gn[kSub];
 #pragma unroll
-  for (int z = 0; z < kSub; ++z) {
-    const float u0 = (float)(b.o0 + s0 + z) - c0;
+  for (int y = 0; y < kSub; ++y) gn[y] = gp[z0 * ZS + y * YS];
+#pragma unroll 2
+  for (int r = 0; r < rows; ++r) {
+    const bool row_ok = r < own_rows;
+    const int z = row_ok ? z0 + r : z0;  // rows past the lane's own stay in the tile
+    const float u0 = (float)(b.o0 + z) - c0;
     float A = fmaf(fmaf(a00, u0, k1), u0, k0);
-    if (kPartial) A = (s0 + z < b.e0) ? A : -INFINITY;
+    A = row_ok ? A : -INFINITY;
     const float B = fmaf(b01, u0, k2);
+    float2 gc[kSub];
+#pragma unroll
+    for (int y = 0; y < kSub; ++y) gc[y] = gn[y];
+    {
+      const int zn = (r + 1 < own_rows) ? z0 + r + 1 : z0;
+#pragma unroll
+      for (int y = 0; y < kSub; ++y) gn[y] = gp[zn * ZS + y * YS];
+    }
     float wt[kSub];
 #pragma unroll
     for (int y = 0; y < kSub; ++y) {
@@ -870,7 +896,7 @@ __device__ __forceinline__ void bwd_subbrick(const float2* __restrict__ sh_g, in
       const float q = fmaf(fmaf(a11, u1, B), u1, Ay);
       const bool ok = ok_lane && (q >= kThr);
       const float w = ok ? __builtin_amdgcn_exp2f(q) : 0.0f;
-      const float2 g = gp[z * ZS + y * YS];
+      const float2 g = gc[y];
       acc.dr = fmaf(w, g.x, acc.dr);
       acc.di = fmaf(w, g.y, acc.di);
       wt[y] = w * fmaf(ri, g.y, rr * g.x);
@@ -921,7 +947,7 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
   // One block per tile: the context's wave-specific fields are unused here.
   const BlockCtx b = make_ctx(logical, geo, ranges, 0);
   if (!b.valid) return;
-  const bool partial = b.e0 < T || b.e1 < T || b.e2 < T;
+  const bool partial = b.e1 < T || b.e2 < T;  // D rows are always clipped to the tile
 
   for (int i = tid; i < T * T * T; i += kTileThreads) {
     const int z = i / (T * T), y = (i / T) % T, x = i % T;
@@ -932,7 +958,7 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
     }
     sh_g[z * grad_zs(T) + y * grad_ys(T) + x] = g;
   }
-  if (tid <= kMaxClass) sh.cls_cnt[tid] = 0;
+  for (int i = tid; i < kKeys; i += kTileThreads) sh.key_cnt[i] = 0;
 
   // Records of the next batch are loaded into registers while the current
   // batch is evaluated (loads stay in flight across the LDS barriers).
@@ -941,7 +967,7 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
   for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
     const int cnt = min(kBatch, b.p_end - bs);
     __syncthreads();
-    // Stage: parameters, sub-brick box and class of every pair of the batch.
+    // Stage: parameters, box and key of every pair of the batch.
     if (tid < cnt) {
       const GRec r = nxt;
       sh.c0[tid] = r.c0;
@@ -963,92 +989,113 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
       const float hi0 = fminf(floorf(r.c0 + r.e0), (float)(b.o0 + b.e0 - 1));
       const float hi1 = fminf(floorf(r.c1 + r.e1), (float)(b.o1 + b.e1 - 1));
       const float hi2 = fminf(floorf(r.c2 + r.e2), (float)(b.o2 + b.e2 - 1));
-      int cls = 0;
+      uint16_t key = kNoKey;
       if (lo0 <= hi0 && lo1 <= hi1 && lo2 <= hi2) {
-        const int l0 = ((int)lo0 - b.o0) >> 2, l1 = ((int)lo1 - b.o1) >> 2, l2 = ((int)lo2 - b.o2) >> 2;
-        const int h0 = ((int)hi0 - b.o0) >> 2, h1 = ((int)hi1 - b.o1) >> 2, h2 = ((int)hi2 - b.o2) >> 2;
-        sh.sbox[tid] = (uint16_t)(l0 | (l1 << 2) | (l2 << 4) | (h0 << 6) | (h1 << 8) | (h2 << 10));
-        cls = (h0 - l0 + 1) * (h1 - l1 + 1) * (h2 - l2 + 1);
-        sh.slot[tid] = (uint16_t)atomicAdd(&sh.cls_cnt[cls], 1);
+        const int z0 = (int)lo0 - b.o0, rows = (int)hi0 - (int)lo0 + 1;
+        const int l1 = ((int)lo1 - b.o1) >> 2, l2 = ((int)lo2 - b.o2) >> 2;
+        const int h1 = ((int)hi1 - b.o1) >> 2, h2 = ((int)hi2 - b.o2) >> 2;
+        const int cols = (h1 - l1 + 1) * (h2 - l2 + 1);
+        sh.box[tid] = (uint16_t)(z0 | ((rows - 1) << 4) | (l1 << 8) | (h1 << 10) | (l2 << 12) | (h2 << 14));
+        key = (uint16_t)((cols - 1) * kMaxRows + rows - 1);
+        sh.slot[tid] = (uint16_t)atomicAdd(&sh.key_cnt[key], 1);
       }
-      sh.cls[tid] = (uint8_t)cls;
+      sh.key[tid] = key;
     }
     {
       const int bn = bs + kBatch;
       if (bn < b.p_end && tid < min(kBatch, b.p_end - bn)) nxt = rec[pair_ids[bn + tid]];
     }
     __syncthreads();
-    // Class offsets and group offsets, classes in descending order.
+    // Key offsets (ascending keys) and, per column count, the sorted segment and
+    // its first group (groups run in descending column count).
     if (wave == 0) {
-      const int c = kMaxClass - lane;  // lane 0 -> class 64 ... lane 63 -> class 1
-      const int n_c = sh.cls_cnt[c];
-      const int ng = (n_c + 15) >> 4;
-      int inc_n = n_c, inc_g = ng;
+      int c[4], sum = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        c[e] = sh.key_cnt[lane * 4 + e];
+        sum += c[e];
+      }
+      int inc = sum;
 #pragma unroll
       for (int off = 1; off < 64; off <<= 1) {
-        const int on = __shfl_up(inc_n, off, 64);
-        const int og = __shfl_up(inc_g, off, 64);
-        if (lane >= off) {
-          inc_n += on;
-          inc_g += og;
-        }
+        const int o = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += o;
       }
-      sh.cls_start[c] = inc_n - n_c;
-      sh.grp_start[lane] = inc_g - ng;
-      if (lane == 63) sh.num_groups = inc_g;
+      int run = inc - sum;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        sh.key_start[lane * 4 + e] = run;
+        run += c[e];
+      }
+      // Column count cc owns keys [16 (cc - 1), 16 cc), i.e. lanes 4 (cc - 1) .. 4 (cc - 1) + 3.
+      const int seg = sum + __shfl_down(sum, 1, 64) + __shfl_down(sum, 2, 64) + __shfl_down(sum, 3, 64);
+      const int ng = (lane & 3) == 0 ? (seg + 15) >> 4 : 0;
+      int gin = ng;  // suffix sum over the segment lanes (stride 4)
+#pragma unroll
+      for (int off = 4; off < 64; off <<= 1) {
+        const int o = __shfl_down(gin, off, 64);
+        if (lane + off < 64) gin += o;
+      }
+      if ((lane & 3) == 0) {
+        sh.seg_begin[lane >> 2] = inc - sum;
+        sh.seg_cnt[lane >> 2] = seg;
+        sh.grp_first[lane >> 2] = gin - ng;
+      }
     }
     __syncthreads();
-    if (tid < cnt && sh.cls[tid] != 0) sh.order[sh.cls_start[sh.cls[tid]] + sh.slot[tid]] = (uint16_t)tid;
+    if (tid < cnt && sh.key[tid] != kNoKey) sh.order[sh.key_start[sh.key[tid]] + sh.slot[tid]] = (uint16_t)tid;
     __syncthreads();
 
-    const int num_groups = sh.num_groups;
-    for (int gi = wave; gi < num_groups; gi += kWaves) {
-      int lo = 0, hi = kMaxClass - 1;  // last class rank whose groups start at or before gi
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (sh.grp_start[mid] <= gi) lo = mid; else hi = mid - 1;
-      }
-      const int c = kMaxClass - lo;
-      const int within = (gi - sh.grp_start[lo]) << 4;
-      const int m_valid = min(16, sh.cls_cnt[c] - within);
-      const bool slot_ok = gl < m_valid;
-      const int k = sh.order[sh.cls_start[c] + within + (slot_ok ? gl : 0)];
-      const float c0 = sh.c0[k], c1 = sh.c1[k], c2 = sh.c2[k], rr = sh.rr[k], ri = sh.ri[k];
-      const float a00 = sh.a00[k], a11 = sh.a11[k], a22 = sh.a22[k];
-      const float b01 = sh.b01[k], b02 = sh.b02[k], b12 = sh.b12[k];
-      const int sb = sh.sbox[k];
-      const int l0 = sb & 3, l1 = (sb >> 2) & 3, l2 = (sb >> 4) & 3;
-      const int d1 = ((sb >> 8) & 3) - l1 + 1, d2 = ((sb >> 10) & 3) - l2 + 1;
-      BwdAcc acc = {};
-      int j0 = 0, j1 = 0, j2 = 0;
-      for (int j = 0; j < c; ++j) {
-        const int s0 = (l0 + j0) * kSub, s1 = (l1 + j1) * kSub, s2 = (l2 + j2) * kSub;
-        if (partial)
-          bwd_subbrick<TT, true>(sh_g, T, b, s0, s1, s2, xi, slot_ok, c0, c1, c2, rr, ri, a00, a11, a22, b01, b02,
-                                 b12, acc);
-        else
-          bwd_subbrick<TT, false>(sh_g, T, b, s0, s1, s2, xi, slot_ok, c0, c1, c2, rr, ri, a00, a11, a22, b01, b02,
-                                  b12, acc);
-        if (++j2 == d2) {
-          j2 = 0;
-          if (++j1 == d1) {
-            j1 = 0;
-            ++j0;
+    // Groups are dealt round-robin to the waves: global group g goes to wave g % 8.
+#pragma unroll 1
+    for (int cc = kMaxCols; cc >= 1; --cc) {
+      const int seg_begin = sh.seg_begin[cc - 1];
+      const int seg_cnt = sh.seg_cnt[cc - 1];
+      const int first = sh.grp_first[cc - 1];
+      const int ng = (seg_cnt + 15) >> 4;
+      for (int r = (wave - first) & (kWaves - 1); r < ng; r += kWaves) {
+        // Group r holds sorted positions top - 15 .. top (largest keys first).
+        const int top = seg_begin + seg_cnt - 1 - 16 * r;
+        const int pos = top - gl;
+        const bool slot_ok = pos >= seg_begin;
+        const int k = sh.order[slot_ok ? pos : top];
+        const int rows = ((sh.box[sh.order[top]] >> 4) & 15) + 1;  // the group's largest
+        const float c0 = sh.c0[k], c1 = sh.c1[k], c2 = sh.c2[k], rr = sh.rr[k], ri = sh.ri[k];
+        const float a00 = sh.a00[k], a11 = sh.a11[k], a22 = sh.a22[k];
+        const float b01 = sh.b01[k], b02 = sh.b02[k], b12 = sh.b12[k];
+        const int bx = sh.box[k];
+        const int z0 = bx & 15, own_rows = ((bx >> 4) & 15) + 1;
+        const int l1 = (bx >> 8) & 3, h1 = (bx >> 10) & 3, l2 = (bx >> 12) & 3, h2 = (bx >> 14) & 3;
+        const int d2 = h2 - l2 + 1;
+        (void)h1;
+        BwdAcc acc = {};
+        int j1 = 0, j2 = 0;
+        for (int j = 0; j < cc; ++j) {
+          const int s1 = (l1 + j1) * kSub, s2 = (l2 + j2) * kSub;
+          if (partial)
+            bwd_column<TT, true>(sh_g, T, b, z0, rows, own_rows, s1, s2, xi, slot_ok, c0, c1, c2, rr, ri, a00, a11,
+                                 a22, b01, b02, b12, acc);
+          else
+            bwd_column<TT, false>(sh_g, T, b, z0, rows, own_rows, s1, s2, xi, slot_ok, c0, c1, c2, rr, ri, a00, a11,
+                                  a22, b01, b02, b12, acc);
+          if (++j2 == d2) {
+            j2 = 0;
+            ++j1;
           }
         }
-      }
-      float v[kMoments] = {acc.dr, acc.di, acc.m0, acc.m1, acc.m2, acc.m00, acc.m11, acc.m22, acc.m01, acc.m02,
-                           acc.m12};
-      // Reduce-scatter over the 4 W lanes of each Gaussian: afterwards the
-      // lane in row xi holds the totals of moments 3*xi .. 3*xi+2.
-      float s6[6], t3[3];
+        float v[kMoments] = {acc.dr, acc.di, acc.m0, acc.m1, acc.m2, acc.m00, acc.m11, acc.m22, acc.m01, acc.m02,
+                             acc.m12};
+        // Reduce-scatter over the 4 W lanes of each Gaussian: afterwards the
+        // lane in row xi holds the totals of moments 3*xi .. 3*xi+2.
+        float s6[6], t3[3];
 #pragma unroll
-      for (int i = 0; i < 6; ++i) s6[i] = pair_sum_rows02(v[i], i + 6 < kMoments ? v[i + 6] : 0.0f);
+        for (int i = 0; i < 6; ++i) s6[i] = pair_sum_rows02(v[i], i + 6 < kMoments ? v[i + 6] : 0.0f);
 #pragma unroll
-      for (int i = 0; i < 3; ++i) t3[i] = pair_sum_rows01(s6[i], s6[i + 3]);
-      if (slot_ok) {
+        for (int i = 0; i < 3; ++i) t3[i] = pair_sum_rows01(s6[i], s6[i + 3]);
+        if (slot_ok) {
 #pragma unroll
-        for (int jj = 0; jj < 3; ++jj) sh.out[3 * xi + jj][k] = t3[jj];
+          for (int jj = 0; jj < 3; ++jj) sh.out[3 * xi + jj][k] = t3[jj];
+        }
       }
     }
     __syncthreads();
@@ -1062,7 +1109,7 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
         if (v != 0.0f) unsafeAtomicAdd(gacc + (int64_t)pair_ids[bs + p] * kAccStride + f, v);
       }
     }
-    if (tid <= kMaxClass) sh.cls_cnt[tid] = 0;
+    for (int i = tid; i < kKeys; i += kTileThreads) sh.key_cnt[i] = 0;
   }
 }
 
@@ -1236,7 +1283,7 @@ __global__ __launch_bounds__(kTileThreads) void k_count(const GRec* __restrict__
       const int idx = gb + gl;
       const bool g_ok = (idx < nhit) && x_ok;
       const int k = sh.list[wave][idx < nhit ? idx : gb];
-      const GaussLane g = load_gauss(sh, k, b, xi);
+      const GaussLane g = lane_terms(load_raw(sh, k), b, xi);
 #pragma unroll
       for (int z = 0; z < kSub; ++z) {
         if (z < b.n0) {
