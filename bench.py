@@ -6,8 +6,9 @@ forward tile kernel) on a 256x256x128 volume with 1M Gaussians (means uniform
 in the volume, log-normal voxel sigma with median 1.5 and sigma 0.5, random
 rotations, N(0,1)+iN(0,1) amplitudes, 8^3 tiles), then autograd of
 <G, vol> with a fixed random upstream gradient G down to positions,
-log-scales, quaternions and complex amplitudes.  Inputs are resident in HBM
-before the timed region.
+log-scales, quaternions and complex amplitudes (G goes to autograd directly;
+no loss reduction is computed).  Inputs are resident in HBM before the timed
+region.
 
 Prints one JSON line (rank 0).  The roofline entry is the dominant tile
 kernel's algorithmic FP32 rate (the reference kernel's FLOP count per
@@ -42,6 +43,12 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector, published (packed FP32)
 SHAPE = (256, 256, 128)
 N_GAUSS = 1_000_000
 TILE = 8
+
+# HBM bytes per k_backward launch at this workload, from two rocprofv3 passes
+# (FETCH_SIZE, WRITE_SIZE) scaled by the same counters' reading of a 512 MiB
+# copy: 545.4 MB read + 535.1 MB written.  Counters cannot be read from inside
+# this process, so the figure is a measured constant of the current kernels.
+BACKWARD_TRAFFIC_BYTES = 1_080_500_000
 
 # Algorithmic FLOPs per evaluation, counted from the reference kernel
 # (FMA = 2, mul/add/sub/div = 1, exp = 1): forward 26 ungated + 6 gated,
@@ -200,8 +207,8 @@ def main():
             p.grad = None
         vr, vi = vox(params["pos"], torch.exp(params["log_s"]), params["rot"],
                      torch.complex(params["dr"], params["di"]))
-        loss = (vr * grad_r).sum() + (vi * grad_i).sum()
-        loss.backward()
+        # Upstream gradient G of the loss <G, vol>, handed straight to autograd.
+        torch.autograd.backward((vr, vi), (grad_r, grad_i))
 
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -289,7 +296,7 @@ def main():
                    "gaussians": N_GAUSS, "tile": TILE, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "kernel": f"k_{dominant}", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None},
+                     "traffic": BACKWARD_TRAFFIC_BYTES if dominant == "backward" else None},
         "cpu_baseline": cpu,
         "learned_init": learned,
         "detail": {
