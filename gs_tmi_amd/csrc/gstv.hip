@@ -35,6 +35,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 
 #include "gstv.h"
 
@@ -342,21 +343,152 @@ __global__ __launch_bounds__(kExpandThreads) void k_bin_expand(
   }
 }
 
+// ---- pair generation inside the first radix pass ----
+//
+// The first radix pass does not read materialised pairs: each block expands
+// its kRadixTile consecutive pair slots (Gaussian-major, row-major-local
+// order, gaussian/tile_voxelizer.py:126-145) from the Gaussians' tile boxes.
+// The block's Gaussians are located by two binary searches in the offsets,
+// their boxes go to LDS (first tile id, tiles per D slab, tiles per H row),
+// and every slot gets its Gaussian from an LDS slot -> Gaussian map.
+
+struct GenLds {
+  int32_t off[kRadixTile + 1];
+  int32_t k0[kRadixTile + 1];
+  int32_t n12[kRadixTile + 1];
+  int32_t n2[kRadixTile + 1];
+  uint16_t gidx[kRadixTile];
+  int g0, ng;
+};
+
+struct ScatterLds {
+  int32_t key[kRadixTile];
+  int32_t val[kRadixTile];
+  int32_t wcnt[kRadixThreads / 64][1 << kRadixBits];
+  int32_t local[1 << kRadixBits];
+  int32_t base[1 << kRadixBits];
+  int64_t scan[kRadixThreads / 64];
+};
+
+// Owner Gaussian of the first slot of every kRadixTile-slot block
+// (first[nblk] = owner of the last slot), written by the Gaussians themselves:
+// no per-block search.
+__global__ void k_block_first(const int32_t* __restrict__ offsets, int n, int total, int32_t* __restrict__ first) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const int lo = offsets[g], hi = offsets[g + 1];
+  if (lo >= hi) return;
+  for (int b = (lo + kRadixTile - 1) / kRadixTile; b * (int64_t)kRadixTile < hi; ++b) first[b] = g;
+  if (lo <= total - 1 && total - 1 < hi) first[(total + kRadixTile - 1) / kRadixTile] = g;
+}
+
+// Block-wide: maps the slots [j0, j1) of block b to their Gaussians
+// (first[b] .. first[b + 1], an upper bound of the last owner).  Returns false
+// (uniformly) when more than kRadixTile + 1 Gaussians span the block, which
+// needs Gaussians with zero pairs inside it; callers then search per slot.
+__device__ bool gen_setup(GenLds& L, const float* __restrict__ center, const float* __restrict__ radius,
+                          const BinGeom& geo, const int32_t* __restrict__ offsets,
+                          const int32_t* __restrict__ first, int j0, int j1) {
+  const int b = j0 / kRadixTile;
+  const int g0 = first[b];
+  const int ng = first[b + 1] - g0 + 1;
+  if (threadIdx.x == 0) L.g0 = g0;
+  if (ng > kRadixTile + 1) return false;
+  const int ntw = geo.nt[2];
+  const int nthw = geo.nt[1] * geo.nt[2];
+  for (int i = threadIdx.x; i < ng; i += kRadixThreads) {
+    int tmin[3], tcnt[3];
+    gauss_tile_box(center, radius, g0 + i, geo, tmin, tcnt);
+    L.off[i] = offsets[g0 + i];
+    L.k0[i] = tmin[0] * nthw + tmin[1] * ntw + tmin[2];
+    L.n12[i] = tcnt[1] * tcnt[2];
+    L.n2[i] = tcnt[2];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < ng; i += kRadixThreads) {
+    const int s_end = min(i + 1 < ng ? L.off[i + 1] : offsets[g0 + ng], j1);
+    for (int sl = max(L.off[i], j0); sl < s_end; ++sl) L.gidx[sl - j0] = (uint16_t)i;
+  }
+  __syncthreads();
+  return true;
+}
+
+// Key (tile id) and value (Gaussian id) of pair slot j (j0 <= j < j1).
+__device__ __forceinline__ void gen_pair(const GenLds& L, int j, int j0, const BinGeom& geo, int32_t& key,
+                                         int32_t& val) {
+  const int i = L.gidx[j - j0];
+  const int l = j - L.off[i];
+  const int n12 = L.n12[i], n2 = L.n2[i];
+  const int ld = l / n12;
+  const int rem = l - ld * n12;
+  const int lh = rem / n2;
+  const int lw = rem - lh * n2;
+  key = L.k0[i] + ld * (geo.nt[1] * geo.nt[2]) + lh * geo.nt[2] + lw;
+  val = L.g0 + i;
+}
+
+// Pair slot j located with a binary search in the global offsets.
+__device__ __forceinline__ void gen_pair_global(const float* __restrict__ center, const float* __restrict__ radius,
+                                                int n, const BinGeom& geo, const int32_t* __restrict__ offsets, int j,
+                                                int32_t& key, int32_t& val) {
+  const int g = last_le(offsets, n + 1, j);
+  int tmin[3], tcnt[3];
+  gauss_tile_box(center, radius, g, geo, tmin, tcnt);
+  const int l = j - offsets[g];
+  const int n12 = tcnt[1] * tcnt[2];
+  const int ld = l / n12;
+  const int rem = l - ld * n12;
+  const int lh = rem / tcnt[2];
+  const int lw = rem - lh * tcnt[2];
+  key = (tmin[0] + ld) * (geo.nt[1] * geo.nt[2]) + (tmin[1] + lh) * geo.nt[2] + (tmin[2] + lw);
+  val = g;
+}
+
+// Where the pairs of a radix pass come from: materialised arrays, or (first
+// pass) generated from the Gaussians' tile boxes.
+struct PairSource {
+  const int32_t* keys;
+  const int32_t* vals;
+  const float* center;
+  const float* radius;
+  const int32_t* offsets;
+  const int32_t* first;  // owner of each block's first slot (k_block_first)
+  int n;
+  BinGeom geo;
+};
+
 // Radix upsweep: digit histogram of each kRadixTile-key block, stored
 // digit-major (hist[digit * nblocks + block]) so one exclusive scan gives
 // every block's scatter base per digit.
-__global__ __launch_bounds__(kRadixThreads) void k_radix_hist(const int32_t* __restrict__ keys, int total,
-                                                             int shift, int bits, int nblocks,
-                                                             int32_t* __restrict__ hist) {
+template <bool kGen>
+__global__ __launch_bounds__(kRadixThreads) void k_radix_hist(PairSource src, int total, int shift, int bits,
+                                                             int nblocks, int32_t* __restrict__ hist) {
+  struct NoGen {};
   __shared__ int32_t sh_h[1 << kRadixBits];
+  __shared__ std::conditional_t<kGen, GenLds, NoGen> gen_lds;
   sh_h[threadIdx.x] = 0;
-  __syncthreads();
   const int mask = (1 << bits) - 1;
   const int base = blockIdx.x * kRadixTile;
+  bool gen_ok = false;
+  if constexpr (kGen)
+    gen_ok = gen_setup(gen_lds, src.center, src.radius, src.geo, src.offsets, src.first, base,
+                       min(base + kRadixTile, total));
+  __syncthreads();
 #pragma unroll 4
   for (int r = 0; r < kRadixRounds; ++r) {
     const int idx = base + r * kRadixThreads + threadIdx.x;
-    if (idx < total) atomicAdd(&sh_h[(keys[idx] >> shift) & mask], 1);
+    if (idx < total) {
+      int32_t key, val;
+      if constexpr (!kGen) {
+        key = src.keys[idx];
+      } else {
+        if (gen_ok)
+          gen_pair(gen_lds, idx, base, src.geo, key, val);
+        else
+          gen_pair_global(src.center, src.radius, src.n, src.geo, src.offsets, idx, key, val);
+      }
+      atomicAdd(&sh_h[(key >> shift) & mask], 1);
+    }
   }
   __syncthreads();
   if (threadIdx.x < (1 << bits)) hist[threadIdx.x * nblocks + blockIdx.x] = sh_h[threadIdx.x];
@@ -368,39 +500,72 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_hist(const int32_t* __r
 // the per-digit wave prefixes then give every key its position in a
 // block-local, digit-sorted copy in LDS, which is written out so that
 // consecutive threads store consecutive addresses of a digit's run.
-__global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
-    const int32_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in, int total, int shift, int bits,
-    int nblocks, const int32_t* __restrict__ hist_scanned, int32_t* __restrict__ keys_out,
-    int32_t* __restrict__ vals_out) {
-  constexpr int kDigits = 1 << kRadixBits;
+//
+// kLast (final pass): only the values are written (they are pair_ids), and
+// tile_ranges come out of the same pass: the block's copy is sorted by the
+// whole key, so the first and last copy of each key in it mark the ends of a
+// sub-run of that tile's global range, merged with atomicMin / atomicMax
+// into ranges pre-set to (INT_MAX, 0).
+template <bool kGen, bool kLast>
+__device__ __forceinline__ void radix_scatter(const PairSource& src, int total, int shift, int bits, int nblocks,
+                                              const int32_t* __restrict__ hist_scanned,
+                                              int32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out,
+                                              int32_t* __restrict__ ranges) {
   constexpr int kW = kRadixThreads / 64;
   constexpr int kPerWave = kRadixTile / kW;
   constexpr int kRounds = kPerWave / 64;
-  __shared__ int32_t sh_key[kRadixTile];
-  __shared__ int32_t sh_val[kRadixTile];
-  __shared__ int32_t sh_wcnt[kW][kDigits];
-  __shared__ int32_t sh_local[kDigits];
-  __shared__ int32_t sh_base[kDigits];
-  __shared__ int64_t sh_scan[kRadixThreads / 64];
+  union GenScatterLds {
+    GenLds gen;
+    ScatterLds sc;
+  };
+  struct ScatterOnly {
+    ScatterLds sc;
+  };
+  __shared__ std::conditional_t<kGen, GenScatterLds, ScatterOnly> L;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
   const int ndig = 1 << bits;
   const int mask = ndig - 1;
+  const int block0 = blockIdx.x * kRadixTile;
+  const int wbase = block0 + wid * kPerWave;
+
+  int32_t kk[kRounds], vv[kRounds], rk[kRounds];
+  if constexpr (kGen) {
+    const bool ok = gen_setup(L.gen, src.center, src.radius, src.geo, src.offsets, src.first, block0,
+                              min(block0 + kRadixTile, total));
 #pragma unroll
-  for (int w = 0; w < kW; ++w) sh_wcnt[w][tid] = 0;
-  sh_base[tid] = tid < ndig ? hist_scanned[tid * nblocks + blockIdx.x] : 0;
+    for (int r = 0; r < kRounds; ++r) {
+      const int idx = wbase + r * 64 + lane;
+      kk[r] = 0;
+      vv[r] = 0;
+      if (idx < total) {
+        if (ok)
+          gen_pair(L.gen, idx, block0, src.geo, kk[r], vv[r]);
+        else
+          gen_pair_global(src.center, src.radius, src.n, src.geo, src.offsets, idx, kk[r], vv[r]);
+      }
+    }
+    __syncthreads();  // the LDS is reused below
+  } else {
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+      const int idx = wbase + r * 64 + lane;
+      kk[r] = idx < total ? src.keys[idx] : 0;
+      vv[r] = idx < total ? src.vals[idx] : 0;
+    }
+  }
+  ScatterLds& sh = L.sc;
+#pragma unroll
+  for (int w = 0; w < kW; ++w) sh.wcnt[w][tid] = 0;
+  sh.base[tid] = tid < ndig ? hist_scanned[tid * nblocks + blockIdx.x] : 0;
   __syncthreads();
 
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int wbase = blockIdx.x * kRadixTile + wid * kPerWave;
-  int32_t kk[kRounds], vv[kRounds], rk[kRounds];
 #pragma unroll
   for (int r = 0; r < kRounds; ++r) {
     const int idx = wbase + r * 64 + lane;
     const bool valid = idx < total;
-    const int32_t key = valid ? keys_in[idx] : 0;
-    const int32_t val = valid ? vals_in[idx] : 0;
-    const int dig = (key >> shift) & mask;
+    const int dig = (kk[r] >> shift) & mask;
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < bits; ++b) {
       const bool bit = (dig >> b) & 1;
@@ -408,10 +573,8 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
       peers &= bit ? bb : ~bb;
     }
     const int wrank = __popcll(peers & lt_mask);
-    const int prev = sh_wcnt[wid][dig];  // read by the whole wave before its leader updates it
-    if (valid && wrank == 0) sh_wcnt[wid][dig] = prev + __popcll(peers);
-    kk[r] = key;
-    vv[r] = val;
+    const int prev = sh.wcnt[wid][dig];  // read by the whole wave before its leader updates it
+    if (valid && wrank == 0) sh.wcnt[wid][dig] = prev + __popcll(peers);
     rk[r] = valid ? prev + wrank : -1;
   }
   __syncthreads();
@@ -421,32 +584,74 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
     int run = 0;
 #pragma unroll
     for (int w = 0; w < kW; ++w) {
-      const int c = sh_wcnt[w][tid];
-      sh_wcnt[w][tid] = run;
+      const int c = sh.wcnt[w][tid];
+      sh.wcnt[w][tid] = run;
       run += c;
     }
     int64_t tot;
-    sh_local[tid] = (int32_t)block_exclusive_scan(run, sh_scan, &tot);
+    sh.local[tid] = (int32_t)block_exclusive_scan(run, sh.scan, &tot);
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kRounds; ++r) {
     if (rk[r] >= 0) {
       const int dig = (kk[r] >> shift) & mask;
-      const int lpos = sh_local[dig] + sh_wcnt[wid][dig] + rk[r];
-      sh_key[lpos] = kk[r];
-      sh_val[lpos] = vv[r];
+      const int lpos = sh.local[dig] + sh.wcnt[wid][dig] + rk[r];
+      sh.key[lpos] = kk[r];
+      sh.val[lpos] = vv[r];
     }
   }
   __syncthreads();
-  const int nblk = min(kRadixTile, total - (int)blockIdx.x * kRadixTile);
+  const int nblk = min(kRadixTile, total - block0);
   for (int i = tid; i < nblk; i += kRadixThreads) {
-    const int32_t key = sh_key[i];
+    const int32_t key = sh.key[i];
     const int dig = (key >> shift) & mask;
-    const int out = sh_base[dig] + (i - sh_local[dig]);
-    keys_out[out] = key;
-    vals_out[out] = sh_val[i];
+    const int out = sh.base[dig] + (i - sh.local[dig]);
+    vals_out[out] = sh.val[i];
+    if (!kLast) {
+      keys_out[out] = key;
+    } else {
+      if (i == 0 || sh.key[i - 1] != key) atomicMin(&ranges[2 * (int64_t)key], out);
+      if (i == nblk - 1 || sh.key[i + 1] != key) atomicMax(&ranges[2 * (int64_t)key + 1], out + 1);
+    }
   }
+}
+
+// First pass (pairs generated in the kernel; LDS limits it to 2 blocks per CU).
+template <bool kLast>
+__global__ __launch_bounds__(kRadixThreads) void k_radix_scatter_gen(PairSource src, int total, int shift, int bits,
+                                                                    int nblocks,
+                                                                    const int32_t* __restrict__ hist_scanned,
+                                                                    int32_t* __restrict__ keys_out,
+                                                                    int32_t* __restrict__ vals_out,
+                                                                    int32_t* __restrict__ ranges) {
+  radix_scatter<true, kLast>(src, total, shift, bits, nblocks, hist_scanned, keys_out, vals_out, ranges);
+}
+
+// Later passes (pairs read from the previous pass; 4 blocks per CU).
+template <bool kLast>
+__global__ __launch_bounds__(kRadixThreads, 4) void k_radix_scatter(PairSource src, int total, int shift, int bits,
+                                                                    int nblocks,
+                                                                    const int32_t* __restrict__ hist_scanned,
+                                                                    int32_t* __restrict__ keys_out,
+                                                                    int32_t* __restrict__ vals_out,
+                                                                    int32_t* __restrict__ ranges) {
+  radix_scatter<false, kLast>(src, total, shift, bits, nblocks, hist_scanned, keys_out, vals_out, ranges);
+}
+
+// tile_ranges of the fused final pass: preset (INT_MAX, 0), then tiles with
+// no pair get (0, 0) as in the reference.
+__global__ void k_ranges_preset(int32_t* __restrict__ ranges, int num_tiles) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < num_tiles) {
+    ranges[2 * (int64_t)t] = INT_MAX;
+    ranges[2 * (int64_t)t + 1] = 0;
+  }
+}
+
+__global__ void k_ranges_fix(int32_t* __restrict__ ranges, int num_tiles) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < num_tiles && ranges[2 * (int64_t)t] == INT_MAX) ranges[2 * (int64_t)t] = 0;
 }
 
 // tile_ranges from sorted keys; gaussian/tile_voxelizer.py:151-161.
@@ -1353,17 +1558,22 @@ VoxGeom make_vox_geom(int d, int h, int w, int tile, const int nt[3], int num_ti
   return g;
 }
 
+// Binning workspace: one key/value buffer pair (the first radix pass output,
+// or the expansion when there is a single tile), a second pair only when three
+// or more radix passes ping-pong, the digit histograms and scan sums, and the
+// per-block first Gaussians.
 size_t fill_ws_layout(int64_t total, int num_tiles, Carve& cv, int32_t** ka, int32_t** va, int32_t** kb,
-                      int32_t** vb, int32_t** hist, int64_t** sums) {
+                      int32_t** vb, int32_t** hist, int64_t** sums, int32_t** first) {
   const int64_t nblk = ceil_div_l(total, kRadixTile);
   const int64_t hist_n = (int64_t)(1 << kRadixBits) * nblk;
+  const size_t second = radix_passes(num_tiles) >= 3 ? (size_t)total : 0;
   *ka = cv.take<int32_t>((size_t)total);
   *va = cv.take<int32_t>((size_t)total);
-  *kb = cv.take<int32_t>((size_t)total);
-  *vb = cv.take<int32_t>((size_t)total);
+  *kb = cv.take<int32_t>(second);
+  *vb = cv.take<int32_t>(second);
   *hist = cv.take<int32_t>((size_t)hist_n);
   *sums = cv.take<int64_t>((size_t)ceil_div_l(hist_n, kScanTile) + 1);
-  (void)num_tiles;
+  *first = cv.take<int32_t>((size_t)nblk + 1);
   return align_up(cv.off, 256);
 }
 
@@ -1454,9 +1664,9 @@ size_t gstv_bin_fill_workspace_bytes(int32_t n, int64_t total_pairs, int32_t num
   (void)n;
   if (total_pairs < 0) return 0;
   Carve cv(nullptr);
-  int32_t *ka, *va, *kb, *vb, *hist;
+  int32_t *ka, *va, *kb, *vb, *hist, *first;
   int64_t* sums;
-  return fill_ws_layout(total_pairs, num_tiles, cv, &ka, &va, &kb, &vb, &hist, &sums);
+  return fill_ws_layout(total_pairs, num_tiles, cv, &ka, &va, &kb, &vb, &hist, &sums, &first);
 }
 
 int gstv_bin_fill(const float* center_vox, const float* radius_vox, int32_t n, int32_t d, int32_t h,
@@ -1474,51 +1684,87 @@ int gstv_bin_fill(const float* center_vox, const float* radius_vox, int32_t n, i
   geo.size[2] = w;
   geo.tile = tile;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_fill_i32, dim3(ceil_div_i(2 * num_tiles, 256)), dim3(256), 0, s, tile_ranges,
-                     (int64_t)2 * num_tiles, 0);
-  GSTV_LAUNCH_CHECK("ranges_clear");
   const int total = (int)total_pairs;
-  if (total == 0) return GSTV_OK;
+  const int passes = radix_passes(num_tiles);
+  if (total == 0 || passes == 0) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(ceil_div_i(2 * num_tiles, 256)), dim3(256), 0, s, tile_ranges,
+                       (int64_t)2 * num_tiles, 0);
+    GSTV_LAUNCH_CHECK("ranges_clear");
+    if (total == 0) return GSTV_OK;
+  }
   if (n <= 0 || center_vox == nullptr || radius_vox == nullptr || gauss_offsets == nullptr ||
       pair_ids == nullptr)
     return GSTV_ERR_INVALID_ARGUMENT;
   if (workspace == nullptr || workspace_bytes < gstv_bin_fill_workspace_bytes(n, total_pairs, num_tiles))
     return GSTV_ERR_WORKSPACE_TOO_SMALL;
   Carve cv(workspace);
-  int32_t *ka, *va, *kb, *vb, *hist;
+  int32_t *ka, *va, *kb, *vb, *hist, *first;
   int64_t* sums;
-  fill_ws_layout(total_pairs, num_tiles, cv, &ka, &va, &kb, &vb, &hist, &sums);
+  fill_ws_layout(total_pairs, num_tiles, cv, &ka, &va, &kb, &vb, &hist, &sums, &first);
 
-  const int passes = radix_passes(num_tiles);
+  if (passes == 0) {
+    // A single tile: the expansion order is final.
+    hipLaunchKernelGGL(k_bin_expand, dim3(ceil_div_i(total, kExpandTile)), dim3(kExpandThreads), 0, s, center_vox,
+                       radius_vox, n, geo, gauss_offsets, total, ka, pair_ids);
+    GSTV_LAUNCH_CHECK("bin_expand");
+    hipLaunchKernelGGL(k_ranges, dim3(ceil_div_i(total, 256)), dim3(256), 0, s, ka, total, tile_ranges);
+    GSTV_LAUNCH_CHECK("ranges");
+    return GSTV_OK;
+  }
+
+  hipLaunchKernelGGL(k_ranges_preset, dim3(ceil_div_i(num_tiles, 256)), dim3(256), 0, s, tile_ranges, num_tiles);
+  GSTV_LAUNCH_CHECK("ranges_preset");
+  hipLaunchKernelGGL(k_block_first, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, gauss_offsets, n, total, first);
+  GSTV_LAUNCH_CHECK("block_first");
   const int bits_total = key_bits(num_tiles);
-  const int width = passes > 0 ? (bits_total + passes - 1) / passes : 0;  // balanced digit widths
-  // With no radix pass (a single tile) the expansion order is already final.
-  int32_t* first_vals = passes == 0 ? pair_ids : va;
-  hipLaunchKernelGGL(k_bin_expand, dim3(ceil_div_i(total, kExpandTile)), dim3(kExpandThreads), 0, s, center_vox,
-                     radius_vox, n, geo, gauss_offsets, total, ka, first_vals);
-  GSTV_LAUNCH_CHECK("bin_expand");
-
+  const int width = (bits_total + passes - 1) / passes;  // balanced digit widths
   const int nblk = ceil_div_i(total, kRadixTile);
-  int32_t* kin = ka;
-  int32_t* vin = va;
+  int32_t* buf_k[2] = {ka, kb};
+  int32_t* buf_v[2] = {va, vb};
+  int cur = -1;  // -1: the first pass generates its pairs
   for (int p = 0; p < passes; ++p) {
     const int shift = p * width;
     const int bits = std::min(width, bits_total - shift);
-    int32_t* kout = (kin == ka) ? kb : ka;
-    int32_t* vout = (p == passes - 1) ? pair_ids : ((vin == va) ? vb : va);
-    hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kRadixThreads), 0, s, kin, total, shift, bits, nblk, hist);
+    const bool last = p == passes - 1;
+    PairSource src;
+    src.keys = cur >= 0 ? buf_k[cur] : nullptr;
+    src.vals = cur >= 0 ? buf_v[cur] : nullptr;
+    src.center = center_vox;
+    src.radius = radius_vox;
+    src.offsets = gauss_offsets;
+    src.first = first;
+    src.n = n;
+    src.geo = geo;
+    const int dst = cur >= 0 ? 1 - cur : 0;
+    int32_t* kout = buf_k[dst];
+    int32_t* vout = last ? pair_ids : buf_v[dst];
+    if (cur < 0)
+      hipLaunchKernelGGL(k_radix_hist<true>, dim3(nblk), dim3(kRadixThreads), 0, s, src, total, shift, bits, nblk,
+                         hist);
+    else
+      hipLaunchKernelGGL(k_radix_hist<false>, dim3(nblk), dim3(kRadixThreads), 0, s, src, total, shift, bits, nblk,
+                         hist);
     GSTV_LAUNCH_CHECK("radix_hist");
     const int64_t hist_n = (int64_t)(1 << bits) * nblk;
     int st = launch_exclusive_scan(hist, hist_n, hist, sums, nullptr, s);
     if (st != GSTV_OK) return st;
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kRadixThreads), 0, s, kin, vin, total, shift, bits, nblk,
-                       hist, kout, vout);
+    if (cur < 0 && last)
+      hipLaunchKernelGGL((k_radix_scatter_gen<true>), dim3(nblk), dim3(kRadixThreads), 0, s, src, total, shift,
+                         bits, nblk, hist, kout, vout, tile_ranges);
+    else if (cur < 0)
+      hipLaunchKernelGGL((k_radix_scatter_gen<false>), dim3(nblk), dim3(kRadixThreads), 0, s, src, total, shift,
+                         bits, nblk, hist, kout, vout, tile_ranges);
+    else if (last)
+      hipLaunchKernelGGL((k_radix_scatter<true>), dim3(nblk), dim3(kRadixThreads), 0, s, src, total, shift,
+                         bits, nblk, hist, kout, vout, tile_ranges);
+    else
+      hipLaunchKernelGGL((k_radix_scatter<false>), dim3(nblk), dim3(kRadixThreads), 0, s, src, total, shift,
+                         bits, nblk, hist, kout, vout, tile_ranges);
     GSTV_LAUNCH_CHECK("radix_scatter");
-    kin = kout;
-    vin = vout;
+    cur = dst;
   }
-  hipLaunchKernelGGL(k_ranges, dim3(ceil_div_i(total, 256)), dim3(256), 0, s, kin, total, tile_ranges);
-  GSTV_LAUNCH_CHECK("ranges");
+  hipLaunchKernelGGL(k_ranges_fix, dim3(ceil_div_i(num_tiles, 256)), dim3(256), 0, s, tile_ranges, num_tiles);
+  GSTV_LAUNCH_CHECK("ranges_fix");
   return GSTV_OK;
 }
 

@@ -144,6 +144,28 @@ def test_empty_and_degenerate_inputs():
     assert all(t.numel() == 0 for t in out)
 
 
+@pytest.mark.parametrize("shape,tile,frac", [((64, 64, 64), 8, 0.97), ((64, 64, 64), 8, 0.85),
+                                             ((20, 20, 20), 1, 0.85), ((8, 8, 8), 8, 0.85)])
+def test_binning_with_many_pairless_gaussians(shape, tile, frac):
+    """A fraction `frac` of the Gaussians has radius -3: their inverted boxes
+    straddle a tile boundary and give no pairs, so a first-pass radix block's
+    4096 slots span up to ~22k Gaussians (more than its LDS map holds: the
+    per-slot search path runs); the single-tile volume takes the no-radix path.
+    Bit-exact against the oracle."""
+    from gs_tmi_amd import build_tile_pairs
+
+    _announce(f"binning with pairless Gaussians shape={shape} tile={tile} frac={frac}")
+    rng = np.random.default_rng(21)
+    n = 30000
+    center = (rng.random((n, 3)) * np.asarray(shape)).astype(np.float32)
+    radius = np.where(rng.random(n) < frac, -3.0, rng.random(n) * 3.0).astype(np.float32)
+    ranges, pids = build_tile_pairs(torch.from_numpy(center).to(DEV), torch.from_numpy(radius).to(DEV), shape, tile)
+    torch.cuda.synchronize()
+    r_exp, p_exp = vo.bin_tiles(center, radius, shape, tile)
+    np.testing.assert_array_equal(ranges.cpu().numpy(), r_exp)
+    np.testing.assert_array_equal(pids.cpu().numpy(), p_exp)
+
+
 def test_forward_writes_every_voxel():
     """Outputs come from torch.empty: every voxel, including empty tiles, must be written."""
     from gs_tmi_amd import _native
