@@ -4,8 +4,8 @@
     python -m gs_tmi_amd.learned.pipeline evaluate --weights W.safetensors --out R.json
     python -m gs_tmi_amd.learned.pipeline all      --phantoms 200 --weights W --out R.json
 
-Training acquisitions use seeds 1000 + i (the evaluation acquisition uses seed
-0, so its phantom is not among them).  Evaluation runs the config-4 workload
+Training acquisitions use seeds 1000 + i (evaluation acquisitions use seeds
+below 1000, by default 0, so their phantoms are not among them).  Evaluation runs the config-4 workload
 (256 x 256 x 128, 60 ellipsoids, 16 coils, Poisson-disc R=8 with a 24 x 24
 calibration centre, noise 1e-3) from both initialisations with identical
 optimiser settings under two schedules:
@@ -86,35 +86,41 @@ def run_evaluate(args, log, model=None):
 
     dev = torch.device("cuda:0")
     shape = tuple(args.shape)
-    mask = poisson_disc_mask((shape[1], shape[2]), 8, 24 * shape[1] // 256, 0)
-    problem = make_problem(shape, 60, 16, mask, 0, noise_sigma=1e-3, device=dev, phantom_on_device=True)
     vox = TileVoxelizer(shape, tile_size=8, device="cuda:0")
     learned_init = LearnedInitialiser(model=model, weights=None if model is not None else args.weights, device=dev,
                                       threshold=args.threshold)
     results = {"workload": {"shape": shape, "coils": 16, "accel": 8, "calib": 24 * shape[1] // 256},
-               "threshold_nmse": args.target_nmse, "runs": {}}
-    for name, (n_plain, cfg) in schedules().items():
-        if args.schedules and name not in args.schedules:
-            continue
-        for kind in ("learned", "plain"):
-            g, info = initialise(problem, vox, learned_init if kind == "learned" else None, plain_points=n_plain)
-            rec = Reconstructor(problem, g, vox, cfg)
-            hist = rec.run(args.steps, eval_every=args.eval_every,
-                           log=lambda r, k=kind, n=name: log({"schedule": n, "init": k, **r}))
-            run = {"init": info, "rows": hist.rows, "final": hist.rows[-1],
-                   "time_to_target_s": _time_to(hist.rows, info["init_s"], args.target_nmse),
-                   "train_time_to_target_s": hist.time_to("nmse", args.target_nmse)}
-            results["runs"][f"{name}/{kind}"] = run
-            log({"schedule": name, "init": kind, "used": info["used"], "init_s": info["init_s"],
-                 "time_to_target_s": run["time_to_target_s"], "final": hist.rows[-1]})
-            del rec, g
-            torch.cuda.empty_cache()
-    for name in schedules():
-        a, b = results["runs"].get(f"{name}/learned"), results["runs"].get(f"{name}/plain")
-        if a and b:
-            ta, tb = a["time_to_target_s"], b["time_to_target_s"]
-            results[f"{name}_ratio"] = (ta / tb) if (ta is not None and tb is not None) else None
-            results[f"{name}_final_psnr_diff_db"] = a["final"]["psnr"] - b["final"]["psnr"]
+               "threshold_nmse": args.target_nmse, "seeds": args.seeds, "stop_at_target": args.stop_at_target,
+               "runs": {}}
+    for seed in args.seeds:
+        mask = poisson_disc_mask((shape[1], shape[2]), 8, 24 * shape[1] // 256, seed)
+        problem = make_problem(shape, 60, 16, mask, seed, noise_sigma=1e-3, device=dev, phantom_on_device=True)
+        for name, (n_plain, cfg) in schedules().items():
+            if args.schedules and name not in args.schedules:
+                continue
+            for kind in ("learned", "plain"):
+                g, info = initialise(problem, vox, learned_init if kind == "learned" else None, plain_points=n_plain)
+                rec = Reconstructor(problem, g, vox, cfg)
+                hist = rec.run(args.steps, eval_every=args.eval_every,
+                               target_nmse=args.target_nmse if args.stop_at_target else None,
+                               log=lambda r, k=kind, n=name, sd=seed: log({"seed": sd, "schedule": n, "init": k, **r}))
+                run = {"init": info, "rows": hist.rows, "final": hist.rows[-1],
+                       "time_to_target_s": _time_to(hist.rows, info["init_s"], args.target_nmse),
+                       "train_time_to_target_s": hist.time_to("nmse", args.target_nmse)}
+                tag = f"{name}/{kind}" if len(args.seeds) == 1 else f"seed{seed}/{name}/{kind}"
+                results["runs"][tag] = run
+                log({"seed": seed, "schedule": name, "init": kind, "used": info["used"], "init_s": info["init_s"],
+                     "time_to_target_s": run["time_to_target_s"], "final": hist.rows[-1]})
+                del rec, g
+                torch.cuda.empty_cache()
+            pre = "" if len(args.seeds) == 1 else f"seed{seed}/"
+            a, b = results["runs"].get(f"{pre}{name}/learned"), results["runs"].get(f"{pre}{name}/plain")
+            if a and b:
+                ta, tb = a["time_to_target_s"], b["time_to_target_s"]
+                results[f"{pre}{name}_ratio"] = (ta / tb) if (ta is not None and tb is not None) else None
+                results[f"{pre}{name}_final_psnr_diff_db"] = a["final"]["psnr"] - b["final"]["psnr"]
+        del problem
+        torch.cuda.empty_cache()
     if args.out:
         Path(args.out).parent.mkdir(parents=True, exist_ok=True)
         with open(args.out, "w") as f:
@@ -144,6 +150,8 @@ def main(argv=None):
     ap.add_argument("--threshold", type=float, default=0.5)
     ap.add_argument("--shape", type=int, nargs=3, default=[256, 256, 128])
     ap.add_argument("--schedules", nargs="*", default=None)
+    ap.add_argument("--seeds", type=int, nargs="+", default=[0], help="evaluation acquisitions (not training seeds)")
+    ap.add_argument("--stop-at-target", action="store_true", help="stop each run at the target NMSE")
     args = ap.parse_args(argv)
     log = _log_to(args.log)
     if args.weights is None:
