@@ -752,6 +752,7 @@ struct Stage {
   float c0[kBatch], c1[kBatch], c2[kBatch], rr[kBatch], ri[kBatch];
   float a00[kBatch], a11[kBatch], a22[kBatch], b01[kBatch], b02[kBatch], b12[kBatch];
   uint32_t box[kBatch];
+  uint8_t sbm[kBatch];  // sub-bricks touched, bit 4*j0 + 2*j1 + j2 (tiles of at most 2x2x2 sub-bricks)
   uint16_t list[kWaves][kBatch];
 };
 
@@ -830,19 +831,32 @@ __device__ __forceinline__ void stage_pair(Stage& sh, int k, const GRec& r, cons
     const uint32_t h0 = (uint32_t)((int)hi0 - b.o0), h1 = (uint32_t)((int)hi1 - b.o1),
                    h2 = (uint32_t)((int)hi2 - b.o2);
     box = l0 | (l1 << 5) | (l2 << 10) | (h0 << 15) | (h1 << 20) | (h2 << 25);
+    // Sub-bricks touched along each axis (bits j = 0, 1), combined into the
+    // 2x2x2 mask; only read when the tile has at most 2 sub-bricks per axis.
+    const uint32_t m0 = ((l0 >> 2) == 0 ? 1u : 0u) | ((h0 >> 2) == 1 ? 2u : 0u);
+    const uint32_t m1 = ((l1 >> 2) == 0 ? 1u : 0u) | ((h1 >> 2) == 1 ? 2u : 0u);
+    const uint32_t m2 = ((l2 >> 2) == 0 ? 1u : 0u) | ((h2 >> 2) == 1 ? 2u : 0u);
+    const uint32_t plane = ((m1 & 1u) ? m2 : 0u) | ((m1 & 2u) ? (m2 << 2) : 0u);
+    sh.sbm[k] = (uint8_t)(((m0 & 1u) ? plane : 0u) | ((m0 & 2u) ? (plane << 4) : 0u));
+  } else {
+    sh.sbm[k] = 0;
   }
   sh.box[k] = box;
 }
 
 // Compacts, in batch order, the staged Gaussians whose box touches this
 // wave's sub-brick into sh.list[wave]; returns the count (wave-uniform).
-__device__ __forceinline__ int build_list(Stage& sh, int cnt, const BlockCtx& b, int wave, int lane) {
+// Tiles of at most 2 sub-bricks per axis test one bit of the staged mask.
+__device__ __forceinline__ int build_list(Stage& sh, int cnt, const BlockCtx& b, int wave, int lane, bool use_mask) {
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int sbi = (b.s0 >> 2) * 4 + (b.s1 >> 2) * 2 + (b.s2 >> 2);
   int nhit = 0;
   for (int base = 0; base < cnt; base += 64) {
     const int k = base + lane;
     bool hit = false;
-    if (k < cnt) {
+    if (k < cnt && use_mask) {
+      hit = (sh.sbm[k] >> sbi) & 1;
+    } else if (k < cnt) {
       const uint32_t bx = sh.box[k];
       const int lo0 = bx & 31, lo1 = (bx >> 5) & 31, lo2 = (bx >> 10) & 31;
       const int hi0 = (bx >> 15) & 31, hi1 = (bx >> 20) & 31, hi2 = (bx >> 25) & 31;
@@ -957,7 +971,7 @@ __global__ __launch_bounds__(kTileThreads) void k_forward(const GRec* __restrict
     if (bn < b.p_end && (int)threadIdx.x < min(kBatch, b.p_end - bn)) nxt = rec[pair_ids[bn + threadIdx.x]];
     __syncthreads();
     if (!b.active) continue;
-    const int nhit = build_list(sh, cnt, b, wave, lane);
+    const int nhit = build_list(sh, cnt, b, wave, lane, geo.nsb <= 2);
     if (full)
       fwd_groups<false>(sh, nhit, b, wave, gl, xi, x_ok, acc_r, acc_i);
     else
@@ -1483,7 +1497,7 @@ __global__ __launch_bounds__(kTileThreads) void k_count(const GRec* __restrict__
     if ((int)threadIdx.x < cnt) stage_pair(sh, threadIdx.x, rec[pair_ids[bs + threadIdx.x]], b);
     __syncthreads();
     if (!b.active) continue;
-    const int nhit = build_list(sh, cnt, b, wave, lane);
+    const int nhit = build_list(sh, cnt, b, wave, lane, geo.nsb <= 2);
     for (int gb = 0; gb < nhit; gb += 16) {
       const int idx = gb + gl;
       const bool g_ok = (idx < nhit) && x_ok;
