@@ -6,10 +6,12 @@ and decodes the slots whose existence probability exceeds the threshold.
 
 ``initialise`` judges every learned initialisation by its data-consistency
 residual ||M F S x - y|| / ||y|| (and reports its NMSE against the phantom
-when the phantom is known): when the residual is not below that of the plain
-initialisation (gaussian/gaussian_model.py:308-355 of the reference, here
-``GaussianSet.from_image``), or the network proposes no Gaussian, the plain
-set is used instead.
+when the phantom is known).  A residual below `accept_residual` (the learned
+set explains at least 1 - 0.3^2 = 91 % of the measured k-space energy) is
+accepted outright; otherwise the learned set must beat the plain
+initialisation's residual (gaussian/gaussian_model.py:308-355 of the
+reference, here ``GaussianSet.from_image``), and when it does not, or the
+network proposes no Gaussian, the plain set is used instead.
 """
 
 from __future__ import annotations
@@ -69,24 +71,27 @@ def judge(problem, gaussians: GaussianSet, voxelizer) -> dict:
 
 
 def initialise(problem, voxelizer, initialiser: LearnedInitialiser | None, plain_points=300_000, seed=0,
-               plain_scale_vox=None):
-    """Learned initialisation judged against the plain one; returns (GaussianSet, info)."""
+               plain_scale_vox=None, accept_residual=0.3):
+    """Learned initialisation, judged, with the plain one as fallback; returns (GaussianSet, info)."""
     sync = torch.cuda.synchronize if problem.target.is_cuda else (lambda: None)
     sync()
     t0 = time.perf_counter()
     info = {"used": "plain"}
-    learned = None
+    chosen = None
     if initialiser is not None:
         params = initialiser.propose(problem.zero_filled)
         if params[0].shape[0] > 0:
             learned = GaussianSet(*params)
             info["learned"] = judge(problem, learned, voxelizer)
-    plain = GaussianSet.from_image(problem.zero_filled, plain_points, mode="importance", scale_vox=plain_scale_vox,
-                                   seed=seed)
-    info["plain"] = judge(problem, plain, voxelizer)
-    chosen = plain
-    if learned is not None and info["learned"]["dc_residual"] < info["plain"]["dc_residual"]:
-        chosen, info["used"] = learned, "learned"
+            if info["learned"]["dc_residual"] < accept_residual:
+                chosen, info["used"] = learned, "learned"
+    if chosen is None:
+        plain = GaussianSet.from_image(problem.zero_filled, plain_points, mode="importance",
+                                       scale_vox=plain_scale_vox, seed=seed)
+        info["plain"] = judge(problem, plain, voxelizer)
+        chosen = plain
+        if "learned" in info and info["learned"]["dc_residual"] < info["plain"]["dc_residual"]:
+            chosen, info["used"] = learned, "learned"
     sync()
     info["init_s"] = time.perf_counter() - t0
     return chosen, info

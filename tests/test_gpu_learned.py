@@ -45,7 +45,7 @@ def test_learned_init_beats_plain_on_held_out_acquisition():
     g_l, info_l = initialise(p, vox, LearnedInitialiser(device=DEV), plain_points=n_plain)
     g_p, info_p = initialise(p, vox, None, plain_points=n_plain)
     assert info_l["used"] == "learned" and info_p["used"] == "plain"
-    lj, pj = info_l["learned"], info_l["plain"]
+    lj, pj = info_l["learned"], info_p["plain"]
     print(f"[learned] init residual {lj['dc_residual']:.4f} vs plain {pj['dc_residual']:.4f}; "
           f"NMSE {lj['nmse']:.5f} vs {pj['nmse']:.5f}", flush=True)
     assert lj["dc_residual"] < 0.75 * pj["dc_residual"]
@@ -83,6 +83,13 @@ def test_fallback_to_plain_when_no_or_bad_proposals(small):
     g, info = initialise(p, vox, bad, plain_points=20_000)
     assert info["used"] == "plain"
     assert info["learned"]["dc_residual"] > info["plain"]["dc_residual"]
+
+    # A residual above the acceptance bound is compared with the plain set; with the
+    # bound at 0 every learned set is compared, and the good one still wins.
+    g, info = initialise(p, vox, li, plain_points=20_000, accept_residual=0.0)
+    assert info["used"] == "learned" and info["learned"]["dc_residual"] < info["plain"]["dc_residual"]
+    g, info = initialise(p, vox, li, plain_points=20_000)
+    assert info["used"] == "learned" and "plain" not in info
 
 
 def test_inference_matches_unbatched_and_is_deterministic(small):
