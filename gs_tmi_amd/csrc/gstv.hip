@@ -1049,7 +1049,7 @@ __host__ __device__ constexpr int grad_zs(int t) { return t * (t + 2) + 4; }
 struct BwdStage {
   float c0[kBatch], c1[kBatch], c2[kBatch], rr[kBatch], ri[kBatch];
   float a00[kBatch], a11[kBatch], a22[kBatch], b01[kBatch], b02[kBatch], b12[kBatch];
-  float out[kAccStride][kBatch];  // per-pair moment totals for this tile
+  int32_t gid[kBatch];            // Gaussian id of each staged pair
   uint16_t box[kBatch];           // z0 (4 bits), rows-1 (4), H blocks lo, hi, W blocks lo, hi (2 each)
   uint16_t key[kBatch];           // (columns - 1) * 16 + rows - 1, or kNoKey
   uint16_t order[kBatch];         // batch indices sorted by key
@@ -1182,13 +1182,18 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
   // Records of the next batch are loaded into registers while the current
   // batch is evaluated (loads stay in flight across the LDS barriers).
   GRec nxt = {};
-  if (tid < min(kBatch, b.p_end - b.p_begin)) nxt = rec[pair_ids[b.p_begin + tid]];
+  int nxt_id = 0;
+  if (tid < min(kBatch, b.p_end - b.p_begin)) {
+    nxt_id = pair_ids[b.p_begin + tid];
+    nxt = rec[nxt_id];
+  }
   for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
     const int cnt = min(kBatch, b.p_end - bs);
     __syncthreads();
     // Stage: parameters, box and key of every pair of the batch.
     if (tid < cnt) {
       const GRec r = nxt;
+      sh.gid[tid] = nxt_id;
       sh.c0[tid] = r.c0;
       sh.c1[tid] = r.c1;
       sh.c2[tid] = r.c2;
@@ -1200,8 +1205,6 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
       sh.b01[tid] = r.b01;
       sh.b02[tid] = r.b02;
       sh.b12[tid] = r.b12;
-#pragma unroll
-      for (int f = 0; f < kAccStride; ++f) sh.out[f][tid] = 0.0f;
       const float lo0 = fmaxf(ceilf(r.c0 - r.e0), (float)b.o0);
       const float lo1 = fmaxf(ceilf(r.c1 - r.e1), (float)b.o1);
       const float lo2 = fmaxf(ceilf(r.c2 - r.e2), (float)b.o2);
@@ -1222,7 +1225,10 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
     }
     {
       const int bn = bs + kBatch;
-      if (bn < b.p_end && tid < min(kBatch, b.p_end - bn)) nxt = rec[pair_ids[bn + tid]];
+      if (bn < b.p_end && tid < min(kBatch, b.p_end - bn)) {
+        nxt_id = pair_ids[bn + tid];
+        nxt = rec[nxt_id];
+      }
     }
     __syncthreads();
     // Key offsets (ascending keys) and, per column count, the sorted segment and
@@ -1311,23 +1317,19 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
         for (int i = 0; i < 6; ++i) s6[i] = pair_sum_rows02(v[i], i + 6 < kMoments ? v[i + 6] : 0.0f);
 #pragma unroll
         for (int i = 0; i < 3; ++i) t3[i] = pair_sum_rows01(s6[i], s6[i + 3]);
+        // Each lane adds its 3 totals straight into the Gaussian's 48-byte row:
+        // one atomic instruction covers 16 rows with 4 lanes each, and the
+        // atomics drain while the wave evaluates its next group.
         if (slot_ok) {
+          float* row = gacc + (int64_t)sh.gid[k] * kAccStride + 3 * xi;
 #pragma unroll
-          for (int jj = 0; jj < 3; ++jj) sh.out[3 * xi + jj][k] = t3[jj];
+          for (int jj = 0; jj < 3; ++jj)
+            if (3 * xi + jj < kMoments && t3[jj] != 0.0f) unsafeAtomicAdd(row + jj, t3[jj]);
         }
       }
     }
-    __syncthreads();
-    // Flush: adjacent lanes add the moments of one pair, so each atomic
-    // instruction touches a few 48-byte rows instead of 64 scattered lines.
-    for (int idx = tid; idx < cnt * kAccStride; idx += kTileThreads) {
-      const int p = idx / kAccStride;
-      const int f = idx - p * kAccStride;
-      if (f < kMoments) {
-        const float v = sh.out[f][p];
-        if (v != 0.0f) unsafeAtomicAdd(gacc + (int64_t)pair_ids[bs + p] * kAccStride + f, v);
-      }
-    }
+    // key_cnt is only read before the group loop; the next batch's barrier
+    // orders this reset before its counts.
     for (int i = tid; i < kKeys; i += kTileThreads) sh.key_cnt[i] = 0;
   }
 }
