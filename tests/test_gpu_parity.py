@@ -145,12 +145,14 @@ def test_empty_and_degenerate_inputs():
 
 
 @pytest.mark.parametrize("shape,tile,frac", [((64, 64, 64), 8, 0.97), ((64, 64, 64), 8, 0.85),
-                                             ((20, 20, 20), 1, 0.85), ((8, 8, 8), 8, 0.85)])
+                                             ((20, 20, 20), 1, 0.85), ((8, 8, 8), 8, 0.85),
+                                             ((48, 48, 40), 1, 0.5)])
 def test_binning_with_many_pairless_gaussians(shape, tile, frac):
     """A fraction `frac` of the Gaussians has radius -3: their inverted boxes
     straddle a tile boundary and give no pairs, so a first-pass radix block's
     4096 slots span up to ~22k Gaussians (more than its LDS map holds: the
-    per-slot search path runs); the single-tile volume takes the no-radix path.
+    per-slot search path runs); the single-tile volume takes the no-radix path,
+    and 92,160 one-voxel tiles need three radix passes (a ping-pong pair).
     Bit-exact against the oracle."""
     from gs_tmi_amd import build_tile_pairs
 

@@ -38,7 +38,10 @@ def test_library_exports_every_declared_symbol():
     assert b"2^31" in lib.gstv_status_string(3)
     assert lib.gstv_forward_workspace_bytes(1000) >= 64 * 1000
     assert lib.gstv_backward_workspace_bytes(1000) >= 64 * 1000 + 48 * 1000
-    assert lib.gstv_bin_fill_workspace_bytes(10, 12345, 64) >= 4 * 4 * 12345
+    # first-pass keys and values (int32 per pair), digit histograms, per-block first Gaussians
+    assert lib.gstv_bin_fill_workspace_bytes(10, 12345, 64) >= 2 * 4 * 12345
+    # three or more radix passes (more than 2^16 tiles) ping-pong through a second pair
+    assert lib.gstv_bin_fill_workspace_bytes(10, 12345, 1 << 17) >= 4 * 4 * 12345
 
 
 MLP_HEADER = ROOT / "include" / "gstv_mlp.h"
