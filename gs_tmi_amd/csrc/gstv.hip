@@ -1308,23 +1308,25 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
             ++j1;
           }
         }
-        float v[kMoments] = {acc.dr, acc.di, acc.m0, acc.m1, acc.m2, acc.m00, acc.m11, acc.m22, acc.m01, acc.m02,
-                             acc.m12};
+        const float v[kAccStride] = {acc.dr,  acc.di,  acc.m0,  acc.m1,  acc.m2,  acc.m00,
+                                     acc.m11, acc.m22, acc.m01, acc.m02, acc.m12, 0.0f};
         // Reduce-scatter over the 4 W lanes of each Gaussian: afterwards the
-        // lane in row xi holds the totals of moments 3*xi .. 3*xi+2.
-        float s6[6], t3[3];
+        // lane in row xi holds the totals of moments 4*jj + xi, jj = 0..2.
+        float t3[3];
 #pragma unroll
-        for (int i = 0; i < 6; ++i) s6[i] = pair_sum_rows02(v[i], i + 6 < kMoments ? v[i + 6] : 0.0f);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) t3[i] = pair_sum_rows01(s6[i], s6[i + 3]);
+        for (int jj = 0; jj < 3; ++jj) {
+          const float lo = pair_sum_rows02(v[4 * jj], v[4 * jj + 2]);      // rows 0,1: m 4jj; rows 2,3: m 4jj+2
+          const float hi = pair_sum_rows02(v[4 * jj + 1], v[4 * jj + 3]);  // rows 0,1: m 4jj+1; rows 2,3: m 4jj+3
+          t3[jj] = pair_sum_rows01(lo, hi);
+        }
         // Each lane adds its 3 totals straight into the Gaussian's 48-byte row:
-        // one atomic instruction covers 16 rows with 4 lanes each, and the
-        // atomics drain while the wave evaluates its next group.
+        // one atomic instruction covers 16 rows with 4 consecutive floats each,
+        // and the atomics drain while the wave evaluates its next group.
         if (slot_ok) {
-          float* row = gacc + (int64_t)sh.gid[k] * kAccStride + 3 * xi;
+          float* row = gacc + (int64_t)sh.gid[k] * kAccStride + xi;
 #pragma unroll
           for (int jj = 0; jj < 3; ++jj)
-            if (3 * xi + jj < kMoments && t3[jj] != 0.0f) unsafeAtomicAdd(row + jj, t3[jj]);
+            if (4 * jj + xi < kMoments && t3[jj] != 0.0f) unsafeAtomicAdd(row + 4 * jj, t3[jj]);
         }
       }
     }
