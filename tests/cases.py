@@ -79,6 +79,21 @@ def edge_case(kind, seed=0):
     return dict(shape=shape, tile=tile, pos=pos, scales=scales, rot=rot, rr=rr, ri=ri)
 
 
+def random_case(seed):
+    """Randomised small case: volume sides 1-40 voxels, tile 1-16, 1-300
+    Gaussians with log-normal voxel sigma (median 0.3-4, log-sd 0.2-0.8), a
+    fifth of the centres pushed outside the volume."""
+    rng = np.random.default_rng(5000 + seed)
+    shape = tuple(int(v) for v in rng.integers(1, 41, size=3))
+    tile = int(rng.integers(1, 17))
+    n = int(rng.integers(1, 301))
+    pos, scales, rot, rr, ri = make_gaussians(n, shape, seed, sigma_median=float(rng.uniform(0.3, 4.0)),
+                                              sigma_log_sd=float(rng.uniform(0.2, 0.8)))
+    out = rng.random(n) < 0.2
+    pos[out] = (pos[out] * 1.5).astype(np.float32)
+    return dict(shape=shape, tile=tile, pos=pos, scales=scales, rot=rot, rr=rr, ri=ri)
+
+
 EDGE_KINDS = ["partial_tiles", "outside", "tile1", "tile3", "tile5", "tile12", "tile16", "needles", "large_sigma",
               "tiny_sigma", "anisotropic_volume", "single"]
 

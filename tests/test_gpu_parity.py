@@ -47,7 +47,8 @@ def _bin_gpu(center, radius, c):
 
 ALL_CASES = [("adjoint_voxel", lambda: cases.adjoint_check_case(0, "voxel")),
              ("adjoint_literal", lambda: cases.adjoint_check_case(1, "literal"))] + \
-            [(k, (lambda k=k: cases.edge_case(k))) for k in cases.EDGE_KINDS]
+            [(k, (lambda k=k: cases.edge_case(k))) for k in cases.EDGE_KINDS] + \
+            [(f"random{i}", (lambda i=i: cases.random_case(i))) for i in range(12)]
 
 
 @pytest.mark.parametrize("name,make", ALL_CASES, ids=[n for n, _ in ALL_CASES])
