@@ -5,28 +5,32 @@
 // wavefronts.  Everything sits behind the C ABI declared in include/gstv.h.
 //
 // Layout of the work
-//   * Binning: per-Gaussian tile counts -> device-wide exclusive scan ->
-//     expansion of (tile id, Gaussian id) pairs in Gaussian-major order -> stable
-//     LSD radix sort on the tile id (8-bit digits, wave-level multisplit
-//     ranking) -> per-tile [start, end) ranges.  Stability makes the order
-//     inside a tile ascending in Gaussian id.
+//   * Binning: per-Gaussian tile counts -> device-wide exclusive scan -> stable
+//     LSD radix sort on the tile id (balanced digits of at most 8 bits,
+//     wave-level multisplit ranking).  The first pass expands the (tile id,
+//     Gaussian id) pairs itself, in Gaussian-major order, from the Gaussians'
+//     tile boxes; the last pass writes only the Gaussian ids and merges the
+//     per-tile [start, end) ranges.  Stability makes the order inside a tile
+//     ascending in Gaussian id.
 //   * A per-Gaussian record (64 B: centre, density, quadratic form in voxel
 //     units pre-scaled by -log2(e)/2, and the exact axis-aligned extent of the
 //     q <= 9 ellipsoid) is built once per call, so the tile kernels gather one
 //     cache line per (tile, Gaussian) pair.
-//   * Tile kernels: one 512-thread block per 8^3 tile (one block per 8
-//     sub-bricks for larger tiles); each wave owns a 4x4x4 sub-brick.  A batch
-//     of up to 512 pairs is staged in LDS; every wave compacts the Gaussians
-//     whose exact extent touches its sub-brick (ballot) and evaluates them 16
-//     at a time: lane = (Gaussian slot 0..15, W offset 0..3), each lane walks
-//     the 4x4 (D, H) positions of its W column, so the quadratic form costs
-//     two FMAs per evaluation.  The forward keeps 16 complex partial sums per
-//     lane and reduces them across the 16 Gaussian slots once per tile.  The
-//     backward accumulates the 11 gradient moments of its Gaussian over the
-//     16 positions, reduces across the 4 W lanes, adds them into a per-batch
-//     LDS table, and flushes each (tile, Gaussian) total with one fp32 atomic
-//     per term.  A final per-Gaussian kernel maps moments to the reference's
-//     outputs (centre, cov_inv, density gradients).
+//   * Forward: one 512-thread block per 8^3 tile (one block per 8 sub-bricks
+//     for larger tiles); each wave owns a 4x4x4 sub-brick.  A batch of up to
+//     512 pairs is staged in LDS; every wave compacts the Gaussians whose exact
+//     extent touches its sub-brick (ballot) and evaluates them 16 at a time:
+//     lane = (Gaussian slot 0..15, W offset 0..3), each lane walks the 4x4
+//     (D, H) positions of its W column, so the quadratic form costs two FMAs
+//     per evaluation; 16 complex partial sums per lane are reduced across the
+//     16 Gaussian slots once per tile and every voxel is written once.
+//   * Backward: one block per tile; each staged Gaussian walks its own box
+//     (4x4 (H, W) columns, exact D rows) with lane = (Gaussian slot, W
+//     offset), in groups of 16 Gaussians with equal column counts, keeping its
+//     11 gradient moments in registers; a lane-swap reduce-scatter over the 4
+//     W lanes leaves 3 totals per lane, added with fp32 atomics into the
+//     Gaussian's moment row.  A final per-Gaussian kernel maps moments to the
+//     reference's outputs (centre, cov_inv, density gradients).
 
 #include <hip/hip_runtime.h>
 
