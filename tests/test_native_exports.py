@@ -68,3 +68,13 @@ def test_mlp_library_exports_every_declared_symbol():
     assert lib.gstv_mlp_gemm_nt_bf16(None, 4, None, 4, None, 0, 2, 2, 4, None, 2, None, 2, None) == 1
     assert lib.gstv_mlp_gemm_nt_bf16(None, 2, None, 4, None, 0, 2, 2, 4, None, 2, None, 2, None) == 1
     assert lib.gstv_mlp_gemm_nt_bf16(None, 4, None, 4, None, 0, 0, 2, 4, 1, 2, None, 2, None) == 0
+
+
+@pytest.mark.skipif(not _native.LIB_PATH.exists(), reason="library not built (run `python -m gs_tmi_amd.build`)")
+def test_pair_count_limit_is_refused_before_any_allocation():
+    """2^31 - 1 or more (tile, Gaussian) pairs cannot be indexed by the int32 pair ids:
+    bin_fill raises the library's TOO_MANY_PAIRS message before allocating anything."""
+    import torch
+
+    with pytest.raises(RuntimeError, match="2\\^31"):
+        _native.bin_fill(torch.zeros(1, 3), torch.zeros(1), (8, 8, 8), 8, torch.zeros(2, dtype=torch.int32), 2**31)
