@@ -16,21 +16,29 @@
 //     units pre-scaled by -log2(e)/2, and the exact axis-aligned extent of the
 //     q <= 9 ellipsoid) is built once per call, so the tile kernels gather one
 //     cache line per (tile, Gaussian) pair.
+//   * Column bound: for a 4x4 (H, W) voxel column the D rows that can hold a
+//     voxel of the q <= 9 ellipsoid lie inside the intersection of the D
+//     extents of its two projections on (D, H) and (D, W), each cut by the
+//     column's strip -- a closed form per strip (clamp, square root).  It
+//     culls sub-bricks in the forward and rows in the backward; about half of
+//     the box-level work has no voxel inside the ellipsoid.
 //   * Forward: one 512-thread block per 8^3 tile (one block per 8 sub-bricks
 //     for larger tiles); each wave owns a 4x4x4 sub-brick.  A batch of up to
-//     512 pairs is staged in LDS; every wave compacts the Gaussians whose exact
-//     extent touches its sub-brick (ballot) and evaluates them 16 at a time:
-//     lane = (Gaussian slot 0..15, W offset 0..3), each lane walks the 4x4
-//     (D, H) positions of its W column, so the quadratic form costs two FMAs
-//     per evaluation; 16 complex partial sums per lane are reduced across the
-//     16 Gaussian slots once per tile and every voxel is written once.
-//   * Backward: one block per tile; each staged Gaussian walks its own box
-//     (4x4 (H, W) columns, exact D rows) with lane = (Gaussian slot, W
-//     offset), in groups of 16 Gaussians with equal column counts, keeping its
-//     11 gradient moments in registers; a lane-swap reduce-scatter over the 4
-//     W lanes leaves 3 totals per lane, added with fp32 atomics into the
-//     Gaussian's moment row.  A final per-Gaussian kernel maps moments to the
-//     reference's outputs (centre, cov_inv, density gradients).
+//     512 pairs is staged in LDS with the mask of sub-bricks the column bound
+//     leaves; every wave compacts its Gaussians (ballot) and evaluates them 16
+//     at a time: lane = (Gaussian slot 0..15, W offset 0..3), each lane walks
+//     the 4x4 (D, H) positions of its W column, so the quadratic form costs
+//     two FMAs per evaluation; 16 complex partial sums per lane are reduced
+//     across the 16 Gaussian slots once per tile and every voxel is written
+//     once.
+//   * Backward: one block per tile; each staged Gaussian walks its live
+//     columns (4x4 (H, W) blocks of its box, each with the D rows the column
+//     bound leaves) with lane = (Gaussian slot, W offset), in groups of 16
+//     Gaussians with equal column counts, keeping its 11 gradient moments in
+//     registers; a lane-swap reduce-scatter over the 4 W lanes leaves 3
+//     totals per lane, added with fp32 atomics into the Gaussian's moment row.
+//     A final per-Gaussian kernel maps moments to the reference's outputs
+//     (centre, cov_inv, density gradients).
 
 #include <hip/hip_runtime.h>
 
@@ -742,6 +750,80 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ center, 
   rec[g] = r;
 }
 
+// ------------------------------------------------ D extent over a column --
+//
+// The D rows of a voxel column (H in [y0, y1], W in [x0, x1]) that can hold a
+// voxel of the widened ellipsoid E = {u : u^T A u <= R^2} (R = 3.06, the
+// radius of the extents e_a = R sqrt(C_aa) + 0.01, C = A^-1) lie in the
+// intersection of two intervals: the D extent of E cut by the H strip alone
+// and by the W strip alone.  Cutting by the strip u_i in [lo, hi] is the same
+// as cutting E's projection on (u_0, u_i), an ellipse with matrix C_sub^-1;
+// along u_i = t that ellipse spans
+//     u_0 in al t -/+ ga sqrt(rho^2 - t^2),
+//     al = C_0i / C_ii,  ga = sqrt(C_00 C_ii - C_0i^2) / C_ii,  rho = R sqrt(C_ii),
+// a concave upper and a convex lower boundary whose extremes lie at
+// t = +/- ta, ta = al rho^2 / rho_0.  Over [lo, hi] the maximum of u_0 is
+// therefore at clamp(ta, lo, hi) and the minimum at clamp(-ta, lo, hi).
+// With cofactors of the scaled form A' = kq A (C_ii = cof_ii / det and, by
+// Jacobi's identity, C_00 C_ii - C_0i^2 = A_jj / det A):
+//     al = cof_0i / cof_ii,  ga = sqrt(A'_jj det A') / cof_ii.
+// The determinant cancels badly for needle-shaped Gaussians, so cofactors and
+// determinant are formed in fp64 from the fp32 form the kernels evaluate.
+struct ZAxis {
+  float al, ga, ta, rho;
+};
+struct ZCoefs {
+  ZAxis h, w;  // projections on (D, H) and (D, W)
+  bool ok;     // false: no bound (extents infinite, or the form is not definite)
+};
+
+__device__ __forceinline__ ZCoefs z_coefs(float a00, float a11, float a22, float b01, float b02, float b12, float e0,
+                                          float e1, float e2) {
+  const double s00 = a00, s11 = a11, s22 = a22;
+  const double s01 = 0.5 * (double)b01, s02 = 0.5 * (double)b02, s12 = 0.5 * (double)b12;
+  const double cof00 = s11 * s22 - s12 * s12;
+  const double cof11 = s00 * s22 - s02 * s02;
+  const double cof22 = s00 * s11 - s01 * s01;
+  const double cof01 = s02 * s12 - s01 * s22;
+  const double cof02 = s01 * s12 - s02 * s11;
+  const double det = s00 * cof00 + s01 * cof01 + s02 * cof02;
+  const float dt = (float)det;
+  const float i11 = __builtin_amdgcn_rcpf((float)cof11), i22 = __builtin_amdgcn_rcpf((float)cof22);
+  const float r0 = e0 - 0.01f, r1 = e1 - 0.01f, r2 = e2 - 0.01f;
+  ZCoefs z;
+  z.h.al = (float)cof01 * i11;
+  z.h.ga = __builtin_amdgcn_sqrtf(a22 * dt) * i11;
+  z.h.rho = r1;
+  z.h.ta = z.h.al * (r1 * r1 * __builtin_amdgcn_rcpf(r0));
+  z.w.al = (float)cof02 * i22;
+  z.w.ga = __builtin_amdgcn_sqrtf(a11 * dt) * i22;
+  z.w.rho = r2;
+  z.w.ta = z.w.al * (r2 * r2 * __builtin_amdgcn_rcpf(r0));
+  z.ok = cof11 > 0.0 && cof22 > 0.0 && det < 0.0 && r0 > 0.0f && r1 > 0.0f && r2 > 0.0f && isfinite(r0) &&
+         isfinite(r1) && isfinite(r2) && isfinite(z.h.al) && isfinite(z.h.ga) && isfinite(z.h.ta) &&
+         isfinite(z.w.al) && isfinite(z.w.ga) && isfinite(z.w.ta);
+  return z;
+}
+
+// D extent [zmin, zmax] (relative to the centre) of the projected ellipse over
+// the strip u_i in [lo, hi].
+__device__ __forceinline__ void z_span(const ZAxis& a, float lo, float hi, float& zmin, float& zmax) {
+  const float tp = fminf(fmaxf(a.ta, lo), hi), tm = fminf(fmaxf(-a.ta, lo), hi);
+  const float rr = a.rho * a.rho;
+  const float sp = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-tp, tp, rr), 0.0f));
+  const float sm = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-tm, tm, rr), 0.0f));
+  zmax = fmaf(a.ga, sp, a.al * tp);
+  zmin = fmaf(-a.ga, sm, a.al * tm);
+}
+
+// Absolute D rows [zl, zh] of a column from the strip extents, clipped to the
+// Gaussian's box rows [lo0, hi0]; a NaN bound falls back to the box.
+__device__ __forceinline__ void column_rows(float c0, float hmin, float hmax, float wmin, float wmax, float lo0,
+                                            float hi0, float& zl, float& zh) {
+  zl = fmaxf(ceilf(c0 + fmaxf(hmin, wmin) - 0.01f), lo0);
+  zh = fminf(floorf(c0 + fminf(hmax, wmax) + 0.01f), hi0);
+}
+
 // ------------------------------------------------------------ tile kernels --
 
 struct VoxGeom {
@@ -808,7 +890,7 @@ __device__ __forceinline__ BlockCtx make_ctx(int logical, const VoxGeom& geo, co
 // Stages one pair: copies the record fields and computes the Gaussian's
 // exact extent clipped to the tile, in tile-local voxel indices, packed as
 // 6 x 5 bits (lo0, lo1, lo2, hi0, hi1, hi2); kEmptyBox if it misses the tile.
-__device__ __forceinline__ void stage_pair(Stage& sh, int k, const GRec& r, const BlockCtx& b) {
+__device__ __forceinline__ void stage_pair(Stage& sh, int k, const GRec& r, const BlockCtx& b, bool use_mask) {
   sh.c0[k] = r.c0;
   sh.c1[k] = r.c1;
   sh.c2[k] = r.c2;
@@ -835,13 +917,41 @@ __device__ __forceinline__ void stage_pair(Stage& sh, int k, const GRec& r, cons
     const uint32_t h0 = (uint32_t)((int)hi0 - b.o0), h1 = (uint32_t)((int)hi1 - b.o1),
                    h2 = (uint32_t)((int)hi2 - b.o2);
     box = l0 | (l1 << 5) | (l2 << 10) | (h0 << 15) | (h1 << 20) | (h2 << 25);
-    // Sub-bricks touched along each axis (bits j = 0, 1), combined into the
-    // 2x2x2 mask; only read when the tile has at most 2 sub-bricks per axis.
-    const uint32_t m0 = ((l0 >> 2) == 0 ? 1u : 0u) | ((h0 >> 2) == 1 ? 2u : 0u);
-    const uint32_t m1 = ((l1 >> 2) == 0 ? 1u : 0u) | ((h1 >> 2) == 1 ? 2u : 0u);
-    const uint32_t m2 = ((l2 >> 2) == 0 ? 1u : 0u) | ((h2 >> 2) == 1 ? 2u : 0u);
-    const uint32_t plane = ((m1 & 1u) ? m2 : 0u) | ((m1 & 2u) ? (m2 << 2) : 0u);
-    sh.sbm[k] = (uint8_t)(((m0 & 1u) ? plane : 0u) | ((m0 & 2u) ? (plane << 4) : 0u));
+    // 2x2x2 mask of the sub-bricks that can hold a voxel of the ellipsoid
+    // (only read when the tile has at most 2 sub-bricks per axis): the box's
+    // (H, W) column blocks, each with the D rows its extent bound leaves.
+    if (use_mask) {
+      const ZCoefs zc = z_coefs(r.a00, r.a11, r.a22, r.b01, r.b02, r.b12, r.e0, r.e1, r.e2);
+      float hmn[2], hmx[2], wmn[2], wmx[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        hmn[j] = wmn[j] = -INFINITY;
+        hmx[j] = wmx[j] = INFINITY;
+        if (zc.ok) {
+          const int y0 = max(4 * j, (int)l1), y1 = min(4 * j + 3, (int)h1);
+          const int x0 = max(4 * j, (int)l2), x1 = min(4 * j + 3, (int)h2);
+          z_span(zc.h, (float)(b.o1 + y0) - r.c1, (float)(b.o1 + y1) - r.c1, hmn[j], hmx[j]);
+          z_span(zc.w, (float)(b.o2 + x0) - r.c2, (float)(b.o2 + x1) - r.c2, wmn[j], wmx[j]);
+        }
+      }
+      uint32_t mask = 0;
+#pragma unroll
+      for (int j1 = 0; j1 < 2; ++j1) {
+#pragma unroll
+        for (int j2 = 0; j2 < 2; ++j2) {
+          if ((int)(l1 >> 2) <= j1 && j1 <= (int)(h1 >> 2) && (int)(l2 >> 2) <= j2 && j2 <= (int)(h2 >> 2)) {
+            float zl, zh;
+            column_rows(r.c0, hmn[j1], hmx[j1], wmn[j2], wmx[j2], lo0, hi0, zl, zh);
+            if (zl <= zh) {
+              const int za = (int)zl - b.o0, zb = (int)zh - b.o0;  // tile-local, inside [l0, h0]
+              if (za <= 3) mask |= 1u << (2 * j1 + j2);
+              if (zb >= 4) mask |= 1u << (4 + 2 * j1 + j2);
+            }
+          }
+        }
+      }
+      sh.sbm[k] = (uint8_t)mask;
+    }
   } else {
     sh.sbm[k] = 0;
   }
@@ -970,7 +1080,7 @@ __global__ __launch_bounds__(kTileThreads) void k_forward(const GRec* __restrict
   for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
     const int cnt = min(kBatch, b.p_end - bs);
     __syncthreads();
-    if ((int)threadIdx.x < cnt) stage_pair(sh, threadIdx.x, nxt, b);
+    if ((int)threadIdx.x < cnt) stage_pair(sh, threadIdx.x, nxt, b, geo.nsb <= 2);
     const int bn = bs + kBatch;
     if (bn < b.p_end && (int)threadIdx.x < min(kBatch, b.p_end - bn)) nxt = rec[pair_ids[bn + threadIdx.x]];
     __syncthreads();
@@ -1029,19 +1139,29 @@ constexpr int kAccStride = 12;
 // ---- backward ----
 //
 // The backward gives every Gaussian of a batch to one set of lanes for the
-// whole tile, so its 11 moments accumulate in registers and no LDS or global
-// atomics are needed inside the tile.  A lane = (Gaussian slot, W offset)
-// walks the columns of its Gaussian's box -- 4 x 4 (H, W) voxel blocks -- and
-// in each column exactly the D rows of the box (no rounding of D to blocks).
-// Gaussians are keyed by (columns, D rows) and counting-sorted; a group is 16
-// Gaussians with the same column count and neighbouring row counts, so a group
-// costs columns x max(rows) x 64 evaluations.  Groups are dealt to the 8 waves
-// in descending cost order for balance.
+// whole tile, so its 11 moments accumulate in registers and no LDS atomics are
+// needed inside the tile.  A lane = (Gaussian slot, W offset) walks the live
+// columns of its Gaussian -- the 4 x 4 (H, W) voxel blocks of its box that the
+// column bound above leaves non-empty -- and in each column exactly the D rows
+// the bound leaves.  A Gaussian's columns are kept in (H, W) block order (the
+// order in which the 16 lanes' upstream-gradient reads spread best over the
+// LDS banks); Gaussians are keyed by (live columns, largest row count) and
+// counting-sorted, a group is 16 Gaussians with the same column count and
+// neighbouring row counts, and the group runs its j-th columns for the largest
+// row count among them.  Groups are dealt to the 8 waves in descending cost
+// order for balance.
 
-constexpr int kMaxCols = 16;           // 4 x 4 column blocks for tile <= 16
-constexpr int kMaxRows = 16;           // D rows for tile <= 16
+constexpr int kMaxBlk = 4;                  // column blocks per axis, tile <= 16
+constexpr int kMaxCols = kMaxBlk * kMaxBlk;  // columns per tile
+constexpr int kMaxRows = 16;                // D rows for tile <= 16
 constexpr int kKeys = kMaxCols * kMaxRows;
 constexpr uint16_t kNoKey = 0xffff;
+
+// Column word: H block (bits 0-1), W block (2-3), first row (4-7), rows - 1 (8-11).
+__device__ __forceinline__ uint16_t pack_col(int j1, int j2, int z0, int rows) {
+  return (uint16_t)(j1 | (j2 << 2) | (z0 << 4) | ((rows - 1) << 8));
+}
+__device__ __forceinline__ int col_rows(uint16_t c) { return ((c >> 8) & 15) + 1; }
 
 // Strides (float2 elements) of the tile's upstream-gradient copy in LDS.  With
 // rows padded by 2 and slabs by 4 elements, the 4 x 4 column blocks of an 8^3
@@ -1050,12 +1170,14 @@ constexpr uint16_t kNoKey = 0xffff;
 __host__ __device__ constexpr int grad_ys(int t) { return t + 2; }
 __host__ __device__ constexpr int grad_zs(int t) { return t * (t + 2) + 4; }
 
+// MC: live columns a pair can have (column blocks per tile).
+template <int MC>
 struct BwdStage {
   float c0[kBatch], c1[kBatch], c2[kBatch], rr[kBatch], ri[kBatch];
   float a00[kBatch], a11[kBatch], a22[kBatch], b01[kBatch], b02[kBatch], b12[kBatch];
   int32_t gid[kBatch];            // Gaussian id of each staged pair
-  uint16_t box[kBatch];           // z0 (4 bits), rows-1 (4), H blocks lo, hi, W blocks lo, hi (2 each)
-  uint16_t key[kBatch];           // (columns - 1) * 16 + rows - 1, or kNoKey
+  uint16_t cols[kBatch][MC];      // live columns in (H, W) block order
+  uint16_t key[kBatch];           // (columns - 1) * 16 + largest row count - 1, or kNoKey
   uint16_t order[kBatch];         // batch indices sorted by key
   uint16_t slot[kBatch];          // rank inside the key
   int key_cnt[kKeys];
@@ -1156,7 +1278,9 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
                                                            const float* __restrict__ grad_r,
                                                            const float* __restrict__ grad_i,
                                                            float* __restrict__ gacc) {
-  __shared__ BwdStage sh;
+  constexpr int NB = TT > 0 ? (TT + kSub - 1) / kSub : kMaxBlk;  // column blocks per axis (bound)
+  constexpr int MC = NB * NB;
+  __shared__ BwdStage<MC> sh;
   // Upstream gradient of the tile, TP^3 with TP the tile edge rounded up to a
   // multiple of 4 (strides grad_zs, grad_ys, 1), zero outside the tile's valid
   // voxels.
@@ -1167,6 +1291,7 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
   const int lane = tid & 63, wave = tid >> 6;
   const int gl = lane & 15, xi = lane >> 4;
   const int T = TT > 0 ? TT : geo.nsb * kSub;
+  const int nb = TT > 0 ? NB : geo.nsb;  // column blocks per axis
   // One block per tile: the context's wave-specific fields are unused here.
   const BlockCtx b = make_ctx(logical, geo, ranges, 0);
   if (!b.valid) return;
@@ -1194,7 +1319,7 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
   for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
     const int cnt = min(kBatch, b.p_end - bs);
     __syncthreads();
-    // Stage: parameters, box and key of every pair of the batch.
+    // Stage: parameters, live columns and key of every pair of the batch.
     if (tid < cnt) {
       const GRec r = nxt;
       sh.gid[tid] = nxt_id;
@@ -1217,13 +1342,47 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
       const float hi2 = fminf(floorf(r.c2 + r.e2), (float)(b.o2 + b.e2 - 1));
       uint16_t key = kNoKey;
       if (lo0 <= hi0 && lo1 <= hi1 && lo2 <= hi2) {
-        const int z0 = (int)lo0 - b.o0, rows = (int)hi0 - (int)lo0 + 1;
-        const int l1 = ((int)lo1 - b.o1) >> 2, l2 = ((int)lo2 - b.o2) >> 2;
-        const int h1 = ((int)hi1 - b.o1) >> 2, h2 = ((int)hi2 - b.o2) >> 2;
-        const int cols = (h1 - l1 + 1) * (h2 - l2 + 1);
-        sh.box[tid] = (uint16_t)(z0 | ((rows - 1) << 4) | (l1 << 8) | (h1 << 10) | (l2 << 12) | (h2 << 14));
-        key = (uint16_t)((cols - 1) * kMaxRows + rows - 1);
-        sh.slot[tid] = (uint16_t)atomicAdd(&sh.key_cnt[key], 1);
+        const int y_lo = (int)lo1 - b.o1, y_hi = (int)hi1 - b.o1;  // tile-local box
+        const int x_lo = (int)lo2 - b.o2, x_hi = (int)hi2 - b.o2;
+        const int l1 = y_lo >> 2, h1 = y_hi >> 2, l2 = x_lo >> 2, h2 = x_hi >> 2;
+        const ZCoefs zc = z_coefs(r.a00, r.a11, r.a22, r.b01, r.b02, r.b12, r.e0, r.e1, r.e2);
+        float hmn[NB], hmx[NB], wmn[NB], wmx[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+          hmn[j] = wmn[j] = -INFINITY;
+          hmx[j] = wmx[j] = INFINITY;
+          if (zc.ok) {
+            const int y0 = max(4 * j, y_lo), y1 = min(4 * j + 3, y_hi);
+            const int x0 = max(4 * j, x_lo), x1 = min(4 * j + 3, x_hi);
+            z_span(zc.h, (float)(b.o1 + y0) - r.c1, (float)(b.o1 + y1) - r.c1, hmn[j], hmx[j]);
+            z_span(zc.w, (float)(b.o2 + x0) - r.c2, (float)(b.o2 + x1) - r.c2, wmn[j], wmx[j]);
+          }
+        }
+        // Live columns in (H, W) block order, and their largest row count.
+        uint16_t* cl = sh.cols[tid];
+        int ncol = 0;
+        int mxr = 0;
+#pragma unroll
+        for (int j1 = 0; j1 < NB; ++j1) {
+#pragma unroll
+          for (int j2 = 0; j2 < NB; ++j2) {
+            if (j1 < nb && j2 < nb && l1 <= j1 && j1 <= h1 && l2 <= j2 && j2 <= h2) {
+              float zl, zh;
+              column_rows(r.c0, hmn[j1], hmx[j1], wmn[j2], wmx[j2], lo0, hi0, zl, zh);
+              if (zl <= zh) {
+                const int rows = (int)zh - (int)zl + 1;
+                const uint16_t c = pack_col(j1, j2, (int)zl - b.o0, rows);
+                cl[ncol] = c;
+                ++ncol;
+                mxr = max(mxr, rows);
+              }
+            }
+          }
+        }
+        if (ncol > 0) {
+          key = (uint16_t)((ncol - 1) * kMaxRows + mxr - 1);
+          sh.slot[tid] = (uint16_t)atomicAdd(&sh.key_cnt[key], 1);
+        }
       }
       sh.key[tid] = key;
     }
@@ -1277,7 +1436,7 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
 
     // Groups are dealt round-robin to the waves: global group g goes to wave g % 8.
 #pragma unroll 1
-    for (int cc = kMaxCols; cc >= 1; --cc) {
+    for (int cc = MC; cc >= 1; --cc) {
       const int seg_begin = sh.seg_begin[cc - 1];
       const int seg_cnt = sh.seg_cnt[cc - 1];
       const int first = sh.grp_first[cc - 1];
@@ -1288,29 +1447,24 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
         const int pos = top - gl;
         const bool slot_ok = pos >= seg_begin;
         const int k = sh.order[slot_ok ? pos : top];
-        const int rows = ((sh.box[sh.order[top]] >> 4) & 15) + 1;  // the group's largest
         const float c0 = sh.c0[k], c1 = sh.c1[k], c2 = sh.c2[k], rr = sh.rr[k], ri = sh.ri[k];
         const float a00 = sh.a00[k], a11 = sh.a11[k], a22 = sh.a22[k];
         const float b01 = sh.b01[k], b02 = sh.b02[k], b12 = sh.b12[k];
-        const int bx = sh.box[k];
-        const int z0 = bx & 15, own_rows = ((bx >> 4) & 15) + 1;
-        const int l1 = (bx >> 8) & 3, h1 = (bx >> 10) & 3, l2 = (bx >> 12) & 3, h2 = (bx >> 14) & 3;
-        const int d2 = h2 - l2 + 1;
-        (void)h1;
         BwdAcc acc = {};
-        int j1 = 0, j2 = 0;
         for (int j = 0; j < cc; ++j) {
-          const int s1 = (l1 + j1) * kSub, s2 = (l2 + j2) * kSub;
+          const uint16_t cw = sh.cols[k][j];
+          const int own_rows = col_rows(cw);
+          // The group's largest row count for its j-th columns (over the 16 slots).
+          int rows = own_rows;
+#pragma unroll
+          for (int off = 1; off < 16; off <<= 1) rows = max(rows, __shfl_xor(rows, off, 16));
+          const int j1 = cw & 3, j2 = (cw >> 2) & 3, z0 = (cw >> 4) & 15;
           if (partial)
-            bwd_column<TT, true>(sh_g, T, b, z0, rows, own_rows, s1, s2, xi, slot_ok, c0, c1, c2, rr, ri, a00, a11,
-                                 a22, b01, b02, b12, acc);
+            bwd_column<TT, true>(sh_g, T, b, z0, rows, own_rows, j1 * kSub, j2 * kSub, xi, slot_ok, c0, c1, c2, rr,
+                                 ri, a00, a11, a22, b01, b02, b12, acc);
           else
-            bwd_column<TT, false>(sh_g, T, b, z0, rows, own_rows, s1, s2, xi, slot_ok, c0, c1, c2, rr, ri, a00, a11,
-                                  a22, b01, b02, b12, acc);
-          if (++j2 == d2) {
-            j2 = 0;
-            ++j1;
-          }
+            bwd_column<TT, false>(sh_g, T, b, z0, rows, own_rows, j1 * kSub, j2 * kSub, xi, slot_ok, c0, c1, c2,
+                                  rr, ri, a00, a11, a22, b01, b02, b12, acc);
         }
         const float v[kAccStride] = {acc.dr,  acc.di,  acc.m0,  acc.m1,  acc.m2,  acc.m00,
                                      acc.m11, acc.m22, acc.m01, acc.m02, acc.m12, 0.0f};
@@ -1502,7 +1656,7 @@ __global__ __launch_bounds__(kTileThreads) void k_count(const GRec* __restrict__
   for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
     const int cnt = min(kBatch, b.p_end - bs);
     __syncthreads();
-    if ((int)threadIdx.x < cnt) stage_pair(sh, threadIdx.x, rec[pair_ids[bs + threadIdx.x]], b);
+    if ((int)threadIdx.x < cnt) stage_pair(sh, threadIdx.x, rec[pair_ids[bs + threadIdx.x]], b, geo.nsb <= 2);
     __syncthreads();
     if (!b.active) continue;
     const int nhit = build_list(sh, cnt, b, wave, lane, geo.nsb <= 2);

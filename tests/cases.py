@@ -71,6 +71,15 @@ def edge_case(kind, seed=0):
     elif kind == "anisotropic_volume":
         shape, n, tile = (64, 32, 16), 800, 8
         pos, scales, rot, rr, ri = make_gaussians(n, shape, seed, sigma_median=1.5, sigma_log_sd=0.6)
+    elif kind == "extreme_aniso":
+        # Needles and pancakes with axis ratios up to several hundred: the
+        # quadratic form's determinant cancels strongly, which the column
+        # bound of the tile kernels must survive.
+        shape, n, tile = (32, 32, 32), 200, 8
+        pos, scales, rot, rr, ri = make_gaussians(n, shape, seed, sigma_median=1.5, sigma_log_sd=0.3)
+        hs = np.asarray(shape, np.float64) / 2
+        s_vox = np.minimum(np.exp(rng.uniform(np.log(0.05), np.log(30.0), size=(n, 3))), 30.0)
+        scales = (s_vox / hs).astype(np.float32)
     elif kind == "single":
         shape, n, tile = (16, 16, 16), 1, 8
         pos, scales, rot, rr, ri = make_gaussians(n, shape, seed, sigma_median=2.0, sigma_log_sd=0.2)
@@ -94,8 +103,8 @@ def random_case(seed):
     return dict(shape=shape, tile=tile, pos=pos, scales=scales, rot=rot, rr=rr, ri=ri)
 
 
-EDGE_KINDS = ["partial_tiles", "outside", "tile1", "tile3", "tile5", "tile12", "tile16", "needles", "large_sigma",
-              "tiny_sigma", "anisotropic_volume", "single"]
+EDGE_KINDS = ["partial_tiles", "outside", "tile1", "tile3", "tile5", "tile12", "tile16", "needles", "extreme_aniso",
+              "large_sigma", "tiny_sigma", "anisotropic_volume", "single"]
 
 
 def rel_l2(a, b, mask=None):
