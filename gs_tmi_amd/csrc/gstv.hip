@@ -984,6 +984,29 @@ __device__ __forceinline__ int build_list(Stage& sh, int cnt, const BlockCtx& b,
   return nhit;
 }
 
+// Reductions over the 16 lanes of each row of a wave with DPP lane moves (no
+// LDS traffic): quad_perm [1,0,3,2] and [2,3,0,1] combine the lanes of each
+// quad, then row_ror:4 and row_ror:8 combine the quads; every lane of the row
+// ends with the result.  All 64 lanes must be active.
+template <int kCtrl>
+__device__ __forceinline__ int dpp_mov(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float row16_sum(float v) {
+  v += __int_as_float(dpp_mov<0xB1>(__float_as_int(v)));
+  v += __int_as_float(dpp_mov<0x4E>(__float_as_int(v)));
+  v += __int_as_float(dpp_mov<0x124>(__float_as_int(v)));
+  v += __int_as_float(dpp_mov<0x128>(__float_as_int(v)));
+  return v;
+}
+__device__ __forceinline__ int row16_max(int v) {
+  v = max(v, dpp_mov<0xB1>(v));
+  v = max(v, dpp_mov<0x4E>(v));
+  v = max(v, dpp_mov<0x124>(v));
+  v = max(v, dpp_mov<0x128>(v));
+  return v;
+}
+
 struct GaussLane {
   float c0, c1, rr, ri, a00, a11, b01;
   float k0, k1, k2;  // u2-dependent terms of the form
@@ -1095,11 +1118,8 @@ __global__ __launch_bounds__(kTileThreads) void k_forward(const GRec* __restrict
   // Sum the 16 Gaussian slots of each W column (a row of 16 lanes).
 #pragma unroll
   for (int i = 0; i < kSub * kSub; ++i) {
-#pragma unroll
-    for (int off = 1; off < 16; off <<= 1) {
-      acc_r[i] += __shfl_xor(acc_r[i], off, 16);
-      acc_i[i] += __shfl_xor(acc_i[i], off, 16);
-    }
+    acc_r[i] = row16_sum(acc_r[i]);
+    acc_i[i] = row16_sum(acc_i[i]);
   }
   // Lane gl of row xi writes position (z, y) = (gl / 4, gl % 4) of column xi.
   float out_r = acc_r[0], out_i = acc_i[0];
@@ -1194,8 +1214,14 @@ struct BwdAcc {
 // One column (4 x 4 block of H, W) of one lane's Gaussian: `rows` D rows from
 // tile-local z0 (rows beyond the lane's own `own_rows` contribute nothing).
 // TT is the tile edge when known at compile time (0: runtime).
+// LDS-qualified view of the tile's upstream-gradient copy (32-bit addressing),
+// as interleaved (real, imag) floats.
+typedef __attribute__((address_space(3))) const float LdsFloat;
+
+__device__ __forceinline__ float2 lds_g(LdsFloat* __restrict__ g, int i) { return make_float2(g[2 * i], g[2 * i + 1]); }
+
 template <int TT, bool kPartial>
-__device__ __forceinline__ void bwd_column(const float2* __restrict__ sh_g, int tp, const BlockCtx& b, int z0,
+__device__ __forceinline__ void bwd_column(LdsFloat* __restrict__ sh_g, int tp, const BlockCtx& b, int z0,
                                            int rows, int own_rows, int s1, int s2, int xi, bool g_ok, float c0,
                                            float c1, float c2, float rr, float ri, float a00, float a11, float a22,
                                            float b01, float b02, float b12, BwdAcc& acc) {
@@ -1203,7 +1229,7 @@ __device__ __forceinline__ void bwd_column(const float2* __restrict__ sh_g, int 
   const int YS = grad_ys(T), ZS = grad_zs(T);
   const float u2 = (float)(b.o2 + s2 + xi) - c2;
   const float k0 = a22 * u2 * u2, k1 = b02 * u2, k2 = b12 * u2;
-  const float2* gp = sh_g + s1 * YS + s2 + xi;
+  const int gbase = s1 * YS + s2 + xi;
   bool ok_lane = g_ok;
   if (kPartial) ok_lane = ok_lane && (s2 + xi < b.e2);
   float m = 0.0f, m0 = 0.0f, m1 = 0.0f;
@@ -1212,25 +1238,26 @@ __device__ __forceinline__ void bwd_column(const float2* __restrict__ sh_g, int 
   for (int y = 0; y < kSub; ++y) u1v[y] = (float)(b.o1 + s1 + y) - c1;
   const float u10 = u1v[0];
   // The next row's upstream gradient is loaded from LDS while the current row
-  // is evaluated.
+  // is evaluated; rows past the lane's own re-read its last row.
+  const int zlast_off = (z0 + own_rows - 1) * ZS;
+  int zn_off = (z0 + 1) * ZS;
   float2 gn[kSub];
 #pragma unroll
-  for (int y = 0; y < kSub; ++y) gn[y] = gp[z0 * ZS + y * YS];
+  for (int y = 0; y < kSub; ++y) gn[y] = lds_g(sh_g, gbase + z0 * ZS + y * YS);
+  float u0 = (float)(b.o0 + z0) - c0;
 #pragma unroll 2
   for (int r = 0; r < rows; ++r) {
-    const bool row_ok = r < own_rows;
-    const int z = row_ok ? z0 + r : z0;  // rows past the lane's own stay in the tile
-    const float u0 = (float)(b.o0 + z) - c0;
     float A = fmaf(fmaf(a00, u0, k1), u0, k0);
-    A = row_ok ? A : -INFINITY;
+    A = r < own_rows ? A : -INFINITY;
     const float B = fmaf(b01, u0, k2);
     float2 gc[kSub];
 #pragma unroll
     for (int y = 0; y < kSub; ++y) gc[y] = gn[y];
     {
-      const int zn = (r + 1 < own_rows) ? z0 + r + 1 : z0;
+      const int nb = gbase + min(zn_off, zlast_off);
+      zn_off += ZS;
 #pragma unroll
-      for (int y = 0; y < kSub; ++y) gn[y] = gp[zn * ZS + y * YS];
+      for (int y = 0; y < kSub; ++y) gn[y] = lds_g(sh_g, nb + y * YS);
     }
     float wt[kSub];
 #pragma unroll
@@ -1261,6 +1288,7 @@ __device__ __forceinline__ void bwd_column(const float2* __restrict__ sh_g, int 
     m1 += t1;
     acc.m01 = fmaf(t1, u0, acc.m01);
     acc.m11 += t2;
+    u0 += 1.0f;
   }
   acc.m += m;
   acc.m0 += m0;
@@ -1455,15 +1483,13 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
           const uint16_t cw = sh.cols[k][j];
           const int own_rows = col_rows(cw);
           // The group's largest row count for its j-th columns (over the 16 slots).
-          int rows = own_rows;
-#pragma unroll
-          for (int off = 1; off < 16; off <<= 1) rows = max(rows, __shfl_xor(rows, off, 16));
+          const int rows = row16_max(own_rows);
           const int j1 = cw & 3, j2 = (cw >> 2) & 3, z0 = (cw >> 4) & 15;
           if (partial)
-            bwd_column<TT, true>(sh_g, T, b, z0, rows, own_rows, j1 * kSub, j2 * kSub, xi, slot_ok, c0, c1, c2, rr,
+            bwd_column<TT, true>((LdsFloat*)sh_g, T, b, z0, rows, own_rows, j1 * kSub, j2 * kSub, xi, slot_ok, c0, c1, c2, rr,
                                  ri, a00, a11, a22, b01, b02, b12, acc);
           else
-            bwd_column<TT, false>(sh_g, T, b, z0, rows, own_rows, j1 * kSub, j2 * kSub, xi, slot_ok, c0, c1, c2,
+            bwd_column<TT, false>((LdsFloat*)sh_g, T, b, z0, rows, own_rows, j1 * kSub, j2 * kSub, xi, slot_ok, c0, c1, c2,
                                   rr, ri, a00, a11, a22, b01, b02, b12, acc);
         }
         const float v[kAccStride] = {acc.dr,  acc.di,  acc.m0,  acc.m1,  acc.m2,  acc.m00,
