@@ -806,12 +806,15 @@ __device__ __forceinline__ ZCoefs z_coefs(float a00, float a11, float a22, float
 }
 
 // D extent [zmin, zmax] (relative to the centre) of the projected ellipse over
-// the strip u_i in [lo, hi].
+// the strip u_i in [lo, hi].  Clamps are single med3 instructions: with a
+// non-finite strip (non-finite centre) the result is arbitrary, which is
+// harmless -- such a Gaussian has q = NaN or -inf everywhere and contributes
+// nothing.
 __device__ __forceinline__ void z_span(const ZAxis& a, float lo, float hi, float& zmin, float& zmax) {
-  const float tp = fminf(fmaxf(a.ta, lo), hi), tm = fminf(fmaxf(-a.ta, lo), hi);
+  const float tp = __builtin_amdgcn_fmed3f(a.ta, lo, hi), tm = __builtin_amdgcn_fmed3f(-a.ta, lo, hi);
   const float rr = a.rho * a.rho;
-  const float sp = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-tp, tp, rr), 0.0f));
-  const float sm = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-tm, tm, rr), 0.0f));
+  const float sp = __builtin_amdgcn_sqrtf(__builtin_amdgcn_fmed3f(fmaf(-tp, tp, rr), 0.0f, INFINITY));
+  const float sm = __builtin_amdgcn_sqrtf(__builtin_amdgcn_fmed3f(fmaf(-tm, tm, rr), 0.0f, INFINITY));
   zmax = fmaf(a.ga, sp, a.al * tp);
   zmin = fmaf(-a.ga, sm, a.al * tm);
 }
@@ -837,9 +840,12 @@ struct VoxGeom {
 struct Stage {
   float c0[kBatch], c1[kBatch], c2[kBatch], rr[kBatch], ri[kBatch];
   float a00[kBatch], a11[kBatch], a22[kBatch], b01[kBatch], b02[kBatch], b12[kBatch];
+  float e0[kBatch], e1[kBatch], e2[kBatch];
   uint32_t box[kBatch];
   uint8_t sbm[kBatch];  // sub-bricks touched, bit 4*j0 + 2*j1 + j2 (tiles of at most 2x2x2 sub-bricks)
   uint16_t list[kWaves][kBatch];
+  uint16_t touch[kBatch];  // pairs whose box touches the tile (any order)
+  int n_touch;
 };
 
 struct BlockCtx {
@@ -890,7 +896,8 @@ __device__ __forceinline__ BlockCtx make_ctx(int logical, const VoxGeom& geo, co
 // Stages one pair: copies the record fields and computes the Gaussian's
 // exact extent clipped to the tile, in tile-local voxel indices, packed as
 // 6 x 5 bits (lo0, lo1, lo2, hi0, hi1, hi2); kEmptyBox if it misses the tile.
-__device__ __forceinline__ void stage_pair(Stage& sh, int k, const GRec& r, const BlockCtx& b, bool use_mask) {
+// Returns whether the box touches the tile.
+__device__ __forceinline__ bool stage_pair(Stage& sh, int k, const GRec& r, const BlockCtx& b) {
   sh.c0[k] = r.c0;
   sh.c1[k] = r.c1;
   sh.c2[k] = r.c2;
@@ -902,6 +909,9 @@ __device__ __forceinline__ void stage_pair(Stage& sh, int k, const GRec& r, cons
   sh.b01[k] = r.b01;
   sh.b02[k] = r.b02;
   sh.b12[k] = r.b12;
+  sh.e0[k] = r.e0;
+  sh.e1[k] = r.e1;
+  sh.e2[k] = r.e2;
   // fmaxf / fminf return the non-NaN operand, so a NaN centre keeps the whole
   // tile (its q is NaN and fails the cutoff, as in the reference).
   const float lo0 = fmaxf(ceilf(r.c0 - r.e0), (float)b.o0);
@@ -917,45 +927,74 @@ __device__ __forceinline__ void stage_pair(Stage& sh, int k, const GRec& r, cons
     const uint32_t h0 = (uint32_t)((int)hi0 - b.o0), h1 = (uint32_t)((int)hi1 - b.o1),
                    h2 = (uint32_t)((int)hi2 - b.o2);
     box = l0 | (l1 << 5) | (l2 << 10) | (h0 << 15) | (h1 << 20) | (h2 << 25);
-    // 2x2x2 mask of the sub-bricks that can hold a voxel of the ellipsoid
-    // (only read when the tile has at most 2 sub-bricks per axis): the box's
-    // (H, W) column blocks, each with the D rows its extent bound leaves.
-    if (use_mask) {
-      const ZCoefs zc = z_coefs(r.a00, r.a11, r.a22, r.b01, r.b02, r.b12, r.e0, r.e1, r.e2);
-      float hmn[2], hmx[2], wmn[2], wmx[2];
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        hmn[j] = wmn[j] = -INFINITY;
-        hmx[j] = wmx[j] = INFINITY;
-        if (zc.ok) {
-          const int y0 = max(4 * j, (int)l1), y1 = min(4 * j + 3, (int)h1);
-          const int x0 = max(4 * j, (int)l2), x1 = min(4 * j + 3, (int)h2);
-          z_span(zc.h, (float)(b.o1 + y0) - r.c1, (float)(b.o1 + y1) - r.c1, hmn[j], hmx[j]);
-          z_span(zc.w, (float)(b.o2 + x0) - r.c2, (float)(b.o2 + x1) - r.c2, wmn[j], wmx[j]);
-        }
-      }
-      uint32_t mask = 0;
-#pragma unroll
-      for (int j1 = 0; j1 < 2; ++j1) {
-#pragma unroll
-        for (int j2 = 0; j2 < 2; ++j2) {
-          if ((int)(l1 >> 2) <= j1 && j1 <= (int)(h1 >> 2) && (int)(l2 >> 2) <= j2 && j2 <= (int)(h2 >> 2)) {
-            float zl, zh;
-            column_rows(r.c0, hmn[j1], hmx[j1], wmn[j2], wmx[j2], lo0, hi0, zl, zh);
-            if (zl <= zh) {
-              const int za = (int)zl - b.o0, zb = (int)zh - b.o0;  // tile-local, inside [l0, h0]
-              if (za <= 3) mask |= 1u << (2 * j1 + j2);
-              if (zb >= 4) mask |= 1u << (4 + 2 * j1 + j2);
-            }
-          }
-        }
-      }
-      sh.sbm[k] = (uint8_t)mask;
-    }
-  } else {
-    sh.sbm[k] = 0;
   }
+  sh.sbm[k] = 0;
   sh.box[k] = box;
+  return box != kEmptyBox;
+}
+
+// 2x2x2 mask of the sub-bricks that can hold a voxel of the ellipsoid, for a
+// staged pair whose box touches a tile of at most 2 sub-brick per axis: the
+// box's (H, W) column blocks, each with the D rows its column bound leaves.
+__device__ __forceinline__ uint32_t pair_mask(const Stage& sh, int k, const BlockCtx& b) {
+  const float c0 = sh.c0[k], c1 = sh.c1[k], c2 = sh.c2[k];
+  const uint32_t bx = sh.box[k];
+  const int l0 = bx & 31, l1 = (bx >> 5) & 31, l2 = (bx >> 10) & 31;
+  const int h0 = (bx >> 15) & 31, h1 = (bx >> 20) & 31, h2 = (bx >> 25) & 31;
+  const float lo0 = (float)(b.o0 + l0), hi0 = (float)(b.o0 + h0);
+  const ZCoefs zc = z_coefs(sh.a00[k], sh.a11[k], sh.a22[k], sh.b01[k], sh.b02[k], sh.b12[k], sh.e0[k], sh.e1[k],
+                            sh.e2[k]);
+  float hmn[2], hmx[2], wmn[2], wmx[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    hmn[j] = wmn[j] = -INFINITY;
+    hmx[j] = wmx[j] = INFINITY;
+    if (zc.ok) {
+      const int y0 = max(4 * j, l1), y1 = min(4 * j + 3, h1);
+      const int x0 = max(4 * j, l2), x1 = min(4 * j + 3, h2);
+      z_span(zc.h, (float)(b.o1 + y0) - c1, (float)(b.o1 + y1) - c1, hmn[j], hmx[j]);
+      z_span(zc.w, (float)(b.o2 + x0) - c2, (float)(b.o2 + x1) - c2, wmn[j], wmx[j]);
+    }
+  }
+  uint32_t mask = 0;
+#pragma unroll
+  for (int j1 = 0; j1 < 2; ++j1) {
+#pragma unroll
+    for (int j2 = 0; j2 < 2; ++j2) {
+      if ((l1 >> 2) <= j1 && j1 <= (h1 >> 2) && (l2 >> 2) <= j2 && j2 <= (h2 >> 2)) {
+        float zl, zh;
+        column_rows(c0, hmn[j1], hmx[j1], wmn[j2], wmx[j2], lo0, hi0, zl, zh);
+        if (zl <= zh) {
+          const int za = (int)zl - b.o0, zb = (int)zh - b.o0;  // tile-local, inside [l0, h0]
+          if (za <= 3) mask |= 1u << (2 * j1 + j2);
+          if (zb >= 4) mask |= 1u << (4 + 2 * j1 + j2);
+        }
+      }
+    }
+  }
+  return mask;
+}
+
+// Appends the calling lanes' touching pairs to sh.touch (one LDS atomic per
+// wave; every lane of the wave must call).
+__device__ __forceinline__ void append_touch(Stage& sh, bool touch, int k, int lane) {
+  const uint64_t m = __ballot(touch);
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int base = 0;
+  if (lane == 0 && m != 0) base = atomicAdd(&sh.n_touch, (int)__popcll(m));
+  base = __builtin_amdgcn_readfirstlane(base);
+  if (touch) sh.touch[base + __popcll(m & lt_mask)] = (uint16_t)k;
+}
+
+// Sub-brick masks of the batch's touching pairs, spread over the block's
+// threads (the non-touching ~half of the pairs costs nothing here).  Called
+// between two barriers.
+__device__ __forceinline__ void batch_masks(Stage& sh, const BlockCtx& b) {
+  const int nt = sh.n_touch;
+  for (int i = threadIdx.x; i < nt; i += kTileThreads) {
+    const int k = sh.touch[i];
+    sh.sbm[k] = (uint8_t)pair_mask(sh, k, b);
+  }
 }
 
 // Compacts, in batch order, the staged Gaussians whose box touches this
@@ -1100,15 +1139,23 @@ __global__ __launch_bounds__(kTileThreads) void k_forward(const GRec* __restrict
   // batch is evaluated (loads stay in flight across the LDS barriers).
   GRec nxt = {};
   if ((int)threadIdx.x < min(kBatch, b.p_end - b.p_begin)) nxt = rec[pair_ids[b.p_begin + threadIdx.x]];
+  const bool use_mask = geo.nsb <= 2;
+  if (threadIdx.x == 0) sh.n_touch = 0;
   for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
     const int cnt = min(kBatch, b.p_end - bs);
     __syncthreads();
-    if ((int)threadIdx.x < cnt) stage_pair(sh, threadIdx.x, nxt, b, geo.nsb <= 2);
+    const bool touch = (int)threadIdx.x < cnt && stage_pair(sh, threadIdx.x, nxt, b);
     const int bn = bs + kBatch;
     if (bn < b.p_end && (int)threadIdx.x < min(kBatch, b.p_end - bn)) nxt = rec[pair_ids[bn + threadIdx.x]];
+    if (use_mask) {
+      append_touch(sh, touch, threadIdx.x, lane);
+      __syncthreads();
+      batch_masks(sh, b);
+    }
     __syncthreads();
+    if (threadIdx.x == 0) sh.n_touch = 0;
     if (!b.active) continue;
-    const int nhit = build_list(sh, cnt, b, wave, lane, geo.nsb <= 2);
+    const int nhit = build_list(sh, cnt, b, wave, lane, use_mask);
     if (full)
       fwd_groups<false>(sh, nhit, b, wave, gl, xi, x_ok, acc_r, acc_i);
     else
@@ -1679,13 +1726,21 @@ __global__ __launch_bounds__(kTileThreads) void k_count(const GRec* __restrict__
     atomicAdd(&counts[0], pairs * (unsigned long long)(b.e0 * b.e1 * b.e2));
   }
   unsigned int active = 0, culled = 0;
+  const bool use_mask = geo.nsb <= 2;
+  if (threadIdx.x == 0) sh.n_touch = 0;
   for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
     const int cnt = min(kBatch, b.p_end - bs);
     __syncthreads();
-    if ((int)threadIdx.x < cnt) stage_pair(sh, threadIdx.x, rec[pair_ids[bs + threadIdx.x]], b, geo.nsb <= 2);
+    const bool touch = (int)threadIdx.x < cnt && stage_pair(sh, threadIdx.x, rec[pair_ids[bs + threadIdx.x]], b);
+    if (use_mask) {
+      append_touch(sh, touch, threadIdx.x, lane);
+      __syncthreads();
+      batch_masks(sh, b);
+    }
     __syncthreads();
+    if (threadIdx.x == 0) sh.n_touch = 0;
     if (!b.active) continue;
-    const int nhit = build_list(sh, cnt, b, wave, lane, geo.nsb <= 2);
+    const int nhit = build_list(sh, cnt, b, wave, lane, use_mask);
     for (int gb = 0; gb < nhit; gb += 16) {
       const int idx = gb + gl;
       const bool g_ok = (idx < nhit) && x_ok;
