@@ -169,8 +169,10 @@ int gstv_epilogue(const float* scales, const float* rotations,
 /* Exact evaluation counts of the tile kernels for the given binning:
  *   counts[0] = E_total   = sum over tiles of pairs(t) * voxels(t)
  *   counts[1] = E_active  = evaluations with q <= 9 (the kernel's own test)
- *   counts[2] = E_culled  = evaluations the kernels actually perform after
- *               sub-brick culling (16 Gaussian slots x 64 voxels per group)
+ *               among those the forward kernel keeps after culling
+ *   counts[2] = E_culled  = (Gaussian, voxel) evaluations the forward kernel
+ *               performs after culling 4x4x4 sub-bricks by the Gaussian's
+ *               extent box and its column bound (padding slots excluded)
  * counts is an int64 [3] device array (overwritten).  Arguments otherwise as
  * for gstv_forward (same workspace size). */
 int gstv_count_evaluations(const float* center_vox, const float* cov_inv,
