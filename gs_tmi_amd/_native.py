@@ -43,6 +43,12 @@ PROTOTYPES = {
     "gstv_backward": (ctypes.c_int,
                       [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32,
                        _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _sz, _c_p]),
+    "gstv_forward_keep": (ctypes.c_int,
+                          [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32,
+                           _c_p, _c_p, _c_p, _sz, _c_p, _c_p]),
+    "gstv_backward_keep": (ctypes.c_int,
+                           [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32,
+                            _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _sz, _c_p, _c_p, _c_p]),
     "gstv_count_evaluations": (ctypes.c_int,
                                [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32,
                                 _c_p, _c_p, _sz, _c_p]),
@@ -162,8 +168,11 @@ def bin_fill(center_vox, radius_vox, shape, tile_size, offsets, total_pairs: int
     return tile_ranges, pair_ids
 
 
-def forward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape, tile_size, d, h, w):
-    """Planar forward.  Returns (vol_r, vol_i), each fp32 [d*h*w]."""
+def forward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape, tile_size, d, h, w,
+            keep=False):
+    """Planar forward.  Returns (vol_r, vol_i), each fp32 [d*h*w]; with
+    ``keep``, (vol_r, vol_i, kept) where ``kept`` = (workspace, pair_columns)
+    is what ``backward(..., kept=kept)`` of the same inputs reuses."""
     lib = get_lib()
     _require_gpu(center_vox, "center_vox")
     dev = center_vox.device
@@ -172,17 +181,24 @@ def forward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape
     vol_i = torch.empty(d * h * w, dtype=torch.float32, device=dev)
     ws_bytes = lib.gstv_forward_workspace_bytes(n)
     ws = _workspace(ws_bytes, dev)
+    args = (_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i), _ptr(tile_ranges), tile_ranges.shape[0],
+            _ptr(pair_ids), _ptr(half_shape), n, tile_size, d, h, w, _ptr(vol_r), _ptr(vol_i), _ptr(ws), ws_bytes)
     with torch.cuda.device(dev):
-        st = lib.gstv_forward(_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i), _ptr(tile_ranges),
-                              tile_ranges.shape[0], _ptr(pair_ids), _ptr(half_shape), n, tile_size, d, h, w,
-                              _ptr(vol_r), _ptr(vol_i), _ptr(ws), ws_bytes, _stream(dev))
-    _check(st, "gstv_forward")
+        if keep:
+            cols = torch.empty(max(pair_ids.numel(), 1), dtype=torch.int32, device=dev)
+            st = lib.gstv_forward_keep(*args, _ptr(cols), _stream(dev))
+        else:
+            st = lib.gstv_forward(*args, _stream(dev))
+    _check(st, "gstv_forward_keep" if keep else "gstv_forward")
+    if keep:
+        return vol_r, vol_i, (ws, cols)
     return vol_r, vol_i
 
 
 def backward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape, grad_r, grad_i, tile_size,
-             d, h, w):
-    """Adjoint.  Returns (grad_center [n,3], grad_cov_inv [n,9], grad_rho_r [n], grad_rho_i [n])."""
+             d, h, w, kept=None):
+    """Adjoint.  Returns (grad_center [n,3], grad_cov_inv [n,9], grad_rho_r [n], grad_rho_i [n]).
+    ``kept``: what ``forward(..., keep=True)`` of the same inputs returned."""
     lib = get_lib()
     _require_gpu(center_vox, "center_vox")
     dev = center_vox.device
@@ -193,12 +209,16 @@ def backward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shap
     g_ri = torch.empty(n, dtype=torch.float32, device=dev)
     ws_bytes = lib.gstv_backward_workspace_bytes(n)
     ws = _workspace(ws_bytes, dev)
+    args = (_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i), _ptr(tile_ranges), tile_ranges.shape[0],
+            _ptr(pair_ids), _ptr(half_shape), n, _ptr(grad_r), _ptr(grad_i), tile_size, d, h, w, _ptr(g_c),
+            _ptr(g_m), _ptr(g_rr), _ptr(g_ri), _ptr(ws), ws_bytes)
     with torch.cuda.device(dev):
-        st = lib.gstv_backward(_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i), _ptr(tile_ranges),
-                               tile_ranges.shape[0], _ptr(pair_ids), _ptr(half_shape), n, _ptr(grad_r),
-                               _ptr(grad_i), tile_size, d, h, w, _ptr(g_c), _ptr(g_m), _ptr(g_rr), _ptr(g_ri),
-                               _ptr(ws), ws_bytes, _stream(dev))
-    _check(st, "gstv_backward")
+        if kept is not None:
+            fws, cols = kept
+            st = lib.gstv_backward_keep(*args, _ptr(fws), _ptr(cols), _stream(dev))
+        else:
+            st = lib.gstv_backward(*args, _stream(dev))
+    _check(st, "gstv_backward_keep" if kept is not None else "gstv_backward")
     return g_c, g_m, g_rr, g_ri
 
 

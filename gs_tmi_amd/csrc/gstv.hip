@@ -933,10 +933,19 @@ __device__ __forceinline__ bool stage_pair(Stage& sh, int k, const GRec& r, cons
   return box != kEmptyBox;
 }
 
+// Column word of a pair in a tile of at most 8 voxels per axis: for column
+// c = 2 j1 + j2 (H block j1, W block j2) the 7-bit field at bit 7c holds
+// live (bit 0), first D row (bits 1-3) and rows - 1 (bits 4-6), tile-local;
+// 0 for a column the bound leaves empty.
+__device__ __forceinline__ uint32_t col_field(int z_lo, int z_hi) {
+  return 1u | ((uint32_t)z_lo << 1) | ((uint32_t)(z_hi - z_lo) << 4);
+}
+
 // 2x2x2 mask of the sub-bricks that can hold a voxel of the ellipsoid, for a
 // staged pair whose box touches a tile of at most 2 sub-brick per axis: the
 // box's (H, W) column blocks, each with the D rows its column bound leaves.
-__device__ __forceinline__ uint32_t pair_mask(const Stage& sh, int k, const BlockCtx& b) {
+// The same rows are returned as a column word in `cols`.
+__device__ __forceinline__ uint32_t pair_mask(const Stage& sh, int k, const BlockCtx& b, uint32_t& cols) {
   const float c0 = sh.c0[k], c1 = sh.c1[k], c2 = sh.c2[k];
   const uint32_t bx = sh.box[k];
   const int l0 = bx & 31, l1 = (bx >> 5) & 31, l2 = (bx >> 10) & 31;
@@ -957,6 +966,7 @@ __device__ __forceinline__ uint32_t pair_mask(const Stage& sh, int k, const Bloc
     }
   }
   uint32_t mask = 0;
+  cols = 0;
 #pragma unroll
   for (int j1 = 0; j1 < 2; ++j1) {
 #pragma unroll
@@ -968,6 +978,7 @@ __device__ __forceinline__ uint32_t pair_mask(const Stage& sh, int k, const Bloc
           const int za = (int)zl - b.o0, zb = (int)zh - b.o0;  // tile-local, inside [l0, h0]
           if (za <= 3) mask |= 1u << (2 * j1 + j2);
           if (zb >= 4) mask |= 1u << (4 + 2 * j1 + j2);
+          cols |= col_field(za, zb) << (7 * (2 * j1 + j2));
         }
       }
     }
@@ -987,13 +998,16 @@ __device__ __forceinline__ void append_touch(Stage& sh, bool touch, int k, int l
 }
 
 // Sub-brick masks of the batch's touching pairs, spread over the block's
-// threads (the non-touching ~half of the pairs costs nothing here).  Called
-// between two barriers.
-__device__ __forceinline__ void batch_masks(Stage& sh, const BlockCtx& b) {
+// threads (the non-touching ~half of the pairs costs nothing here), and, when
+// pair_cols is given, their column words at the pairs' positions bs + k.
+// Called between two barriers.
+__device__ __forceinline__ void batch_masks(Stage& sh, const BlockCtx& b, int bs, uint32_t* __restrict__ pair_cols) {
   const int nt = sh.n_touch;
   for (int i = threadIdx.x; i < nt; i += kTileThreads) {
     const int k = sh.touch[i];
-    sh.sbm[k] = (uint8_t)pair_mask(sh, k, b);
+    uint32_t cols;
+    sh.sbm[k] = (uint8_t)pair_mask(sh, k, b, cols);
+    if (pair_cols != nullptr) pair_cols[bs + k] = cols;
   }
 }
 
@@ -1116,7 +1130,8 @@ __device__ __forceinline__ void fwd_groups(const Stage& sh, int nhit, const Bloc
 __global__ __launch_bounds__(kTileThreads) void k_forward(const GRec* __restrict__ rec,
                                                           const int32_t* __restrict__ ranges,
                                                           const int32_t* __restrict__ pair_ids, VoxGeom geo,
-                                                          float* __restrict__ vol_r, float* __restrict__ vol_i) {
+                                                          float* __restrict__ vol_r, float* __restrict__ vol_i,
+                                                          uint32_t* __restrict__ pair_cols) {
   __shared__ Stage sh;
   const int logical = xcd_remap(blockIdx.x, gridDim.x);
   if (logical >= geo.blocks) return;
@@ -1150,7 +1165,7 @@ __global__ __launch_bounds__(kTileThreads) void k_forward(const GRec* __restrict
     if (use_mask) {
       append_touch(sh, touch, threadIdx.x, lane);
       __syncthreads();
-      batch_masks(sh, b);
+      batch_masks(sh, b, bs, pair_cols);
     }
     __syncthreads();
     if (threadIdx.x == 0) sh.n_touch = 0;
@@ -1346,13 +1361,16 @@ __device__ __forceinline__ void bwd_column(LdsFloat* __restrict__ sh_g, int tp, 
   acc.m12 = fmaf(u2, m1, acc.m12);
 }
 
+// pair_cols (optional, tiles of at most 8 voxels per axis): the forward's
+// column words of the same pairs, used instead of recomputing the bound.
 template <int TT>
 __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restrict__ rec,
                                                            const int32_t* __restrict__ ranges,
                                                            const int32_t* __restrict__ pair_ids, VoxGeom geo,
                                                            const float* __restrict__ grad_r,
                                                            const float* __restrict__ grad_i,
-                                                           float* __restrict__ gacc) {
+                                                           float* __restrict__ gacc,
+                                                           const uint32_t* __restrict__ pair_cols) {
   constexpr int NB = TT > 0 ? (TT + kSub - 1) / kSub : kMaxBlk;  // column blocks per axis (bound)
   constexpr int MC = NB * NB;
   __shared__ BwdStage<MC> sh;
@@ -1385,11 +1403,14 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
 
   // Records of the next batch are loaded into registers while the current
   // batch is evaluated (loads stay in flight across the LDS barriers).
+  const bool use_cols = pair_cols != nullptr && nb <= 2;
   GRec nxt = {};
   int nxt_id = 0;
+  uint32_t nxt_cols = 0;
   if (tid < min(kBatch, b.p_end - b.p_begin)) {
     nxt_id = pair_ids[b.p_begin + tid];
     nxt = rec[nxt_id];
+    if (use_cols) nxt_cols = pair_cols[b.p_begin + tid];
   }
   for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
     const int cnt = min(kBatch, b.p_end - bs);
@@ -1409,14 +1430,35 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
       sh.b01[tid] = r.b01;
       sh.b02[tid] = r.b02;
       sh.b12[tid] = r.b12;
+      uint16_t key = kNoKey;
       const float lo0 = fmaxf(ceilf(r.c0 - r.e0), (float)b.o0);
       const float lo1 = fmaxf(ceilf(r.c1 - r.e1), (float)b.o1);
       const float lo2 = fmaxf(ceilf(r.c2 - r.e2), (float)b.o2);
       const float hi0 = fminf(floorf(r.c0 + r.e0), (float)(b.o0 + b.e0 - 1));
       const float hi1 = fminf(floorf(r.c1 + r.e1), (float)(b.o1 + b.e1 - 1));
       const float hi2 = fminf(floorf(r.c2 + r.e2), (float)(b.o2 + b.e2 - 1));
-      uint16_t key = kNoKey;
-      if (lo0 <= hi0 && lo1 <= hi1 && lo2 <= hi2) {
+      const bool touch = lo0 <= hi0 && lo1 <= hi1 && lo2 <= hi2;
+      if (use_cols && touch) {
+        // Live columns from the forward's column word (written for the pairs
+        // whose box touches the tile), in (H, W) block order.
+        uint16_t* cl = sh.cols[tid];
+        int ncol = 0, mxr = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const uint32_t f = (nxt_cols >> (7 * c)) & 127u;
+          if (f & 1u) {
+            const int rows = (int)((f >> 4) & 7u) + 1;
+            cl[ncol] = pack_col(c >> 1, c & 1, (int)((f >> 1) & 7u), rows);
+            ++ncol;
+            mxr = max(mxr, rows);
+          }
+        }
+        if (ncol > 0) {
+          key = (uint16_t)((ncol - 1) * kMaxRows + mxr - 1);
+          sh.slot[tid] = (uint16_t)atomicAdd(&sh.key_cnt[key], 1);
+        }
+      }
+      if (!use_cols && touch) {
         const int y_lo = (int)lo1 - b.o1, y_hi = (int)hi1 - b.o1;  // tile-local box
         const int x_lo = (int)lo2 - b.o2, x_hi = (int)hi2 - b.o2;
         const int l1 = y_lo >> 2, h1 = y_hi >> 2, l2 = x_lo >> 2, h2 = x_hi >> 2;
@@ -1466,6 +1508,7 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
       if (bn < b.p_end && tid < min(kBatch, b.p_end - bn)) {
         nxt_id = pair_ids[bn + tid];
         nxt = rec[nxt_id];
+        if (use_cols) nxt_cols = pair_cols[bn + tid];
       }
     }
     __syncthreads();
@@ -1735,7 +1778,7 @@ __global__ __launch_bounds__(kTileThreads) void k_count(const GRec* __restrict__
     if (use_mask) {
       append_touch(sh, touch, threadIdx.x, lane);
       __syncthreads();
-      batch_masks(sh, b);
+      batch_masks(sh, b, bs, nullptr);
     }
     __syncthreads();
     if (threadIdx.x == 0) sh.n_touch = 0;
@@ -2051,10 +2094,11 @@ int launch_prep(const float* center_vox, const float* cov_inv, const float* rho_
 int tile_grid(const VoxGeom& g) { return ((g.blocks + 7) / 8) * 8; }
 }  // namespace
 
-int gstv_forward(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
+namespace {
+int forward_impl(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
                  const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
                  int32_t n, int32_t tile, int32_t d, int32_t h, int32_t w, float* vol_r, float* vol_i,
-                 void* workspace, size_t workspace_bytes, void* stream) {
+                 void* workspace, size_t workspace_bytes, uint32_t* pair_columns, void* stream) {
   int nt[3];
   int st = check_tile_args(tile_ranges, num_tiles, n, tile, d, h, w, nt);
   if (st != GSTV_OK) return st;
@@ -2071,10 +2115,29 @@ int gstv_forward(const float* center_vox, const float* cov_inv, const float* rho
   const VoxGeom geo = make_vox_geom(d, h, w, tile, nt, num_tiles);
   const int slot = prof_start(0, s);
   hipLaunchKernelGGL(k_forward, dim3(tile_grid(geo)), dim3(kTileThreads), 0, s, (const GRec*)rec, tile_ranges,
-                     pair_ids, geo, vol_r, vol_i);
+                     pair_ids, geo, vol_r, vol_i, geo.nsb <= 2 ? pair_columns : nullptr);
   GSTV_LAUNCH_CHECK("forward");
   prof_stop(0, slot, s);
   return GSTV_OK;
+}
+}  // namespace
+
+int gstv_forward(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
+                 const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
+                 int32_t n, int32_t tile, int32_t d, int32_t h, int32_t w, float* vol_r, float* vol_i,
+                 void* workspace, size_t workspace_bytes, void* stream) {
+  return forward_impl(center_vox, cov_inv, rho_r, rho_i, tile_ranges, num_tiles, pair_ids, half_shape, n, tile, d,
+                      h, w, vol_r, vol_i, workspace, workspace_bytes, nullptr, stream);
+}
+
+int gstv_forward_keep(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
+                      const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
+                      const float* half_shape, int32_t n, int32_t tile, int32_t d, int32_t h, int32_t w,
+                      float* vol_r, float* vol_i, void* workspace, size_t workspace_bytes,
+                      uint32_t* pair_columns, void* stream) {
+  if (tile <= 8 && n > 0 && pair_columns == nullptr) return GSTV_ERR_INVALID_ARGUMENT;
+  return forward_impl(center_vox, cov_inv, rho_r, rho_i, tile_ranges, num_tiles, pair_ids, half_shape, n, tile, d,
+                      h, w, vol_r, vol_i, workspace, workspace_bytes, pair_columns, stream);
 }
 
 size_t gstv_backward_workspace_bytes(int32_t n) {
@@ -2085,11 +2148,13 @@ size_t gstv_backward_workspace_bytes(int32_t n) {
   return align_up(cv.off, 256);
 }
 
-int gstv_backward(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
+namespace {
+int backward_impl(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
                   const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
                   int32_t n, const float* grad_r, const float* grad_i, int32_t tile, int32_t d, int32_t h,
                   int32_t w, float* grad_center, float* grad_cov_inv, float* grad_rho_r, float* grad_rho_i,
-                  void* workspace, size_t workspace_bytes, void* stream) {
+                  void* workspace, size_t workspace_bytes, const void* forward_workspace,
+                  const uint32_t* pair_columns, void* stream) {
   int nt[3];
   int st = check_tile_args(tile_ranges, num_tiles, n, tile, d, h, w, nt);
   if (st != GSTV_OK) return st;
@@ -2104,8 +2169,12 @@ int gstv_backward(const float* center_vox, const float* cov_inv, const float* rh
   Carve cv(workspace);
   GRec* rec = cv.take<GRec>((size_t)n);
   float* gacc = cv.take<float>((size_t)n * kAccStride);
-  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, half_shape, n, rec, s);
-  if (st != GSTV_OK) return st;
+  if (forward_workspace != nullptr) {
+    rec = static_cast<GRec*>(const_cast<void*>(forward_workspace));  // records of the same call's forward
+  } else {
+    st = launch_prep(center_vox, cov_inv, rho_r, rho_i, half_shape, n, rec, s);
+    if (st != GSTV_OK) return st;
+  }
   const int64_t nacc = (int64_t)n * kAccStride;
   hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)ceil_div_l(nacc, 256)), dim3(256), 0, s, gacc, nacc, 0.0f);
   GSTV_LAUNCH_CHECK("grad_clear");
@@ -2116,18 +2185,43 @@ int gstv_backward(const float* center_vox, const float* cov_inv, const float* rh
   geo_b.blocks = num_tiles;
   const int tp = geo.nsb * kSub;
   const size_t lds_grad = (size_t)tp * grad_zs(tp) * sizeof(float2);
+  const uint32_t* cols = geo.nsb <= 2 ? pair_columns : nullptr;
   if (tile == 8)
     hipLaunchKernelGGL(k_backward<8>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, (const GRec*)rec,
-                       tile_ranges, pair_ids, geo_b, grad_r, grad_i, gacc);
+                       tile_ranges, pair_ids, geo_b, grad_r, grad_i, gacc, cols);
   else
     hipLaunchKernelGGL(k_backward<0>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, (const GRec*)rec,
-                       tile_ranges, pair_ids, geo_b, grad_r, grad_i, gacc);
+                       tile_ranges, pair_ids, geo_b, grad_r, grad_i, gacc, cols);
   GSTV_LAUNCH_CHECK("backward");
   prof_stop(1, slot, s);
   hipLaunchKernelGGL(k_finalize, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, (const float*)gacc, cov_inv,
                      half_shape, n, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i);
   GSTV_LAUNCH_CHECK("finalize");
   return GSTV_OK;
+}
+}  // namespace
+
+int gstv_backward(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
+                  const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
+                  int32_t n, const float* grad_r, const float* grad_i, int32_t tile, int32_t d, int32_t h,
+                  int32_t w, float* grad_center, float* grad_cov_inv, float* grad_rho_r, float* grad_rho_i,
+                  void* workspace, size_t workspace_bytes, void* stream) {
+  return backward_impl(center_vox, cov_inv, rho_r, rho_i, tile_ranges, num_tiles, pair_ids, half_shape, n, grad_r,
+                       grad_i, tile, d, h, w, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, workspace,
+                       workspace_bytes, nullptr, nullptr, stream);
+}
+
+int gstv_backward_keep(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
+                       const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
+                       const float* half_shape, int32_t n, const float* grad_r, const float* grad_i, int32_t tile,
+                       int32_t d, int32_t h, int32_t w, float* grad_center, float* grad_cov_inv,
+                       float* grad_rho_r, float* grad_rho_i, void* workspace, size_t workspace_bytes,
+                       const void* forward_workspace, const uint32_t* pair_columns, void* stream) {
+  if (n > 0 && (forward_workspace == nullptr || (tile <= 8 && pair_columns == nullptr)))
+    return GSTV_ERR_INVALID_ARGUMENT;
+  return backward_impl(center_vox, cov_inv, rho_r, rho_i, tile_ranges, num_tiles, pair_ids, half_shape, n, grad_r,
+                       grad_i, tile, d, h, w, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, workspace,
+                       workspace_bytes, forward_workspace, pair_columns, stream);
 }
 
 int gstv_count_evaluations(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,

@@ -119,8 +119,10 @@ class _TileVoxelizeFusedHIP(torch.autograd.Function):
         tile_ranges, pair_ids = build_tile_pairs(center, radius, volume_shape, tile_size)
         rr = density_r.detach().contiguous().float()
         ri = density_i.detach().contiguous().float()
-        vol_r, vol_i = _native.forward(center, cov, rr, ri, tile_ranges, pair_ids, half_shape, tile_size, d_size,
-                                       h_size, w_size)
+        # The forward keeps its Gaussian records and per-pair column rows for
+        # the adjoint of this call, which then rebuilds neither.
+        vol_r, vol_i, ctx.kept = _native.forward(center, cov, rr, ri, tile_ranges, pair_ids, half_shape, tile_size,
+                                                 d_size, h_size, w_size, keep=True)
         ctx.save_for_backward(scales_c, rot_c, center, cov, rr, ri, tile_ranges, pair_ids, half_shape)
         ctx.tile_size = tile_size
         ctx.dims = (d_size, h_size, w_size)
@@ -138,7 +140,8 @@ class _TileVoxelizeFusedHIP(torch.autograd.Function):
         g_c, g_m, g_rr, g_ri = _native.backward(
             center, cov, rr, ri, tile_ranges, pair_ids, half_shape,
             grad_vol_r.contiguous().float().view(-1), grad_vol_i.contiguous().float().view(-1),
-            ctx.tile_size, d_size, h_size, w_size)
+            ctx.tile_size, d_size, h_size, w_size, kept=ctx.kept)
+        ctx.kept = None
         g_p, g_s, g_q = _native.epilogue(scales_c, rot_c, g_c, g_m, shape)
         return g_p, g_s, g_q, g_rr, g_ri, None, None
 

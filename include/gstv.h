@@ -139,6 +139,40 @@ int gstv_backward(const float* center_vox, const float* cov_inv,
                   float* grad_rho_r, float* grad_rho_i,
                   void* workspace, size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------- forward and adjoint of one call --
+ * No counterpart in the reference: a fast path for callers that run the
+ * adjoint of the same call right after the forward (the fused autograd call).
+ * gstv_forward_keep is gstv_forward that also leaves, for that adjoint,
+ *   - the Gaussian records in its workspace (keep it unchanged), and
+ *   - pair_columns [total pairs] (uint32, parallel to pair_ids): for every pair
+ *     whose extent box touches its tile, the D rows that its column bound
+ *     leaves in each 4x4 (H, W) column of the tile (7 bits per column; other
+ *     entries are left unwritten).  Required, and written, when tile <= 8;
+ *     ignored for larger tiles.
+ * gstv_backward_keep is gstv_backward given that workspace and pair_columns,
+ * for exactly the inputs of that forward; it skips rebuilding the records and
+ * the column bounds.  Results equal gstv_backward's up to the order of fp32
+ * atomic additions. */
+int gstv_forward_keep(const float* center_vox, const float* cov_inv,
+                      const float* rho_r, const float* rho_i,
+                      const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
+                      const float* half_shape, int32_t n,
+                      int32_t tile, int32_t d, int32_t h, int32_t w,
+                      float* vol_r, float* vol_i,
+                      void* workspace, size_t workspace_bytes,
+                      uint32_t* pair_columns, void* stream);
+
+int gstv_backward_keep(const float* center_vox, const float* cov_inv,
+                       const float* rho_r, const float* rho_i,
+                       const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
+                       const float* half_shape, int32_t n,
+                       const float* grad_r, const float* grad_i,
+                       int32_t tile, int32_t d, int32_t h, int32_t w,
+                       float* grad_center, float* grad_cov_inv,
+                       float* grad_rho_r, float* grad_rho_i,
+                       void* workspace, size_t workspace_bytes,
+                       const void* forward_workspace, const uint32_t* pair_columns, void* stream);
+
 /* ------------------------------------------------------ prologue / epilogue --
  * Replaces the torch prologue of TileVoxelizer.forward (gaussian/tile_voxelizer.py:281-290,
  * with _quaternion_to_rotation_matrix :77-84) and torch autograd through it.

@@ -163,6 +163,34 @@ def test_culling_keeps_every_active_evaluation(name, make):
     assert e_active <= e_culled <= e_total
 
 
+@pytest.mark.parametrize("name,make", ALL_CASES, ids=[n for n, _ in ALL_CASES])
+def test_keep_path_matches_plain_calls(name, make):
+    """gstv_forward_keep / gstv_backward_keep (records and column rows handed
+    from the forward to the adjoint) give the plain calls' results: the same
+    volume bit for bit, gradients up to the order of fp32 atomics."""
+    from gs_tmi_amd import _native
+
+    c = make()
+    d, h, w = c["shape"]
+    _announce(f"keep path {name} shape={c['shape']} tile={c['tile']} n={len(c['rr'])}")
+    center, cov, rr, ri, ranges, pids, hs = _function_inputs(c)
+    args = (center, cov, rr, ri, ranges, pids, hs)
+    vol_r, vol_i = _native.forward(*args, c["tile"], d, h, w)
+    kvol_r, kvol_i, kept = _native.forward(*args, c["tile"], d, h, w, keep=True)
+    assert torch.equal(vol_r, kvol_r) and torch.equal(vol_i, kvol_i)
+    rng = np.random.default_rng(9)
+    g_r = torch.from_numpy(rng.normal(size=d * h * w).astype(np.float32)).to(DEV)
+    g_i = torch.from_numpy(rng.normal(size=d * h * w).astype(np.float32)).to(DEV)
+    plain = _native.backward(*args, g_r, g_i, c["tile"], d, h, w)
+    keep = _native.backward(*args, g_r, g_i, c["tile"], d, h, w, kept=kept)
+    torch.cuda.synchronize()
+    # The needles of the extreme-anisotropy case amplify the fp32 atomic-order
+    # differences between any two calls (centre gradient ~3e-6).
+    tol = 2e-5 if name in CASE_TOL else 1e-6
+    for a, b in zip(plain, keep):
+        assert cases.rel_l2(b.cpu().numpy(), a.cpu().numpy()) < tol
+
+
 def test_empty_and_degenerate_inputs():
     from gs_tmi_amd import _native, build_tile_pairs
 
