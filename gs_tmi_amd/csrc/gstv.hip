@@ -1230,8 +1230,8 @@ constexpr int kAccStride = 12;
 // LDS banks); Gaussians are keyed by (live columns, largest row count) and
 // counting-sorted, a group is 16 Gaussians with the same column count and
 // neighbouring row counts, and the group runs its j-th columns for the largest
-// row count among them.  Groups are dealt to the 8 waves in descending cost
-// order for balance.
+// row count among them.  The 8 waves take groups from a block-wide counter in
+// descending cost order, so a wave that finishes early takes the next group.
 
 constexpr int kMaxBlk = 4;                  // column blocks per axis, tile <= 16
 constexpr int kMaxCols = kMaxBlk * kMaxBlk;  // columns per tile
@@ -1267,6 +1267,8 @@ struct BwdStage {
   int seg_begin[kMaxCols];        // per column count cc (index cc - 1): first sorted position
   int seg_cnt[kMaxCols];          //   number of Gaussians
   int grp_first[kMaxCols];        //   first group; groups run in descending column count
+  int n_groups;                   // groups of the batch
+  int next_group;                 // next group to take
 };
 
 struct BwdAcc {
@@ -1416,6 +1418,7 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
     const int cnt = min(kBatch, b.p_end - bs);
     __syncthreads();
     // Stage: parameters, live columns and key of every pair of the batch.
+    if (tid == 0) sh.next_group = 0;
     if (tid < cnt) {
       const GRec r = nxt;
       sh.gid[tid] = nxt_id;
@@ -1547,19 +1550,28 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
         sh.seg_cnt[lane >> 2] = seg;
         sh.grp_first[lane >> 2] = gin - ng;
       }
+      if (lane == 0) sh.n_groups = gin;
     }
     __syncthreads();
     if (tid < cnt && sh.key[tid] != kNoKey) sh.order[sh.key_start[sh.key[tid]] + sh.slot[tid]] = (uint16_t)tid;
     __syncthreads();
 
-    // Groups are dealt round-robin to the waves: global group g goes to wave g % 8.
+    // Groups are taken in descending cost order (column count, then rows) from
+    // a block-wide counter, so a wave that finishes early takes the next one.
+    const int n_groups = sh.n_groups;
 #pragma unroll 1
-    for (int cc = MC; cc >= 1; --cc) {
+    for (;;) {
+      int g_next = 0;
+      if (lane == 0) g_next = atomicAdd(&sh.next_group, 1);
+      g_next = __builtin_amdgcn_readfirstlane(g_next);
+      if (g_next >= n_groups) break;
+      // Its segment (column count cc): grp_first[cc-1] <= g < grp_first[cc-1] + groups(cc).
+      int cc = MC;
+      while (g_next >= sh.grp_first[cc - 1] + ((sh.seg_cnt[cc - 1] + 15) >> 4)) --cc;
       const int seg_begin = sh.seg_begin[cc - 1];
       const int seg_cnt = sh.seg_cnt[cc - 1];
-      const int first = sh.grp_first[cc - 1];
-      const int ng = (seg_cnt + 15) >> 4;
-      for (int r = (wave - first) & (kWaves - 1); r < ng; r += kWaves) {
+      {
+        const int r = g_next - sh.grp_first[cc - 1];
         // Group r holds sorted positions top - 15 .. top (largest keys first).
         const int top = seg_begin + seg_cnt - 1 - 16 * r;
         const int pos = top - gl;
