@@ -46,10 +46,10 @@ TILE = 8
 
 # HBM bytes per k_backward launch at this workload, from two rocprofv3 passes
 # (FETCH_SIZE, WRITE_SIZE) scaled by the same counters' reading of a 512 MiB
-# copy: 544.4 MB read + 778.5 MB written (the writes are mostly the fp32
+# copy: 659.7 MB read + 778.5 MB written (the writes are mostly the fp32
 # moment atomics).  Counters cannot be read from inside this process, so the
 # figure is a measured constant of the current kernels.
-BACKWARD_TRAFFIC_BYTES = 1_322_900_000
+BACKWARD_TRAFFIC_BYTES = 1_438_200_000
 
 # Algorithmic FLOPs per evaluation, counted from the reference kernel
 # (FMA = 2, mul/add/sub/div = 1, exp = 1): forward 26 ungated + 6 gated,
