@@ -1245,12 +1245,12 @@ __device__ __forceinline__ uint16_t pack_col(int j1, int j2, int z0, int rows) {
 }
 __device__ __forceinline__ int col_rows(uint16_t c) { return ((c >> 8) & 15) + 1; }
 
-// Strides (float2 elements) of the tile's upstream-gradient copy in LDS.  With
-// rows padded by 2 and slabs by 4 elements, the 4 x 4 column blocks of an 8^3
-// tile at different D origins fall on different groups of 8 banks, so the 64
-// lanes of a wave (64-bit loads) rarely conflict.
+// Strides (float2 elements) of the tile's upstream-gradient copy in LDS: rows
+// padded by 2 and slabs by 2 elements spread the 16 Gaussians' column reads of
+// a wave over the banks (a sweep of row/slab paddings on the microbenchmark
+// found this among the fastest, ~1 % ahead of slabs padded by 4).
 __host__ __device__ constexpr int grad_ys(int t) { return t + 2; }
-__host__ __device__ constexpr int grad_zs(int t) { return t * (t + 2) + 4; }
+__host__ __device__ constexpr int grad_zs(int t) { return t * (t + 2) + 2; }
 
 // MC: live columns a pair can have (column blocks per tile).
 template <int MC>
