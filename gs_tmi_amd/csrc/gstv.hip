@@ -364,11 +364,17 @@ __global__ __launch_bounds__(kExpandThreads) void k_bin_expand(
 // their boxes go to LDS (first tile id, tiles per D slab, tiles per H row),
 // and every slot gets its Gaussian from an LDS slot -> Gaussian map.
 
+// Gaussians whose boxes a generating block keeps in LDS.  A block of
+// kRadixTile slots spanning more (only Gaussians with a few pairs each, or none)
+// searches per slot instead; the smaller table keeps LDS per block under 40 KB
+// so that four blocks share a CU.
+constexpr int kGenCap = 1920;
+
 struct GenLds {
-  int32_t off[kRadixTile + 1];
-  int32_t k0[kRadixTile + 1];
-  int32_t n12[kRadixTile + 1];
-  int32_t n2[kRadixTile + 1];
+  int32_t off[kGenCap];
+  int32_t k0[kGenCap];
+  int32_t n12[kGenCap];
+  int32_t n2[kGenCap];
   uint16_t gidx[kRadixTile];
   int g0, ng;
 };
@@ -396,8 +402,8 @@ __global__ void k_block_first(const int32_t* __restrict__ offsets, int n, int to
 
 // Block-wide: maps the slots [j0, j1) of block b to their Gaussians
 // (first[b] .. first[b + 1], an upper bound of the last owner).  Returns false
-// (uniformly) when more than kRadixTile + 1 Gaussians span the block, which
-// needs Gaussians with zero pairs inside it; callers then search per slot.
+// (uniformly) when more than kGenCap Gaussians span the block; callers then
+// search per slot.
 __device__ bool gen_setup(GenLds& L, const float* __restrict__ center, const float* __restrict__ radius,
                           const BinGeom& geo, const int32_t* __restrict__ offsets,
                           const int32_t* __restrict__ first, int j0, int j1) {
@@ -405,7 +411,7 @@ __device__ bool gen_setup(GenLds& L, const float* __restrict__ center, const flo
   const int g0 = first[b];
   const int ng = first[b + 1] - g0 + 1;
   if (threadIdx.x == 0) L.g0 = g0;
-  if (ng > kRadixTile + 1) return false;
+  if (ng > kGenCap) return false;
   const int ntw = geo.nt[2];
   const int nthw = geo.nt[1] * geo.nt[2];
   for (int i = threadIdx.x; i < ng; i += kRadixThreads) {
@@ -473,7 +479,7 @@ struct PairSource {
 // digit-major (hist[digit * nblocks + block]) so one exclusive scan gives
 // every block's scatter base per digit.
 template <bool kGen>
-__global__ __launch_bounds__(kRadixThreads) void k_radix_hist(PairSource src, int total, int shift, int bits,
+__global__ __launch_bounds__(kRadixThreads, 4) void k_radix_hist(PairSource src, int total, int shift, int bits,
                                                              int nblocks, int32_t* __restrict__ hist) {
   struct NoGen {};
   __shared__ int32_t sh_h[1 << kRadixBits];
@@ -631,7 +637,7 @@ __device__ __forceinline__ void radix_scatter(const PairSource& src, int total, 
 
 // First pass (pairs generated in the kernel; LDS limits it to 2 blocks per CU).
 template <bool kLast>
-__global__ __launch_bounds__(kRadixThreads) void k_radix_scatter_gen(PairSource src, int total, int shift, int bits,
+__global__ __launch_bounds__(kRadixThreads, 4) void k_radix_scatter_gen(PairSource src, int total, int shift, int bits,
                                                                     int nblocks,
                                                                     const int32_t* __restrict__ hist_scanned,
                                                                     int32_t* __restrict__ keys_out,

@@ -167,7 +167,7 @@ def test_culling_keeps_every_active_evaluation(name, make):
 def test_keep_path_matches_plain_calls(name, make):
     """gstv_forward_keep / gstv_backward_keep (records and column rows handed
     from the forward to the adjoint) give the plain calls' results: the same
-    volume bit for bit, gradients up to the order of fp32 atomics."""
+    volume bit for bit, gradients up to the order of fp32 atomic additions."""
     from gs_tmi_amd import _native
 
     c = make()
@@ -184,9 +184,10 @@ def test_keep_path_matches_plain_calls(name, make):
     plain = _native.backward(*args, g_r, g_i, c["tile"], d, h, w)
     keep = _native.backward(*args, g_r, g_i, c["tile"], d, h, w, kept=kept)
     torch.cuda.synchronize()
-    # The needles of the extreme-anisotropy case amplify the fp32 atomic-order
-    # differences between any two calls (centre gradient ~3e-6).
-    tol = 2e-5 if name in CASE_TOL else 1e-6
+    # Two calls differ by the order of fp32 atomic additions (~1e-6 on small
+    # cases); the needles of the extreme-anisotropy case amplify it (centre
+    # gradient ~3e-6).
+    tol = 2e-5 if name in CASE_TOL else 5e-6
     for a, b in zip(plain, keep):
         assert cases.rel_l2(b.cpu().numpy(), a.cpu().numpy()) < tol
 
