@@ -44,11 +44,11 @@ PROTOTYPES = {
                       [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32,
                        _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _sz, _c_p]),
     "gstv_forward_keep": (ctypes.c_int,
-                          [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32,
+                          [_c_p, _c_p, _c_p, _c_p, _i32, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32,
                            _c_p, _c_p, _c_p, _sz, _c_p, _c_p]),
     "gstv_backward_keep": (ctypes.c_int,
-                           [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32,
-                            _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _sz, _c_p, _c_p, _c_p]),
+                           [_c_p, _c_p, _c_p, _c_p, _i32, _c_p, _i32, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32,
+                            _i32, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _sz, _c_p, _c_p, _c_p]),
     "gstv_count_evaluations": (ctypes.c_int,
                                [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32,
                                 _c_p, _c_p, _sz, _c_p]),
@@ -168,27 +168,47 @@ def bin_fill(center_vox, radius_vox, shape, tile_size, offsets, total_pairs: int
     return tile_ranges, pair_ids
 
 
+def _rho_stride(rho_r, rho_i, keep):
+    """Element stride shared by the 1-D density views rho_r / rho_i: 1 for
+    planar arrays; with ``keep``, also 2 for the columns of an interleaved
+    [n, 2] (complex64) array."""
+    if rho_r.dim() != 1 or rho_i.dim() != 1:
+        raise ValueError("rho_r and rho_i must be 1-D")
+    st = rho_r.stride(0)
+    if rho_i.stride(0) != st:
+        raise ValueError("rho_r and rho_i must have the same stride")
+    if st != 1 and not keep:
+        raise ValueError("strided density views need the keep entry points")
+    if st < 1:
+        raise ValueError("density stride must be positive")
+    return st
+
+
 def forward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape, tile_size, d, h, w,
             keep=False):
     """Planar forward.  Returns (vol_r, vol_i), each fp32 [d*h*w]; with
     ``keep``, (vol_r, vol_i, kept) where ``kept`` = (workspace, pair_columns)
-    is what ``backward(..., kept=kept)`` of the same inputs reuses."""
+    is what ``backward(..., kept=kept)`` of the same inputs reuses.  With
+    ``keep``, rho_r / rho_i may also be the two columns of one interleaved
+    [n, 2] fp32 array (``torch.view_as_real`` of a complex64 density)."""
     lib = get_lib()
     _require_gpu(center_vox, "center_vox")
     dev = center_vox.device
     n = center_vox.shape[0]
+    stride = _rho_stride(rho_r, rho_i, keep)
     vol_r = torch.empty(d * h * w, dtype=torch.float32, device=dev)
     vol_i = torch.empty(d * h * w, dtype=torch.float32, device=dev)
     ws_bytes = lib.gstv_forward_workspace_bytes(n)
     ws = _workspace(ws_bytes, dev)
-    args = (_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i), _ptr(tile_ranges), tile_ranges.shape[0],
-            _ptr(pair_ids), _ptr(half_shape), n, tile_size, d, h, w, _ptr(vol_r), _ptr(vol_i), _ptr(ws), ws_bytes)
+    rest = (_ptr(tile_ranges), tile_ranges.shape[0], _ptr(pair_ids), _ptr(half_shape), n, tile_size, d, h, w,
+            _ptr(vol_r), _ptr(vol_i), _ptr(ws), ws_bytes)
+    head = (_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i))
     with torch.cuda.device(dev):
         if keep:
             cols = torch.empty(max(pair_ids.numel(), 1), dtype=torch.int32, device=dev)
-            st = lib.gstv_forward_keep(*args, _ptr(cols), _stream(dev))
+            st = lib.gstv_forward_keep(*head, stride, *rest, _ptr(cols), _stream(dev))
         else:
-            st = lib.gstv_forward(*args, _stream(dev))
+            st = lib.gstv_forward(*head, *rest, _stream(dev))
     _check(st, "gstv_forward_keep" if keep else "gstv_forward")
     if keep:
         return vol_r, vol_i, (ws, cols)
@@ -196,28 +216,40 @@ def forward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape
 
 
 def backward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape, grad_r, grad_i, tile_size,
-             d, h, w, kept=None):
+             d, h, w, kept=None, grad_rho=None):
     """Adjoint.  Returns (grad_center [n,3], grad_cov_inv [n,9], grad_rho_r [n], grad_rho_i [n]).
-    ``kept``: what ``forward(..., keep=True)`` of the same inputs returned."""
+    ``kept``: what ``forward(..., keep=True)`` of the same inputs returned.
+    ``grad_rho`` (with ``kept`` only): an fp32 [n, 2] array that receives the
+    density gradient interleaved (``torch.view_as_complex`` gives the complex64
+    gradient); grad_rho_r / grad_rho_i are then its columns."""
     lib = get_lib()
     _require_gpu(center_vox, "center_vox")
     dev = center_vox.device
     n = center_vox.shape[0]
     g_c = torch.empty(n, 3, dtype=torch.float32, device=dev)
     g_m = torch.empty(n, 9, dtype=torch.float32, device=dev)
-    g_rr = torch.empty(n, dtype=torch.float32, device=dev)
-    g_ri = torch.empty(n, dtype=torch.float32, device=dev)
+    stride = _rho_stride(rho_r, rho_i, kept is not None)
+    if grad_rho is not None:
+        if kept is None or stride != 2 or grad_rho.shape != (n, 2) or grad_rho.dtype != torch.float32 or \
+                not grad_rho.is_contiguous():
+            raise ValueError("grad_rho needs kept, an interleaved density and a contiguous fp32 [n, 2] array")
+        g_rr, g_ri = grad_rho[:, 0], grad_rho[:, 1]
+    else:
+        if stride != 1:
+            raise ValueError("an interleaved density needs grad_rho")
+        g_rr = torch.empty(n, dtype=torch.float32, device=dev)
+        g_ri = torch.empty(n, dtype=torch.float32, device=dev)
     ws_bytes = lib.gstv_backward_workspace_bytes(n)
     ws = _workspace(ws_bytes, dev)
-    args = (_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i), _ptr(tile_ranges), tile_ranges.shape[0],
-            _ptr(pair_ids), _ptr(half_shape), n, _ptr(grad_r), _ptr(grad_i), tile_size, d, h, w, _ptr(g_c),
-            _ptr(g_m), _ptr(g_rr), _ptr(g_ri), _ptr(ws), ws_bytes)
+    head = (_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i))
+    rest = (_ptr(tile_ranges), tile_ranges.shape[0], _ptr(pair_ids), _ptr(half_shape), n, _ptr(grad_r),
+            _ptr(grad_i), tile_size, d, h, w, _ptr(g_c), _ptr(g_m), _ptr(g_rr), _ptr(g_ri), _ptr(ws), ws_bytes)
     with torch.cuda.device(dev):
         if kept is not None:
             fws, cols = kept
-            st = lib.gstv_backward_keep(*args, _ptr(fws), _ptr(cols), _stream(dev))
+            st = lib.gstv_backward_keep(*head, stride, *rest, _ptr(fws), _ptr(cols), _stream(dev))
         else:
-            st = lib.gstv_backward(*args, _stream(dev))
+            st = lib.gstv_backward(*head, *rest, _stream(dev))
     _check(st, "gstv_backward_keep" if kept is not None else "gstv_backward")
     return g_c, g_m, g_rr, g_ri
 

@@ -402,8 +402,9 @@ __global__ void k_block_first(const int32_t* __restrict__ offsets, int n, int to
 
 // Block-wide: maps the slots [j0, j1) of block b to their Gaussians
 // (first[b] .. first[b + 1], an upper bound of the last owner).  Returns false
-// (uniformly) when more than kGenCap Gaussians span the block; callers then
-// search per slot.
+// (uniformly) when more than kGenCap Gaussians span the block, or when the
+// volume has more than 2^20 tiles (the bound of gen_pair's divisions);
+// callers then search per slot.
 __device__ bool gen_setup(GenLds& L, const float* __restrict__ center, const float* __restrict__ radius,
                           const BinGeom& geo, const int32_t* __restrict__ offsets,
                           const int32_t* __restrict__ first, int j0, int j1) {
@@ -411,7 +412,7 @@ __device__ bool gen_setup(GenLds& L, const float* __restrict__ center, const flo
   const int g0 = first[b];
   const int ng = first[b + 1] - g0 + 1;
   if (threadIdx.x == 0) L.g0 = g0;
-  if (ng > kGenCap) return false;
+  if (ng > kGenCap || geo.nt[0] * geo.nt[1] * geo.nt[2] > (1 << 20)) return false;
   const int ntw = geo.nt[2];
   const int nthw = geo.nt[1] * geo.nt[2];
   for (int i = threadIdx.x; i < ng; i += kRadixThreads) {
@@ -431,15 +432,27 @@ __device__ bool gen_setup(GenLds& L, const float* __restrict__ center, const flo
   return true;
 }
 
-// Key (tile id) and value (Gaussian id) of pair slot j (j0 <= j < j1).
+// floor(l / d) for 0 <= l < 2^20 and d >= 1 from one float reciprocal.  The
+// fraction of (l + 1/2) / d is at least 1/(2d) away from an integer; with
+// v_rcp_f32's 1-ulp reciprocal the product's relative error is below
+// 1.5 * 2^-23, an absolute error below (l + 1/2) / d * 1.5 * 2^-23 < 1/(2d)
+// for l < 2^20 (with a margin of more than 2x), so truncation gives the exact
+// quotient.
+__device__ __forceinline__ int div_small(int l, int d) {
+  return (int)(((float)l + 0.5f) * __builtin_amdgcn_rcpf((float)d));
+}
+
+// Key (tile id) and value (Gaussian id) of pair slot j (j0 <= j < j1).  A
+// slot's local index l is below its Gaussian's pair count <= num_tiles, which
+// gen_setup bounds by 2^20 for div_small.
 __device__ __forceinline__ void gen_pair(const GenLds& L, int j, int j0, const BinGeom& geo, int32_t& key,
                                          int32_t& val) {
   const int i = L.gidx[j - j0];
   const int l = j - L.off[i];
   const int n12 = L.n12[i], n2 = L.n2[i];
-  const int ld = l / n12;
+  const int ld = div_small(l, n12);
   const int rem = l - ld * n12;
-  const int lh = rem / n2;
+  const int lh = div_small(rem, n2);
   const int lw = rem - lh * n2;
   key = L.k0[i] + ld * (geo.nt[1] * geo.nt[2]) + lh * geo.nt[2] + lw;
   val = L.g0 + i;
@@ -492,6 +505,25 @@ __global__ __launch_bounds__(kRadixThreads, 4) void k_radix_hist(PairSource src,
     gen_ok = gen_setup(gen_lds, src.center, src.radius, src.geo, src.offsets, src.first, base,
                        min(base + kRadixTile, total));
   __syncthreads();
+  if constexpr (!kGen) {
+    if (base + kRadixTile <= total) {
+      // A full block: 16-byte loads, all in flight before the first atomic.
+      const int4* kv = reinterpret_cast<const int4*>(src.keys + base);
+      int4 q[kRadixRounds / 4];
+#pragma unroll
+      for (int r = 0; r < kRadixRounds / 4; ++r) q[r] = kv[r * kRadixThreads + threadIdx.x];
+#pragma unroll
+      for (int r = 0; r < kRadixRounds / 4; ++r) {
+        atomicAdd(&sh_h[(q[r].x >> shift) & mask], 1);
+        atomicAdd(&sh_h[(q[r].y >> shift) & mask], 1);
+        atomicAdd(&sh_h[(q[r].z >> shift) & mask], 1);
+        atomicAdd(&sh_h[(q[r].w >> shift) & mask], 1);
+      }
+      __syncthreads();
+      if (threadIdx.x < (1 << bits)) hist[threadIdx.x * nblocks + blockIdx.x] = sh_h[threadIdx.x];
+      return;
+    }
+  }
 #pragma unroll 4
   for (int r = 0; r < kRadixRounds; ++r) {
     const int idx = base + r * kRadixThreads + threadIdx.x;
@@ -621,6 +653,7 @@ __device__ __forceinline__ void radix_scatter(const PairSource& src, int total, 
   }
   __syncthreads();
   const int nblk = min(kRadixTile, total - block0);
+#pragma unroll 4
   for (int i = tid; i < nblk; i += kRadixThreads) {
     const int32_t key = sh.key[i];
     const int dig = (key >> shift) & mask;
@@ -698,10 +731,12 @@ static_assert(sizeof(GRec) == 64, "record must be one 64-byte line");
 // ellipsoid q <= 9: e_a = 3 hs_a sqrt((S^-1)_aa), S = sym(M), evaluated in
 // fp64 and widened by 2% + 0.01 voxel so that no voxel whose fp32 q passes the
 // cutoff is culled.  A matrix that is not positive definite, or not finite,
-// gets an infinite extent (no culling).
+// gets an infinite extent (no culling).  Gaussian g's density is
+// rho_r[g * rho_stride], rho_i[g * rho_stride] (stride 2 with rho_i = rho_r + 1
+// reads an interleaved complex64 array).
 __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ center, const float* __restrict__ cov,
                                               const float* __restrict__ rho_r, const float* __restrict__ rho_i,
-                                              const float* __restrict__ half_shape, int n,
+                                              int rho_stride, const float* __restrict__ half_shape, int n,
                                               GRec* __restrict__ rec) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= n) return;
@@ -715,8 +750,8 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ center, 
   r.c0 = center[3 * (int64_t)g + 0];
   r.c1 = center[3 * (int64_t)g + 1];
   r.c2 = center[3 * (int64_t)g + 2];
-  r.rr = rho_r[g];
-  r.ri = rho_i[g];
+  r.rr = rho_r[(int64_t)g * rho_stride];
+  r.ri = rho_i[(int64_t)g * rho_stride];
   r.a00 = kq * (mm[0] / (hs0 * hs0));
   r.a11 = kq * (mm[4] / (hs1 * hs1));
   r.a22 = kq * (mm[8] / (hs2 * hs2));
@@ -1634,13 +1669,14 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
 __global__ __launch_bounds__(256) void k_finalize(const float* __restrict__ gacc, const float* __restrict__ cov,
                                                   const float* __restrict__ half_shape, int n,
                                                   float* __restrict__ g_center, float* __restrict__ g_cov,
-                                                  float* __restrict__ g_rr, float* __restrict__ g_ri) {
+                                                  float* __restrict__ g_rr, float* __restrict__ g_ri,
+                                                  int rho_stride) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= n) return;
   const float* a = gacc + (int64_t)g * kAccStride;
   const float hs[3] = {half_shape[0], half_shape[1], half_shape[2]};
-  g_rr[g] = a[0];
-  g_ri[g] = a[1];
+  g_rr[(int64_t)g * rho_stride] = a[0];
+  g_ri[(int64_t)g * rho_stride] = a[1];
   const float m[3] = {-0.5f * a[2], -0.5f * a[3], -0.5f * a[4]};
   const float mm[3][3] = {{-0.5f * a[5], -0.5f * a[8], -0.5f * a[9]},
                           {-0.5f * a[8], -0.5f * a[6], -0.5f * a[10]},
@@ -2101,10 +2137,10 @@ int check_tile_args(const int32_t* tile_ranges, int32_t num_tiles, int32_t n, in
 }
 
 int launch_prep(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
-                const float* half_shape, int32_t n, GRec* rec, hipStream_t s) {
+                int rho_stride, const float* half_shape, int32_t n, GRec* rec, hipStream_t s) {
   if (n == 0) return GSTV_OK;
   hipLaunchKernelGGL(k_prep, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, center_vox, cov_inv, rho_r, rho_i,
-                     half_shape, n, rec);
+                     rho_stride, half_shape, n, rec);
   GSTV_LAUNCH_CHECK("prep");
   return GSTV_OK;
 }
@@ -2114,7 +2150,7 @@ int tile_grid(const VoxGeom& g) { return ((g.blocks + 7) / 8) * 8; }
 
 namespace {
 int forward_impl(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
-                 const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
+                 int rho_stride, const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
                  int32_t n, int32_t tile, int32_t d, int32_t h, int32_t w, float* vol_r, float* vol_i,
                  void* workspace, size_t workspace_bytes, uint32_t* pair_columns, void* stream) {
   int nt[3];
@@ -2128,7 +2164,7 @@ int forward_impl(const float* center_vox, const float* cov_inv, const float* rho
     return GSTV_ERR_WORKSPACE_TOO_SMALL;
   hipStream_t s = static_cast<hipStream_t>(stream);
   GRec* rec = static_cast<GRec*>(workspace);
-  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, half_shape, n, rec, s);
+  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, rho_stride, half_shape, n, rec, s);
   if (st != GSTV_OK) return st;
   const VoxGeom geo = make_vox_geom(d, h, w, tile, nt, num_tiles);
   const int slot = prof_start(0, s);
@@ -2144,18 +2180,18 @@ int gstv_forward(const float* center_vox, const float* cov_inv, const float* rho
                  const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
                  int32_t n, int32_t tile, int32_t d, int32_t h, int32_t w, float* vol_r, float* vol_i,
                  void* workspace, size_t workspace_bytes, void* stream) {
-  return forward_impl(center_vox, cov_inv, rho_r, rho_i, tile_ranges, num_tiles, pair_ids, half_shape, n, tile, d,
-                      h, w, vol_r, vol_i, workspace, workspace_bytes, nullptr, stream);
+  return forward_impl(center_vox, cov_inv, rho_r, rho_i, 1, tile_ranges, num_tiles, pair_ids, half_shape, n, tile,
+                      d, h, w, vol_r, vol_i, workspace, workspace_bytes, nullptr, stream);
 }
 
 int gstv_forward_keep(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
-                      const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
+                      int32_t rho_stride, const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
                       const float* half_shape, int32_t n, int32_t tile, int32_t d, int32_t h, int32_t w,
                       float* vol_r, float* vol_i, void* workspace, size_t workspace_bytes,
                       uint32_t* pair_columns, void* stream) {
-  if (tile <= 8 && n > 0 && pair_columns == nullptr) return GSTV_ERR_INVALID_ARGUMENT;
-  return forward_impl(center_vox, cov_inv, rho_r, rho_i, tile_ranges, num_tiles, pair_ids, half_shape, n, tile, d,
-                      h, w, vol_r, vol_i, workspace, workspace_bytes, pair_columns, stream);
+  if (rho_stride < 1 || (tile <= 8 && n > 0 && pair_columns == nullptr)) return GSTV_ERR_INVALID_ARGUMENT;
+  return forward_impl(center_vox, cov_inv, rho_r, rho_i, rho_stride, tile_ranges, num_tiles, pair_ids, half_shape,
+                      n, tile, d, h, w, vol_r, vol_i, workspace, workspace_bytes, pair_columns, stream);
 }
 
 size_t gstv_backward_workspace_bytes(int32_t n) {
@@ -2168,7 +2204,7 @@ size_t gstv_backward_workspace_bytes(int32_t n) {
 
 namespace {
 int backward_impl(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
-                  const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
+                  int rho_stride, const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
                   int32_t n, const float* grad_r, const float* grad_i, int32_t tile, int32_t d, int32_t h,
                   int32_t w, float* grad_center, float* grad_cov_inv, float* grad_rho_r, float* grad_rho_i,
                   void* workspace, size_t workspace_bytes, const void* forward_workspace,
@@ -2190,7 +2226,7 @@ int backward_impl(const float* center_vox, const float* cov_inv, const float* rh
   if (forward_workspace != nullptr) {
     rec = static_cast<GRec*>(const_cast<void*>(forward_workspace));  // records of the same call's forward
   } else {
-    st = launch_prep(center_vox, cov_inv, rho_r, rho_i, half_shape, n, rec, s);
+    st = launch_prep(center_vox, cov_inv, rho_r, rho_i, rho_stride, half_shape, n, rec, s);
     if (st != GSTV_OK) return st;
   }
   const int64_t nacc = (int64_t)n * kAccStride;
@@ -2213,7 +2249,7 @@ int backward_impl(const float* center_vox, const float* cov_inv, const float* rh
   GSTV_LAUNCH_CHECK("backward");
   prof_stop(1, slot, s);
   hipLaunchKernelGGL(k_finalize, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, (const float*)gacc, cov_inv,
-                     half_shape, n, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i);
+                     half_shape, n, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, rho_stride);
   GSTV_LAUNCH_CHECK("finalize");
   return GSTV_OK;
 }
@@ -2224,22 +2260,22 @@ int gstv_backward(const float* center_vox, const float* cov_inv, const float* rh
                   int32_t n, const float* grad_r, const float* grad_i, int32_t tile, int32_t d, int32_t h,
                   int32_t w, float* grad_center, float* grad_cov_inv, float* grad_rho_r, float* grad_rho_i,
                   void* workspace, size_t workspace_bytes, void* stream) {
-  return backward_impl(center_vox, cov_inv, rho_r, rho_i, tile_ranges, num_tiles, pair_ids, half_shape, n, grad_r,
-                       grad_i, tile, d, h, w, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, workspace,
+  return backward_impl(center_vox, cov_inv, rho_r, rho_i, 1, tile_ranges, num_tiles, pair_ids, half_shape, n,
+                       grad_r, grad_i, tile, d, h, w, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, workspace,
                        workspace_bytes, nullptr, nullptr, stream);
 }
 
 int gstv_backward_keep(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
-                       const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
+                       int32_t rho_stride, const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
                        const float* half_shape, int32_t n, const float* grad_r, const float* grad_i, int32_t tile,
                        int32_t d, int32_t h, int32_t w, float* grad_center, float* grad_cov_inv,
                        float* grad_rho_r, float* grad_rho_i, void* workspace, size_t workspace_bytes,
                        const void* forward_workspace, const uint32_t* pair_columns, void* stream) {
-  if (n > 0 && (forward_workspace == nullptr || (tile <= 8 && pair_columns == nullptr)))
+  if (rho_stride < 1 || (n > 0 && (forward_workspace == nullptr || (tile <= 8 && pair_columns == nullptr))))
     return GSTV_ERR_INVALID_ARGUMENT;
-  return backward_impl(center_vox, cov_inv, rho_r, rho_i, tile_ranges, num_tiles, pair_ids, half_shape, n, grad_r,
-                       grad_i, tile, d, h, w, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, workspace,
-                       workspace_bytes, forward_workspace, pair_columns, stream);
+  return backward_impl(center_vox, cov_inv, rho_r, rho_i, rho_stride, tile_ranges, num_tiles, pair_ids, half_shape,
+                       n, grad_r, grad_i, tile, d, h, w, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i,
+                       workspace, workspace_bytes, forward_workspace, pair_columns, stream);
 }
 
 int gstv_count_evaluations(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
@@ -2259,7 +2295,7 @@ int gstv_count_evaluations(const float* center_vox, const float* cov_inv, const 
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, reinterpret_cast<int32_t*>(counts), (int64_t)6, 0);
   GSTV_LAUNCH_CHECK("count_clear");
   GRec* rec = static_cast<GRec*>(workspace);
-  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, half_shape, n, rec, s);
+  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, 1, half_shape, n, rec, s);
   if (st != GSTV_OK) return st;
   const VoxGeom geo = make_vox_geom(d, h, w, tile, nt, num_tiles);
   hipLaunchKernelGGL(k_count, dim3(tile_grid(geo)), dim3(kTileThreads), 0, s, (const GRec*)rec, tile_ranges, pair_ids,

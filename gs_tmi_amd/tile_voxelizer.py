@@ -110,40 +110,51 @@ class _TileVoxelizeFusedHIP(torch.autograd.Function):
     kernels fewer per step."""
 
     @staticmethod
-    def forward(ctx, positions, scales, rotations, density_r, density_i, volume_shape, tile_size):
+    def forward(ctx, positions, scales, rotations, density, volume_shape, tile_size):
         _load_hip_ext(strict=True)
         d_size, h_size, w_size = volume_shape
         scales_c = scales.detach().contiguous().float()
         rot_c = rotations.detach().contiguous().float()
         center, radius, cov, half_shape = _native.prologue(positions, scales_c, rot_c, volume_shape)
         tile_ranges, pair_ids = build_tile_pairs(center, radius, volume_shape, tile_size)
-        rr = density_r.detach().contiguous().float()
-        ri = density_i.detach().contiguous().float()
+        # The kernels read the complex64 density interleaved, in place: no
+        # real / imag copies here, and one complex gradient in backward.
+        dens = density.detach().reshape(-1).resolve_conj()
+        if dens.dtype != torch.complex64:
+            dens = dens.to(torch.complex64)
+        dv = torch.view_as_real(dens.contiguous())
         # The forward keeps its Gaussian records and per-pair column rows for
         # the adjoint of this call, which then rebuilds neither.
-        vol_r, vol_i, ctx.kept = _native.forward(center, cov, rr, ri, tile_ranges, pair_ids, half_shape, tile_size,
-                                                 d_size, h_size, w_size, keep=True)
-        ctx.save_for_backward(scales_c, rot_c, center, cov, rr, ri, tile_ranges, pair_ids, half_shape)
+        vol_r, vol_i, ctx.kept = _native.forward(center, cov, dv[:, 0], dv[:, 1], tile_ranges, pair_ids,
+                                                 half_shape, tile_size, d_size, h_size, w_size, keep=True)
+        ctx.save_for_backward(scales_c, rot_c, center, cov, dv, tile_ranges, pair_ids, half_shape)
+        ctx.density_dtype = density.dtype
+        ctx.density_shape = density.shape
         ctx.tile_size = tile_size
         ctx.dims = (d_size, h_size, w_size)
         return vol_r.view(d_size, h_size, w_size), vol_i.view(d_size, h_size, w_size)
 
     @staticmethod
     def backward(ctx, grad_vol_r, grad_vol_i):
-        scales_c, rot_c, center, cov, rr, ri, tile_ranges, pair_ids, half_shape = ctx.saved_tensors
+        scales_c, rot_c, center, cov, dv, tile_ranges, pair_ids, half_shape = ctx.saved_tensors
         d_size, h_size, w_size = ctx.dims
         shape = (d_size, h_size, w_size)
         if grad_vol_r is None:
             grad_vol_r = torch.zeros(shape, dtype=torch.float32, device=center.device)
         if grad_vol_i is None:
             grad_vol_i = torch.zeros(shape, dtype=torch.float32, device=center.device)
-        g_c, g_m, g_rr, g_ri = _native.backward(
-            center, cov, rr, ri, tile_ranges, pair_ids, half_shape,
+        g_dv = torch.empty_like(dv)
+        g_c, g_m, _, _ = _native.backward(
+            center, cov, dv[:, 0], dv[:, 1], tile_ranges, pair_ids, half_shape,
             grad_vol_r.contiguous().float().view(-1), grad_vol_i.contiguous().float().view(-1),
-            ctx.tile_size, d_size, h_size, w_size, kept=ctx.kept)
+            ctx.tile_size, d_size, h_size, w_size, kept=ctx.kept, grad_rho=g_dv)
         ctx.kept = None
         g_p, g_s, g_q = _native.epilogue(scales_c, rot_c, g_c, g_m, shape)
-        return g_p, g_s, g_q, g_rr, g_ri, None, None
+        if ctx.density_dtype.is_complex:
+            g_d = torch.view_as_complex(g_dv).to(ctx.density_dtype)
+        else:
+            g_d = g_dv[:, 0].to(ctx.density_dtype)
+        return g_p, g_s, g_q, g_d.reshape(ctx.density_shape), None, None
 
 
 def voxel_space_params(positions, scales, rotations, volume_shape):
@@ -195,8 +206,8 @@ class TileVoxelizer(nn.Module):
     def forward(self, positions: torch.Tensor, scales: torch.Tensor, rotations: torch.Tensor,
                 density: torch.Tensor, chunk_size: int = 4096):
         del chunk_size
-        return _TileVoxelizeFusedHIP.apply(positions, scales, rotations, density.real, density.imag,
-                                           self.volume_shape, self.tile_size)
+        return _TileVoxelizeFusedHIP.apply(positions, scales, rotations, density, self.volume_shape,
+                                           self.tile_size)
 
     def forward_unfused(self, positions, scales, rotations, density):
         """The same call composed as the reference composes it: torch prologue,

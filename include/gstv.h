@@ -152,9 +152,13 @@ int gstv_backward(const float* center_vox, const float* cov_inv,
  * gstv_backward_keep is gstv_backward given that workspace and pair_columns,
  * for exactly the inputs of that forward; it skips rebuilding the records and
  * the column bounds.  Results equal gstv_backward's up to the order of fp32
- * atomic additions. */
+ * atomic additions.
+ * rho_stride (>= 1) is the distance, in floats, between consecutive Gaussians'
+ * values in rho_r / rho_i and in grad_rho_r / grad_rho_i: 1 for planar arrays,
+ * 2 with rho_i = rho_r + 1 (and grad_rho_i = grad_rho_r + 1) for an
+ * interleaved complex64 density and its gradient. */
 int gstv_forward_keep(const float* center_vox, const float* cov_inv,
-                      const float* rho_r, const float* rho_i,
+                      const float* rho_r, const float* rho_i, int32_t rho_stride,
                       const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
                       const float* half_shape, int32_t n,
                       int32_t tile, int32_t d, int32_t h, int32_t w,
@@ -163,7 +167,7 @@ int gstv_forward_keep(const float* center_vox, const float* cov_inv,
                       uint32_t* pair_columns, void* stream);
 
 int gstv_backward_keep(const float* center_vox, const float* cov_inv,
-                       const float* rho_r, const float* rho_i,
+                       const float* rho_r, const float* rho_i, int32_t rho_stride,
                        const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
                        const float* half_shape, int32_t n,
                        const float* grad_r, const float* grad_i,

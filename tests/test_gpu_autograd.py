@@ -175,3 +175,35 @@ def test_fused_call_matches_unfused_composition(name):
         e = cases.rel_l2(a.cpu().numpy(), b.cpu().numpy())
         print(f"  {nm}: fused vs unfused rel L2 {e:.2e}")
         assert e < 2e-6, (nm, e)
+
+
+def test_fused_call_density_layouts():
+    """The fused call reads the complex64 density in place; other complex
+    dtypes and shapes are converted, and their gradients come back in the
+    caller's dtype and shape."""
+    from gs_tmi_amd import TileVoxelizer
+
+    c = cases.edge_case("partial_tiles")
+    shape, tile = c["shape"], c["tile"]
+    _announce(f"fused call density layouts shape={shape} tile={tile}")
+    vox = TileVoxelizer(shape, tile_size=tile, device=DEV)
+    rng = np.random.default_rng(5)
+    g_r = torch.from_numpy(rng.normal(size=shape).astype(np.float32)).to(DEV)
+    g_i = torch.from_numpy(rng.normal(size=shape).astype(np.float32)).to(DEV)
+    pos = torch.from_numpy(c["pos"]).to(DEV)
+    scales = torch.from_numpy(c["scales"]).to(DEV)
+    rot = torch.from_numpy(c["rot"]).to(DEV)
+    z64 = torch.complex(torch.from_numpy(c["rr"]), torch.from_numpy(c["ri"])).to(DEV)
+    out = {}
+    for name, dens in (("c64", z64), ("c128", z64.to(torch.complex128)), ("column", z64.view(-1, 1))):
+        dens = dens.clone().requires_grad_()
+        vr, vi = vox(pos, scales, rot, dens)
+        ((vr * g_r).sum() + (vi * g_i).sum()).backward()
+        assert dens.grad.dtype == dens.dtype and dens.grad.shape == dens.shape, name
+        out[name] = (vr.detach(), vi.detach(), dens.grad.reshape(-1).to(torch.complex64))
+    for name in ("c128", "column"):
+        assert torch.equal(out[name][0], out["c64"][0]) and torch.equal(out[name][1], out["c64"][1])
+        e = cases.rel_l2(torch.view_as_real(out[name][2]).cpu().numpy(),
+                         torch.view_as_real(out["c64"][2]).cpu().numpy())
+        print(f"  density grad {name} vs complex64: rel L2 {e:.2e}")
+        assert e < 5e-6, (name, e)

@@ -190,6 +190,17 @@ def test_keep_path_matches_plain_calls(name, make):
     tol = 2e-5 if name in CASE_TOL else 5e-6
     for a, b in zip(plain, keep):
         assert cases.rel_l2(b.cpu().numpy(), a.cpu().numpy()) < tol
+    # The same pair of calls on an interleaved (complex64) density and gradient.
+    dv = torch.stack([rr, ri], 1).contiguous()
+    iargs = (center, cov, dv[:, 0], dv[:, 1], ranges, pids, hs)
+    ivol_r, ivol_i, ikept = _native.forward(*iargs, c["tile"], d, h, w, keep=True)
+    assert torch.equal(vol_r, ivol_r) and torch.equal(vol_i, ivol_i)
+    g_dv = torch.empty_like(dv)
+    inter = _native.backward(*iargs, g_r, g_i, c["tile"], d, h, w, kept=ikept, grad_rho=g_dv)
+    torch.cuda.synchronize()
+    assert inter[2].data_ptr() == g_dv.data_ptr() and inter[3].data_ptr() == g_dv.data_ptr() + 4
+    for a, b in zip(plain, inter):
+        assert cases.rel_l2(b.cpu().numpy(), a.cpu().numpy()) < tol
 
 
 def test_empty_and_degenerate_inputs():
