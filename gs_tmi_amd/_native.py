@@ -43,12 +43,13 @@ PROTOTYPES = {
     "gstv_backward": (ctypes.c_int,
                       [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32,
                        _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _sz, _c_p]),
+    "gstv_keep_workspace_bytes": (ctypes.c_size_t, [_i32]),
+    "gstv_keep_prepare": (ctypes.c_int, [_c_p, _c_p, _c_p, _c_p, _i32, _c_p, _i32, _c_p, _sz, _c_p]),
     "gstv_forward_keep": (ctypes.c_int,
-                          [_c_p, _c_p, _c_p, _c_p, _i32, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32,
-                           _c_p, _c_p, _c_p, _sz, _c_p, _c_p]),
+                          [_c_p, _sz, _c_p, _i32, _c_p, _i32, _i32, _i32, _i32, _i32, _c_p, _c_p, _c_p, _c_p]),
     "gstv_backward_keep": (ctypes.c_int,
-                           [_c_p, _c_p, _c_p, _c_p, _i32, _c_p, _i32, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32,
-                            _i32, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _sz, _c_p, _c_p, _c_p]),
+                           [_c_p, _sz, _c_p, _c_p, _c_p, _i32, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32, _i32,
+                            _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p]),
     "gstv_count_evaluations": (ctypes.c_int,
                                [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32,
                                 _c_p, _c_p, _sz, _c_p]),
@@ -184,41 +185,80 @@ def _rho_stride(rho_r, rho_i, keep):
     return st
 
 
-def forward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape, tile_size, d, h, w,
-            keep=False):
-    """Planar forward.  Returns (vol_r, vol_i), each fp32 [d*h*w]; with
-    ``keep``, (vol_r, vol_i, kept) where ``kept`` = (workspace, pair_columns)
-    is what ``backward(..., kept=kept)`` of the same inputs reuses.  With
-    ``keep``, rho_r / rho_i may also be the two columns of one interleaved
-    [n, 2] fp32 array (``torch.view_as_real`` of a complex64 density)."""
+class Kept:
+    """State the keep entry points hand from the forward to the adjoint of one
+    call: the keep workspace (Gaussian records and moment accumulator) and the
+    per-pair column words.  ``fresh``: the accumulator is zero (a second
+    adjoint prepares it again)."""
+
+    __slots__ = ("ws", "cols", "fresh", "stride")
+
+    def __init__(self, ws, stride):
+        self.ws = ws
+        self.cols = None
+        self.fresh = True
+        self.stride = stride
+
+
+def keep_prepare(center_vox, cov_inv, rho_r, rho_i, half_shape, kept=None):
+    """Queues gstv_keep_prepare: Gaussian records and a zeroed accumulator in a
+    keep workspace (a new one, or ``kept``'s again).  rho_r / rho_i may be the
+    columns of an interleaved [n, 2] fp32 array.  Returns the ``Kept``."""
     lib = get_lib()
     _require_gpu(center_vox, "center_vox")
     dev = center_vox.device
     n = center_vox.shape[0]
-    stride = _rho_stride(rho_r, rho_i, keep)
+    stride = _rho_stride(rho_r, rho_i, True)
+    if kept is None:
+        kept = Kept(_workspace(lib.gstv_keep_workspace_bytes(n), dev), stride)
+    elif kept.stride != stride:
+        raise ValueError("density layout differs from the kept forward's")
+    with torch.cuda.device(dev):
+        st = lib.gstv_keep_prepare(_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i), stride,
+                                   _ptr(half_shape), n, _ptr(kept.ws), kept.ws.numel(), _stream(dev))
+    _check(st, "gstv_keep_prepare")
+    kept.fresh = True
+    return kept
+
+
+def forward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape, tile_size, d, h, w,
+            keep=False):
+    """Planar forward.  Returns (vol_r, vol_i), each fp32 [d*h*w].  With
+    ``keep`` (True, or a ``Kept`` from ``keep_prepare`` of the same inputs),
+    returns (vol_r, vol_i, kept), where ``kept`` is what
+    ``backward(..., kept=kept)`` of the same inputs reuses; rho_r / rho_i may
+    then also be the two columns of one interleaved [n, 2] fp32 array
+    (``torch.view_as_real`` of a complex64 density)."""
+    lib = get_lib()
+    _require_gpu(center_vox, "center_vox")
+    dev = center_vox.device
+    n = center_vox.shape[0]
+    _rho_stride(rho_r, rho_i, keep is not False)
     vol_r = torch.empty(d * h * w, dtype=torch.float32, device=dev)
     vol_i = torch.empty(d * h * w, dtype=torch.float32, device=dev)
+    if keep is not False:
+        kept = keep_prepare(center_vox, cov_inv, rho_r, rho_i, half_shape) if keep is True else keep
+        kept.cols = torch.empty(max(pair_ids.numel(), 1), dtype=torch.int32, device=dev)
+        with torch.cuda.device(dev):
+            st = lib.gstv_forward_keep(_ptr(kept.ws), kept.ws.numel(), _ptr(tile_ranges), tile_ranges.shape[0],
+                                       _ptr(pair_ids), n, tile_size, d, h, w, _ptr(vol_r), _ptr(vol_i),
+                                       _ptr(kept.cols), _stream(dev))
+        _check(st, "gstv_forward_keep")
+        return vol_r, vol_i, kept
     ws_bytes = lib.gstv_forward_workspace_bytes(n)
     ws = _workspace(ws_bytes, dev)
-    rest = (_ptr(tile_ranges), tile_ranges.shape[0], _ptr(pair_ids), _ptr(half_shape), n, tile_size, d, h, w,
-            _ptr(vol_r), _ptr(vol_i), _ptr(ws), ws_bytes)
-    head = (_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i))
     with torch.cuda.device(dev):
-        if keep:
-            cols = torch.empty(max(pair_ids.numel(), 1), dtype=torch.int32, device=dev)
-            st = lib.gstv_forward_keep(*head, stride, *rest, _ptr(cols), _stream(dev))
-        else:
-            st = lib.gstv_forward(*head, *rest, _stream(dev))
-    _check(st, "gstv_forward_keep" if keep else "gstv_forward")
-    if keep:
-        return vol_r, vol_i, (ws, cols)
+        st = lib.gstv_forward(_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i), _ptr(tile_ranges),
+                              tile_ranges.shape[0], _ptr(pair_ids), _ptr(half_shape), n, tile_size, d, h, w,
+                              _ptr(vol_r), _ptr(vol_i), _ptr(ws), ws_bytes, _stream(dev))
+    _check(st, "gstv_forward")
     return vol_r, vol_i
 
 
 def backward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape, grad_r, grad_i, tile_size,
              d, h, w, kept=None, grad_rho=None):
     """Adjoint.  Returns (grad_center [n,3], grad_cov_inv [n,9], grad_rho_r [n], grad_rho_i [n]).
-    ``kept``: what ``forward(..., keep=True)`` of the same inputs returned.
+    ``kept``: what ``forward(..., keep=...)`` of the same inputs returned.
     ``grad_rho`` (with ``kept`` only): an fp32 [n, 2] array that receives the
     density gradient interleaved (``torch.view_as_complex`` gives the complex64
     gradient); grad_rho_r / grad_rho_i are then its columns."""
@@ -239,18 +279,25 @@ def backward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shap
             raise ValueError("an interleaved density needs grad_rho")
         g_rr = torch.empty(n, dtype=torch.float32, device=dev)
         g_ri = torch.empty(n, dtype=torch.float32, device=dev)
+    if kept is not None:
+        if not kept.fresh:
+            keep_prepare(center_vox, cov_inv, rho_r, rho_i, half_shape, kept)
+        with torch.cuda.device(dev):
+            st = lib.gstv_backward_keep(_ptr(kept.ws), kept.ws.numel(), _ptr(cov_inv), _ptr(half_shape),
+                                        _ptr(tile_ranges), tile_ranges.shape[0], _ptr(pair_ids), n, _ptr(grad_r),
+                                        _ptr(grad_i), tile_size, d, h, w, _ptr(g_c), _ptr(g_m), _ptr(g_rr),
+                                        _ptr(g_ri), stride, _ptr(kept.cols), _stream(dev))
+        kept.fresh = False
+        _check(st, "gstv_backward_keep")
+        return g_c, g_m, g_rr, g_ri
     ws_bytes = lib.gstv_backward_workspace_bytes(n)
     ws = _workspace(ws_bytes, dev)
-    head = (_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i))
-    rest = (_ptr(tile_ranges), tile_ranges.shape[0], _ptr(pair_ids), _ptr(half_shape), n, _ptr(grad_r),
-            _ptr(grad_i), tile_size, d, h, w, _ptr(g_c), _ptr(g_m), _ptr(g_rr), _ptr(g_ri), _ptr(ws), ws_bytes)
     with torch.cuda.device(dev):
-        if kept is not None:
-            fws, cols = kept
-            st = lib.gstv_backward_keep(*head, stride, *rest, _ptr(fws), _ptr(cols), _stream(dev))
-        else:
-            st = lib.gstv_backward(*head, *rest, _stream(dev))
-    _check(st, "gstv_backward_keep" if kept is not None else "gstv_backward")
+        st = lib.gstv_backward(_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i), _ptr(tile_ranges),
+                               tile_ranges.shape[0], _ptr(pair_ids), _ptr(half_shape), n, _ptr(grad_r),
+                               _ptr(grad_i), tile_size, d, h, w, _ptr(g_c), _ptr(g_m), _ptr(g_rr), _ptr(g_ri),
+                               _ptr(ws), ws_bytes, _stream(dev))
+    _check(st, "gstv_backward")
     return g_c, g_m, g_rr, g_ri
 
 

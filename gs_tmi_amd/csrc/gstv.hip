@@ -375,9 +375,12 @@ struct GenLds {
   int32_t k0[kGenCap];
   int32_t n12[kGenCap];
   int32_t n2[kGenCap];
-  uint16_t gidx[kRadixTile];
+  uint16_t gidx[kRadixTile];  // 16-byte aligned: follows 4 * kGenCap int32
+  uint32_t wmax[kRadixThreads / 64];
   int g0, ng;
 };
+static_assert((4 * kGenCap * sizeof(int32_t)) % 16 == 0, "gidx must be 16-byte aligned");
+static_assert(kRadixTile == kRadixThreads * 16, "the owner scan gives each thread 16 slots");
 
 struct ScatterLds {
   int32_t key[kRadixTile];
@@ -415,6 +418,10 @@ __device__ bool gen_setup(GenLds& L, const float* __restrict__ center, const flo
   if (ng > kGenCap || geo.nt[0] * geo.nt[1] * geo.nt[2] > (1 << 20)) return false;
   const int ntw = geo.nt[2];
   const int nthw = geo.nt[1] * geo.nt[2];
+  uint4* g16 = reinterpret_cast<uint4*>(L.gidx);
+  const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+  g16[2 * threadIdx.x] = zero4;
+  g16[2 * threadIdx.x + 1] = zero4;
   for (int i = threadIdx.x; i < ng; i += kRadixThreads) {
     int tmin[3], tcnt[3];
     gauss_tile_box(center, radius, g0 + i, geo, tmin, tcnt);
@@ -424,10 +431,48 @@ __device__ bool gen_setup(GenLds& L, const float* __restrict__ center, const flo
     L.n2[i] = tcnt[2];
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < ng; i += kRadixThreads) {
-    const int s_end = min(i + 1 < ng ? L.off[i + 1] : offsets[g0 + ng], j1);
-    for (int sl = max(L.off[i], j0); sl < s_end; ++sl) L.gidx[sl - j0] = (uint16_t)i;
+  // Slot -> Gaussian map.  The owner of a slot is the last Gaussian starting
+  // at or before it: each Gaussian with pairs that starts inside the block
+  // marks its first slot (Gaussian 0, first[b], owns slot 0 through the zero
+  // fill), and an inclusive max-scan over the slots spreads the marks.
+  for (int i = threadIdx.x + 1; i < ng; i += kRadixThreads) {
+    const int o = L.off[i];
+    const int next = i + 1 < ng ? L.off[i + 1] : offsets[g0 + ng];
+    if (o < next && o < j1) L.gidx[o - j0] = (uint16_t)i;  // o > j0: slot j0 belongs to Gaussian 0
   }
+  __syncthreads();
+  // Each thread scans its 16 consecutive slots (two per 32-bit word), then the
+  // running maxima of the threads are combined across the wave and the waves.
+  const uint4 a = g16[2 * threadIdx.x], c = g16[2 * threadIdx.x + 1];
+  uint32_t v[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
+  uint32_t run = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const uint32_t lo = max(v[k] & 0xffffu, run);
+    const uint32_t hi = max(v[k] >> 16, lo);
+    v[k] = lo | (hi << 16);
+    run = hi;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t incl = run;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl = max(incl, o);
+  }
+  uint32_t excl = __shfl_up(incl, 1, 64);
+  if (lane == 0) excl = 0;
+  if (lane == 63) L.wmax[wid] = incl;
+  __syncthreads();
+  for (int w = 0; w < wid; ++w) excl = max(excl, L.wmax[w]);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    // Both halves are at most 0xffff: a per-half max of v[k] and excl.
+    const uint32_t lo = max(v[k] & 0xffffu, excl), hi = max(v[k] >> 16, excl);
+    v[k] = lo | (hi << 16);
+  }
+  g16[2 * threadIdx.x] = make_uint4(v[0], v[1], v[2], v[3]);
+  g16[2 * threadIdx.x + 1] = make_uint4(v[4], v[5], v[6], v[7]);
   __syncthreads();
   return true;
 }
@@ -2149,10 +2194,68 @@ int tile_grid(const VoxGeom& g) { return ((g.blocks + 7) / 8) * 8; }
 }  // namespace
 
 namespace {
-int forward_impl(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
-                 int rho_stride, const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
+// Forward tile kernel on built records.  pair_columns: written when non-null
+// and the tile is at most 8 voxels per axis.
+void launch_forward(const GRec* rec, const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
+                    int32_t tile, int32_t d, int32_t h, int32_t w, const int nt[3], float* vol_r, float* vol_i,
+                    uint32_t* pair_columns, hipStream_t s) {
+  const VoxGeom geo = make_vox_geom(d, h, w, tile, nt, num_tiles);
+  const int slot = prof_start(0, s);
+  hipLaunchKernelGGL(k_forward, dim3(tile_grid(geo)), dim3(kTileThreads), 0, s, rec, tile_ranges, pair_ids, geo,
+                     vol_r, vol_i, geo.nsb <= 2 ? pair_columns : nullptr);
+  prof_stop(0, slot, s);
+}
+
+void launch_clear_acc(float* gacc, int32_t n, hipStream_t s) {
+  const int64_t nacc = (int64_t)n * kAccStride;
+  hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)ceil_div_l(nacc, 256)), dim3(256), 0, s, gacc, nacc, 0.0f);
+}
+
+// Backward tile kernel into a zeroed moment accumulator, then the moments ->
+// gradients pass.  pair_columns: the forward's column words (tiles of at most
+// 8 voxels per axis), or null to compute the column bounds here.
+void launch_backward(const GRec* rec, float* gacc, const float* cov_inv, const float* half_shape,
+                     const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, int32_t n,
+                     const float* grad_r, const float* grad_i, int32_t tile, int32_t d, int32_t h, int32_t w,
+                     const int nt[3], float* grad_center, float* grad_cov_inv, float* grad_rho_r, float* grad_rho_i,
+                     int rho_stride, const uint32_t* pair_columns, hipStream_t s) {
+  const VoxGeom geo = make_vox_geom(d, h, w, tile, nt, num_tiles);
+  const int slot = prof_start(1, s);
+  VoxGeom geo_b = geo;  // the backward runs one block per tile
+  geo_b.chunks = 1;
+  geo_b.blocks = num_tiles;
+  const int tp = geo.nsb * kSub;
+  const size_t lds_grad = (size_t)tp * grad_zs(tp) * sizeof(float2);
+  const uint32_t* cols = geo.nsb <= 2 ? pair_columns : nullptr;
+  if (tile == 8)
+    hipLaunchKernelGGL(k_backward<8>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, rec, tile_ranges,
+                       pair_ids, geo_b, grad_r, grad_i, gacc, cols);
+  else
+    hipLaunchKernelGGL(k_backward<0>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, rec, tile_ranges,
+                       pair_ids, geo_b, grad_r, grad_i, gacc, cols);
+  prof_stop(1, slot, s);
+  hipLaunchKernelGGL(k_finalize, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, (const float*)gacc, cov_inv,
+                     half_shape, n, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, rho_stride);
+}
+
+// Keep workspace: the records, then the 48-byte moment rows.
+struct KeepLayout {
+  GRec* rec;
+  float* gacc;
+};
+size_t keep_layout(void* base, int32_t n, KeepLayout* out) {
+  Carve cv(base);
+  GRec* rec = cv.take<GRec>((size_t)n);
+  float* gacc = cv.take<float>((size_t)n * kAccStride);
+  if (out != nullptr) *out = KeepLayout{rec, gacc};
+  return align_up(cv.off, 256);
+}
+}  // namespace
+
+int gstv_forward(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
+                 const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
                  int32_t n, int32_t tile, int32_t d, int32_t h, int32_t w, float* vol_r, float* vol_i,
-                 void* workspace, size_t workspace_bytes, uint32_t* pair_columns, void* stream) {
+                 void* workspace, size_t workspace_bytes, void* stream) {
   int nt[3];
   int st = check_tile_args(tile_ranges, num_tiles, n, tile, d, h, w, nt);
   if (st != GSTV_OK) return st;
@@ -2164,51 +2267,23 @@ int forward_impl(const float* center_vox, const float* cov_inv, const float* rho
     return GSTV_ERR_WORKSPACE_TOO_SMALL;
   hipStream_t s = static_cast<hipStream_t>(stream);
   GRec* rec = static_cast<GRec*>(workspace);
-  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, rho_stride, half_shape, n, rec, s);
+  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, 1, half_shape, n, rec, s);
   if (st != GSTV_OK) return st;
-  const VoxGeom geo = make_vox_geom(d, h, w, tile, nt, num_tiles);
-  const int slot = prof_start(0, s);
-  hipLaunchKernelGGL(k_forward, dim3(tile_grid(geo)), dim3(kTileThreads), 0, s, (const GRec*)rec, tile_ranges,
-                     pair_ids, geo, vol_r, vol_i, geo.nsb <= 2 ? pair_columns : nullptr);
+  launch_forward(rec, tile_ranges, num_tiles, pair_ids, tile, d, h, w, nt, vol_r, vol_i, nullptr, s);
   GSTV_LAUNCH_CHECK("forward");
-  prof_stop(0, slot, s);
   return GSTV_OK;
-}
-}  // namespace
-
-int gstv_forward(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
-                 const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
-                 int32_t n, int32_t tile, int32_t d, int32_t h, int32_t w, float* vol_r, float* vol_i,
-                 void* workspace, size_t workspace_bytes, void* stream) {
-  return forward_impl(center_vox, cov_inv, rho_r, rho_i, 1, tile_ranges, num_tiles, pair_ids, half_shape, n, tile,
-                      d, h, w, vol_r, vol_i, workspace, workspace_bytes, nullptr, stream);
-}
-
-int gstv_forward_keep(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
-                      int32_t rho_stride, const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
-                      const float* half_shape, int32_t n, int32_t tile, int32_t d, int32_t h, int32_t w,
-                      float* vol_r, float* vol_i, void* workspace, size_t workspace_bytes,
-                      uint32_t* pair_columns, void* stream) {
-  if (rho_stride < 1 || (tile <= 8 && n > 0 && pair_columns == nullptr)) return GSTV_ERR_INVALID_ARGUMENT;
-  return forward_impl(center_vox, cov_inv, rho_r, rho_i, rho_stride, tile_ranges, num_tiles, pair_ids, half_shape,
-                      n, tile, d, h, w, vol_r, vol_i, workspace, workspace_bytes, pair_columns, stream);
 }
 
 size_t gstv_backward_workspace_bytes(int32_t n) {
   if (n < 0) return 0;
-  Carve cv(nullptr);
-  cv.take<GRec>((size_t)n);
-  cv.take<float>((size_t)n * kAccStride);
-  return align_up(cv.off, 256);
+  return keep_layout(nullptr, n, nullptr);
 }
 
-namespace {
-int backward_impl(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
-                  int rho_stride, const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
+int gstv_backward(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
+                  const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
                   int32_t n, const float* grad_r, const float* grad_i, int32_t tile, int32_t d, int32_t h,
                   int32_t w, float* grad_center, float* grad_cov_inv, float* grad_rho_r, float* grad_rho_i,
-                  void* workspace, size_t workspace_bytes, const void* forward_workspace,
-                  const uint32_t* pair_columns, void* stream) {
+                  void* workspace, size_t workspace_bytes, void* stream) {
   int nt[3];
   int st = check_tile_args(tile_ranges, num_tiles, n, tile, d, h, w, nt);
   if (st != GSTV_OK) return st;
@@ -2220,62 +2295,83 @@ int backward_impl(const float* center_vox, const float* cov_inv, const float* rh
   if (workspace == nullptr || workspace_bytes < gstv_backward_workspace_bytes(n))
     return GSTV_ERR_WORKSPACE_TOO_SMALL;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  Carve cv(workspace);
-  GRec* rec = cv.take<GRec>((size_t)n);
-  float* gacc = cv.take<float>((size_t)n * kAccStride);
-  if (forward_workspace != nullptr) {
-    rec = static_cast<GRec*>(const_cast<void*>(forward_workspace));  // records of the same call's forward
-  } else {
-    st = launch_prep(center_vox, cov_inv, rho_r, rho_i, rho_stride, half_shape, n, rec, s);
-    if (st != GSTV_OK) return st;
-  }
-  const int64_t nacc = (int64_t)n * kAccStride;
-  hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)ceil_div_l(nacc, 256)), dim3(256), 0, s, gacc, nacc, 0.0f);
-  GSTV_LAUNCH_CHECK("grad_clear");
-  const VoxGeom geo = make_vox_geom(d, h, w, tile, nt, num_tiles);
-  const int slot = prof_start(1, s);
-  VoxGeom geo_b = geo;  // the backward runs one block per tile
-  geo_b.chunks = 1;
-  geo_b.blocks = num_tiles;
-  const int tp = geo.nsb * kSub;
-  const size_t lds_grad = (size_t)tp * grad_zs(tp) * sizeof(float2);
-  const uint32_t* cols = geo.nsb <= 2 ? pair_columns : nullptr;
-  if (tile == 8)
-    hipLaunchKernelGGL(k_backward<8>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, (const GRec*)rec,
-                       tile_ranges, pair_ids, geo_b, grad_r, grad_i, gacc, cols);
-  else
-    hipLaunchKernelGGL(k_backward<0>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, (const GRec*)rec,
-                       tile_ranges, pair_ids, geo_b, grad_r, grad_i, gacc, cols);
+  KeepLayout ws;
+  keep_layout(workspace, n, &ws);
+  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, 1, half_shape, n, ws.rec, s);
+  if (st != GSTV_OK) return st;
+  launch_clear_acc(ws.gacc, n, s);
+  launch_backward(ws.rec, ws.gacc, cov_inv, half_shape, tile_ranges, num_tiles, pair_ids, n, grad_r, grad_i, tile, d,
+                  h, w, nt, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, 1, nullptr, s);
   GSTV_LAUNCH_CHECK("backward");
-  prof_stop(1, slot, s);
-  hipLaunchKernelGGL(k_finalize, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, (const float*)gacc, cov_inv,
-                     half_shape, n, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, rho_stride);
-  GSTV_LAUNCH_CHECK("finalize");
   return GSTV_OK;
 }
-}  // namespace
 
-int gstv_backward(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
-                  const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, const float* half_shape,
-                  int32_t n, const float* grad_r, const float* grad_i, int32_t tile, int32_t d, int32_t h,
-                  int32_t w, float* grad_center, float* grad_cov_inv, float* grad_rho_r, float* grad_rho_i,
-                  void* workspace, size_t workspace_bytes, void* stream) {
-  return backward_impl(center_vox, cov_inv, rho_r, rho_i, 1, tile_ranges, num_tiles, pair_ids, half_shape, n,
-                       grad_r, grad_i, tile, d, h, w, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, workspace,
-                       workspace_bytes, nullptr, nullptr, stream);
+size_t gstv_keep_workspace_bytes(int32_t n) {
+  if (n < 0) return 0;
+  return keep_layout(nullptr, n, nullptr);
 }
 
-int gstv_backward_keep(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
-                       int32_t rho_stride, const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
-                       const float* half_shape, int32_t n, const float* grad_r, const float* grad_i, int32_t tile,
-                       int32_t d, int32_t h, int32_t w, float* grad_center, float* grad_cov_inv,
-                       float* grad_rho_r, float* grad_rho_i, void* workspace, size_t workspace_bytes,
-                       const void* forward_workspace, const uint32_t* pair_columns, void* stream) {
-  if (rho_stride < 1 || (n > 0 && (forward_workspace == nullptr || (tile <= 8 && pair_columns == nullptr))))
+int gstv_keep_prepare(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
+                      int32_t rho_stride, const float* half_shape, int32_t n, void* keep_workspace,
+                      size_t keep_workspace_bytes, void* stream) {
+  if (n < 0 || rho_stride < 1) return GSTV_ERR_INVALID_ARGUMENT;
+  if (n == 0) return GSTV_OK;
+  if (center_vox == nullptr || cov_inv == nullptr || rho_r == nullptr || rho_i == nullptr || half_shape == nullptr)
     return GSTV_ERR_INVALID_ARGUMENT;
-  return backward_impl(center_vox, cov_inv, rho_r, rho_i, rho_stride, tile_ranges, num_tiles, pair_ids, half_shape,
-                       n, grad_r, grad_i, tile, d, h, w, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i,
-                       workspace, workspace_bytes, forward_workspace, pair_columns, stream);
+  if (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n))
+    return GSTV_ERR_WORKSPACE_TOO_SMALL;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  KeepLayout ws;
+  keep_layout(keep_workspace, n, &ws);
+  int st = launch_prep(center_vox, cov_inv, rho_r, rho_i, rho_stride, half_shape, n, ws.rec, s);
+  if (st != GSTV_OK) return st;
+  launch_clear_acc(ws.gacc, n, s);
+  GSTV_LAUNCH_CHECK("keep_prepare");
+  return GSTV_OK;
+}
+
+int gstv_forward_keep(const void* keep_workspace, size_t keep_workspace_bytes, const int32_t* tile_ranges,
+                      int32_t num_tiles, const int32_t* pair_ids, int32_t n, int32_t tile, int32_t d, int32_t h,
+                      int32_t w, float* vol_r, float* vol_i, uint32_t* pair_columns, void* stream) {
+  int nt[3];
+  int st = check_tile_args(tile_ranges, num_tiles, n, tile, d, h, w, nt);
+  if (st != GSTV_OK) return st;
+  if (vol_r == nullptr || vol_i == nullptr) return GSTV_ERR_INVALID_ARGUMENT;
+  if (n > 0 && (pair_ids == nullptr || (tile <= 8 && pair_columns == nullptr))) return GSTV_ERR_INVALID_ARGUMENT;
+  if (n > 0 && (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n)))
+    return GSTV_ERR_WORKSPACE_TOO_SMALL;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  KeepLayout ws;
+  keep_layout(const_cast<void*>(keep_workspace), n, &ws);
+  launch_forward(ws.rec, tile_ranges, num_tiles, pair_ids, tile, d, h, w, nt, vol_r, vol_i, pair_columns, s);
+  GSTV_LAUNCH_CHECK("forward_keep");
+  return GSTV_OK;
+}
+
+int gstv_backward_keep(void* keep_workspace, size_t keep_workspace_bytes, const float* cov_inv,
+                       const float* half_shape, const int32_t* tile_ranges, int32_t num_tiles,
+                       const int32_t* pair_ids, int32_t n, const float* grad_r, const float* grad_i, int32_t tile,
+                       int32_t d, int32_t h, int32_t w, float* grad_center, float* grad_cov_inv,
+                       float* grad_rho_r, float* grad_rho_i, int32_t rho_stride, const uint32_t* pair_columns,
+                       void* stream) {
+  int nt[3];
+  int st = check_tile_args(tile_ranges, num_tiles, n, tile, d, h, w, nt);
+  if (st != GSTV_OK) return st;
+  if (rho_stride < 1) return GSTV_ERR_INVALID_ARGUMENT;
+  if (n == 0) return GSTV_OK;
+  if (cov_inv == nullptr || pair_ids == nullptr || half_shape == nullptr || grad_r == nullptr ||
+      grad_i == nullptr || grad_center == nullptr || grad_cov_inv == nullptr || grad_rho_r == nullptr ||
+      grad_rho_i == nullptr || (tile <= 8 && pair_columns == nullptr))
+    return GSTV_ERR_INVALID_ARGUMENT;
+  if (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n))
+    return GSTV_ERR_WORKSPACE_TOO_SMALL;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  KeepLayout ws;
+  keep_layout(keep_workspace, n, &ws);
+  launch_backward(ws.rec, ws.gacc, cov_inv, half_shape, tile_ranges, num_tiles, pair_ids, n, grad_r, grad_i, tile, d,
+                  h, w, nt, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, rho_stride, pair_columns, s);
+  GSTV_LAUNCH_CHECK("backward_keep");
+  return GSTV_OK;
 }
 
 int gstv_count_evaluations(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,

@@ -116,17 +116,29 @@ class _TileVoxelizeFusedHIP(torch.autograd.Function):
         scales_c = scales.detach().contiguous().float()
         rot_c = rotations.detach().contiguous().float()
         center, radius, cov, half_shape = _native.prologue(positions, scales_c, rot_c, volume_shape)
-        tile_ranges, pair_ids = build_tile_pairs(center, radius, volume_shape, tile_size)
         # The kernels read the complex64 density interleaved, in place: no
         # real / imag copies here, and one complex gradient in backward.
         dens = density.detach().reshape(-1).resolve_conj()
         if dens.dtype != torch.complex64:
             dens = dens.to(torch.complex64)
         dv = torch.view_as_real(dens.contiguous())
-        # The forward keeps its Gaussian records and per-pair column rows for
-        # the adjoint of this call, which then rebuilds neither.
+        # Binning reads the pair total back to the host once; the Gaussian
+        # records and the adjoint's zeroed accumulator, which need no binning,
+        # are queued behind that read so the GPU builds them while the host
+        # waits and then sizes the pair arrays.
+        offsets, total = _native.bin_count(center, radius, volume_shape, tile_size)
+        total_host = torch.empty(1, dtype=torch.int64, pin_memory=True)
+        total_host.copy_(total, non_blocking=True)
+        total_ready = torch.cuda.Event()
+        total_ready.record()
+        kept = _native.keep_prepare(center, cov, dv[:, 0], dv[:, 1], half_shape)
+        total_ready.synchronize()
+        tile_ranges, pair_ids = _native.bin_fill(center, radius, volume_shape, tile_size, offsets,
+                                                 int(total_host[0]))
+        # The forward keeps its records and per-pair column rows for the
+        # adjoint of this call, which then rebuilds neither.
         vol_r, vol_i, ctx.kept = _native.forward(center, cov, dv[:, 0], dv[:, 1], tile_ranges, pair_ids,
-                                                 half_shape, tile_size, d_size, h_size, w_size, keep=True)
+                                                 half_shape, tile_size, d_size, h_size, w_size, keep=kept)
         ctx.save_for_backward(scales_c, rot_c, center, cov, dv, tile_ranges, pair_ids, half_shape)
         ctx.density_dtype = density.dtype
         ctx.density_shape = density.shape
@@ -148,7 +160,6 @@ class _TileVoxelizeFusedHIP(torch.autograd.Function):
             center, cov, dv[:, 0], dv[:, 1], tile_ranges, pair_ids, half_shape,
             grad_vol_r.contiguous().float().view(-1), grad_vol_i.contiguous().float().view(-1),
             ctx.tile_size, d_size, h_size, w_size, kept=ctx.kept, grad_rho=g_dv)
-        ctx.kept = None
         g_p, g_s, g_q = _native.epilogue(scales_c, rot_c, g_c, g_m, shape)
         if ctx.density_dtype.is_complex:
             g_d = torch.view_as_complex(g_dv).to(ctx.density_dtype)

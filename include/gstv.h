@@ -141,41 +141,51 @@ int gstv_backward(const float* center_vox, const float* cov_inv,
 
 /* ------------------------------------------- forward and adjoint of one call --
  * No counterpart in the reference: a fast path for callers that run the
- * adjoint of the same call right after the forward (the fused autograd call).
- * gstv_forward_keep is gstv_forward that also leaves, for that adjoint,
- *   - the Gaussian records in its workspace (keep it unchanged), and
- *   - pair_columns [total pairs] (uint32, parallel to pair_ids): for every pair
- *     whose extent box touches its tile, the D rows that its column bound
- *     leaves in each 4x4 (H, W) column of the tile (7 bits per column; other
- *     entries are left unwritten).  Required, and written, when tile <= 8;
- *     ignored for larger tiles.
- * gstv_backward_keep is gstv_backward given that workspace and pair_columns,
- * for exactly the inputs of that forward; it skips rebuilding the records and
- * the column bounds.  Results equal gstv_backward's up to the order of fp32
- * atomic additions.
+ * adjoint of the same call after the forward (the fused autograd call), in
+ * three calls that share one caller-allocated keep workspace
+ * (gstv_keep_workspace_bytes(n) bytes; keep it, unchanged, between them):
+ *   gstv_keep_prepare   builds the Gaussian records that gstv_forward and
+ *                       gstv_backward build internally, and zeroes the
+ *                       adjoint's moment accumulator.  It needs no binning,
+ *                       so a caller can queue it before reading the pair
+ *                       total back to the host.
+ *   gstv_forward_keep   gstv_forward on those records; also writes
+ *                       pair_columns [total pairs] (uint32, parallel to
+ *                       pair_ids): for every pair whose extent box touches its
+ *                       tile, the D rows that its column bound leaves in each
+ *                       4x4 (H, W) column of the tile (7 bits per column; other
+ *                       entries are left unwritten).  Required, and written,
+ *                       when tile <= 8; ignored for larger tiles.
+ *   gstv_backward_keep  gstv_backward on the records, the accumulator and
+ *                       pair_columns, for exactly the inputs of that forward.
+ *                       It consumes the zeroed accumulator: a second adjoint
+ *                       of the same forward needs gstv_keep_prepare again.
+ * Results equal gstv_forward's bit for bit and gstv_backward's up to the order
+ * of fp32 atomic additions.
  * rho_stride (>= 1) is the distance, in floats, between consecutive Gaussians'
  * values in rho_r / rho_i and in grad_rho_r / grad_rho_i: 1 for planar arrays,
  * 2 with rho_i = rho_r + 1 (and grad_rho_i = grad_rho_r + 1) for an
  * interleaved complex64 density and its gradient. */
-int gstv_forward_keep(const float* center_vox, const float* cov_inv,
-                      const float* rho_r, const float* rho_i, int32_t rho_stride,
-                      const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
-                      const float* half_shape, int32_t n,
-                      int32_t tile, int32_t d, int32_t h, int32_t w,
-                      float* vol_r, float* vol_i,
-                      void* workspace, size_t workspace_bytes,
-                      uint32_t* pair_columns, void* stream);
+size_t gstv_keep_workspace_bytes(int32_t n);
 
-int gstv_backward_keep(const float* center_vox, const float* cov_inv,
-                       const float* rho_r, const float* rho_i, int32_t rho_stride,
+int gstv_keep_prepare(const float* center_vox, const float* cov_inv,
+                      const float* rho_r, const float* rho_i, int32_t rho_stride,
+                      const float* half_shape, int32_t n,
+                      void* keep_workspace, size_t keep_workspace_bytes, void* stream);
+
+int gstv_forward_keep(const void* keep_workspace, size_t keep_workspace_bytes,
+                      const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
+                      int32_t n, int32_t tile, int32_t d, int32_t h, int32_t w,
+                      float* vol_r, float* vol_i, uint32_t* pair_columns, void* stream);
+
+int gstv_backward_keep(void* keep_workspace, size_t keep_workspace_bytes,
+                       const float* cov_inv, const float* half_shape,
                        const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
-                       const float* half_shape, int32_t n,
-                       const float* grad_r, const float* grad_i,
+                       int32_t n, const float* grad_r, const float* grad_i,
                        int32_t tile, int32_t d, int32_t h, int32_t w,
                        float* grad_center, float* grad_cov_inv,
-                       float* grad_rho_r, float* grad_rho_i,
-                       void* workspace, size_t workspace_bytes,
-                       const void* forward_workspace, const uint32_t* pair_columns, void* stream);
+                       float* grad_rho_r, float* grad_rho_i, int32_t rho_stride,
+                       const uint32_t* pair_columns, void* stream);
 
 /* ------------------------------------------------------ prologue / epilogue --
  * Replaces the torch prologue of TileVoxelizer.forward (gaussian/tile_voxelizer.py:281-290,

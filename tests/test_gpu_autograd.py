@@ -207,3 +207,28 @@ def test_fused_call_density_layouts():
                          torch.view_as_real(out["c64"][2]).cpu().numpy())
         print(f"  density grad {name} vs complex64: rel L2 {e:.2e}")
         assert e < 5e-6, (name, e)
+
+
+def test_fused_call_backward_twice():
+    """backward(retain_graph=True) twice through the fused call: the second
+    adjoint rebuilds its zeroed accumulator and gives the same gradients."""
+    from gs_tmi_amd import TileVoxelizer
+
+    c = cases.edge_case("partial_tiles")
+    shape, tile = c["shape"], c["tile"]
+    _announce(f"fused call backward twice shape={shape} tile={tile}")
+    vox = TileVoxelizer(shape, tile_size=tile, device=DEV)
+    rng = np.random.default_rng(6)
+    g_r = torch.from_numpy(rng.normal(size=shape).astype(np.float32)).to(DEV)
+    g_i = torch.from_numpy(rng.normal(size=shape).astype(np.float32)).to(DEV)
+    leaves = [torch.from_numpy(np.ascontiguousarray(a)).to(DEV).requires_grad_()
+              for a in (c["pos"], np.log(c["scales"]), c["rot"], c["rr"], c["ri"])]
+    pos, log_s, rot, dr, di = leaves
+    vr, vi = vox(pos, torch.exp(log_s), rot, torch.complex(dr, di))
+    loss = (vr * g_r).sum() + (vi * g_i).sum()
+    first = torch.autograd.grad(loss, leaves, retain_graph=True)
+    second = torch.autograd.grad(loss, leaves)
+    for nm, a, b in zip(["pos", "log_s", "rot", "dr", "di"], first, second):
+        e = cases.rel_l2(b.cpu().numpy(), a.cpu().numpy())
+        print(f"  second backward {nm}: rel L2 {e:.2e}")
+        assert e < 5e-6, (nm, e)

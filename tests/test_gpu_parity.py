@@ -190,6 +190,11 @@ def test_keep_path_matches_plain_calls(name, make):
     tol = 2e-5 if name in CASE_TOL else 5e-6
     for a, b in zip(plain, keep):
         assert cases.rel_l2(b.cpu().numpy(), a.cpu().numpy()) < tol
+    # A second adjoint of the same forward (its accumulator is prepared again).
+    again = _native.backward(*args, g_r, g_i, c["tile"], d, h, w, kept=kept)
+    torch.cuda.synchronize()
+    for a, b in zip(plain, again):
+        assert cases.rel_l2(b.cpu().numpy(), a.cpu().numpy()) < tol
     # The same pair of calls on an interleaved (complex64) density and gradient.
     dv = torch.stack([rr, ri], 1).contiguous()
     iargs = (center, cov, dv[:, 0], dv[:, 1], ranges, pids, hs)
@@ -244,6 +249,27 @@ def test_binning_with_many_pairless_gaussians(shape, tile, frac):
     ranges, pids = build_tile_pairs(torch.from_numpy(center).to(DEV), torch.from_numpy(radius).to(DEV), shape, tile)
     torch.cuda.synchronize()
     r_exp, p_exp = vo.bin_tiles(center, radius, shape, tile)
+    np.testing.assert_array_equal(ranges.cpu().numpy(), r_exp)
+    np.testing.assert_array_equal(pids.cpu().numpy(), p_exp)
+
+
+@pytest.mark.parametrize("shape", [(128, 128, 64), (130, 130, 130)])
+def test_binning_at_tile_count_bounds(shape):
+    """One-voxel tiles: 2^20 tiles (the largest count for which the first radix
+    pass derives tile ids with reciprocal divisions; three Gaussians cover every
+    tile, so local pair indices reach 2^20 - 1) and 130^3 > 2^20 tiles (exact
+    integer divisions).  Bit-exact against the oracle."""
+    from gs_tmi_amd import build_tile_pairs
+
+    _announce(f"binning at tile-count bounds shape={shape} tile=1")
+    rng = np.random.default_rng(23)
+    n = 4000
+    center = (rng.random((n, 3)) * np.asarray(shape)).astype(np.float32)
+    radius = (rng.random(n) * 2.5).astype(np.float32)
+    radius[[5, 1700, 3999]] = 1000.0
+    ranges, pids = build_tile_pairs(torch.from_numpy(center).to(DEV), torch.from_numpy(radius).to(DEV), shape, 1)
+    torch.cuda.synchronize()
+    r_exp, p_exp = vo.bin_tiles(center, radius, shape, 1)
     np.testing.assert_array_equal(ranges.cpu().numpy(), r_exp)
     np.testing.assert_array_equal(pids.cpu().numpy(), p_exp)
 
