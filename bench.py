@@ -240,6 +240,21 @@ def main():
             e1.record()
             torch.cuda.synchronize()
             bt.append(e0.elapsed_time(e1))
+        # The fused call bins only the pairs whose tile meets a Gaussian's
+        # extent box (records first, then the two binning phases).
+        dv = torch.stack([params["dr"].detach(), params["di"].detach()], 1).contiguous()
+        kbt = []
+        for _ in range(5):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            kept = _native.keep_prepare(center, cov.contiguous(), dv[:, 0], dv[:, 1], hs)
+            offs, tot = _native.bin_count(center, radius, SHAPE, TILE, kept=kept)
+            _, kpids = _native.bin_fill(center, radius, SHAPE, TILE, offs, int(tot.item()), kept=kept)
+            e1.record()
+            torch.cuda.synchronize()
+            kbt.append(e0.elapsed_time(e1))
+        pairs_kept = int(kpids.numel())
+        del kept, kpids, dv
     log(f"[bench] pairs {pairs}, E_total {e_total:.4e}, E_active {e_active:.4e}, E_culled {e_culled:.4e}")
 
     log(f"[bench] timing {args.steps} steps")
@@ -316,7 +331,9 @@ def main():
             "forward_kernel_ms": round(fwd_ms, 4),
             "backward_kernel_ms": round(bwd_ms, 4),
             "binning_ms": round(statistics.median(bt), 4),
+            "keep_binning_ms": round(statistics.median(kbt), 4),
             "pairs": pairs,
+            "pairs_kept": pairs_kept,
             "E_total": e_total,
             "E_active": e_active,
             "E_culled": e_culled,

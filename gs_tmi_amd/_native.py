@@ -45,6 +45,12 @@ PROTOTYPES = {
                        _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _sz, _c_p]),
     "gstv_keep_workspace_bytes": (ctypes.c_size_t, [_i32]),
     "gstv_keep_prepare": (ctypes.c_int, [_c_p, _c_p, _c_p, _c_p, _i32, _c_p, _i32, _c_p, _sz, _c_p]),
+    "gstv_keep_clear": (ctypes.c_int, [_c_p, _sz, _i32, _c_p]),
+    "gstv_keep_bin_count": (ctypes.c_int,
+                            [_c_p, _sz, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32, _c_p, _c_p, _c_p, _sz, _c_p]),
+    "gstv_keep_bin_fill": (ctypes.c_int,
+                           [_c_p, _sz, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32, _c_p, ctypes.c_int64, _c_p, _c_p,
+                            _c_p, _sz, _c_p]),
     "gstv_forward_keep": (ctypes.c_int,
                           [_c_p, _sz, _c_p, _i32, _c_p, _i32, _i32, _i32, _i32, _i32, _c_p, _c_p, _c_p, _c_p]),
     "gstv_backward_keep": (ctypes.c_int,
@@ -127,8 +133,10 @@ def num_tiles_for(shape, tile_size: int) -> int:
     return (-(-d // tile_size)) * (-(-h // tile_size)) * (-(-w // tile_size))
 
 
-def bin_count(center_vox: torch.Tensor, radius_vox: torch.Tensor, shape, tile_size: int):
-    """Phase one of binning.  Returns (gauss_offsets int32 [n+1], total int64 [1]) on the device."""
+def bin_count(center_vox: torch.Tensor, radius_vox: torch.Tensor, shape, tile_size: int, kept=None):
+    """Phase one of binning.  Returns (gauss_offsets int32 [n+1], total int64 [1]) on the device.
+    With ``kept`` (a ``Kept`` of these Gaussians' records), only the pairs
+    whose tile meets a Gaussian's extent box are counted."""
     lib = get_lib()
     _require_gpu(center_vox, "center_vox")
     dev = center_vox.device
@@ -141,14 +149,19 @@ def bin_count(center_vox: torch.Tensor, radius_vox: torch.Tensor, shape, tile_si
     ws = _workspace(ws_bytes, dev)
     d, h, w = shape
     with torch.cuda.device(dev):
-        st = lib.gstv_bin_count(_ptr(c), _ptr(r), n, d, h, w, tile_size, _ptr(offsets), _ptr(total), _ptr(ws),
-                                ws_bytes, _stream(dev))
-    _check(st, "gstv_bin_count")
+        if kept is None:
+            st = lib.gstv_bin_count(_ptr(c), _ptr(r), n, d, h, w, tile_size, _ptr(offsets), _ptr(total), _ptr(ws),
+                                    ws_bytes, _stream(dev))
+        else:
+            st = lib.gstv_keep_bin_count(_ptr(kept.ws), kept.ws.numel(), _ptr(c), _ptr(r), n, d, h, w, tile_size,
+                                         _ptr(offsets), _ptr(total), _ptr(ws), ws_bytes, _stream(dev))
+    _check(st, "gstv_bin_count" if kept is None else "gstv_keep_bin_count")
     return offsets, total
 
 
-def bin_fill(center_vox, radius_vox, shape, tile_size, offsets, total_pairs: int):
-    """Phase two of binning.  Returns (tile_ranges int32 [T,2], pair_ids int32 [P])."""
+def bin_fill(center_vox, radius_vox, shape, tile_size, offsets, total_pairs: int, kept=None):
+    """Phase two of binning.  Returns (tile_ranges int32 [T,2], pair_ids int32 [P]).
+    ``kept``: as in ``bin_count`` (both phases must agree)."""
     lib = get_lib()
     dev = center_vox.device
     c = center_vox.detach().contiguous().float()
@@ -163,9 +176,14 @@ def bin_fill(center_vox, radius_vox, shape, tile_size, offsets, total_pairs: int
     ws = _workspace(ws_bytes, dev)
     d, h, w = shape
     with torch.cuda.device(dev):
-        st = lib.gstv_bin_fill(_ptr(c), _ptr(r), n, d, h, w, tile_size, _ptr(offsets), total_pairs,
-                               _ptr(tile_ranges), _ptr(pair_ids), _ptr(ws), ws_bytes, _stream(dev))
-    _check(st, "gstv_bin_fill")
+        if kept is None:
+            st = lib.gstv_bin_fill(_ptr(c), _ptr(r), n, d, h, w, tile_size, _ptr(offsets), total_pairs,
+                                   _ptr(tile_ranges), _ptr(pair_ids), _ptr(ws), ws_bytes, _stream(dev))
+        else:
+            st = lib.gstv_keep_bin_fill(_ptr(kept.ws), kept.ws.numel(), _ptr(c), _ptr(r), n, d, h, w, tile_size,
+                                        _ptr(offsets), total_pairs, _ptr(tile_ranges), _ptr(pair_ids), _ptr(ws),
+                                        ws_bytes, _stream(dev))
+    _check(st, "gstv_bin_fill" if kept is None else "gstv_keep_bin_fill")
     return tile_ranges, pair_ids
 
 
@@ -189,36 +207,43 @@ class Kept:
     """State the keep entry points hand from the forward to the adjoint of one
     call: the keep workspace (Gaussian records and moment accumulator) and the
     per-pair column words.  ``fresh``: the accumulator is zero (a second
-    adjoint prepares it again)."""
+    adjoint clears it again)."""
 
-    __slots__ = ("ws", "cols", "fresh", "stride")
+    __slots__ = ("ws", "n", "cols", "fresh", "stride")
 
-    def __init__(self, ws, stride):
+    def __init__(self, ws, n, stride):
         self.ws = ws
+        self.n = n
         self.cols = None
-        self.fresh = True
+        self.fresh = False
         self.stride = stride
 
 
-def keep_prepare(center_vox, cov_inv, rho_r, rho_i, half_shape, kept=None):
-    """Queues gstv_keep_prepare: Gaussian records and a zeroed accumulator in a
-    keep workspace (a new one, or ``kept``'s again).  rho_r / rho_i may be the
-    columns of an interleaved [n, 2] fp32 array.  Returns the ``Kept``."""
+def keep_prepare(center_vox, cov_inv, rho_r, rho_i, half_shape):
+    """Queues gstv_keep_prepare: the Gaussian records in a new keep workspace.
+    rho_r / rho_i may be the columns of an interleaved [n, 2] fp32 array.
+    Returns the ``Kept`` (its accumulator not yet cleared)."""
     lib = get_lib()
     _require_gpu(center_vox, "center_vox")
     dev = center_vox.device
     n = center_vox.shape[0]
     stride = _rho_stride(rho_r, rho_i, True)
-    if kept is None:
-        kept = Kept(_workspace(lib.gstv_keep_workspace_bytes(n), dev), stride)
-    elif kept.stride != stride:
-        raise ValueError("density layout differs from the kept forward's")
+    kept = Kept(_workspace(lib.gstv_keep_workspace_bytes(n), dev), n, stride)
     with torch.cuda.device(dev):
         st = lib.gstv_keep_prepare(_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i), stride,
                                    _ptr(half_shape), n, _ptr(kept.ws), kept.ws.numel(), _stream(dev))
     _check(st, "gstv_keep_prepare")
-    kept.fresh = True
     return kept
+
+
+def keep_clear(kept):
+    """Queues gstv_keep_clear: zeroes ``kept``'s moment accumulator."""
+    lib = get_lib()
+    dev = kept.ws.device
+    with torch.cuda.device(dev):
+        st = lib.gstv_keep_clear(_ptr(kept.ws), kept.ws.numel(), kept.n, _stream(dev))
+    _check(st, "gstv_keep_clear")
+    kept.fresh = True
 
 
 def forward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape, tile_size, d, h, w,
@@ -237,7 +262,11 @@ def forward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape
     vol_r = torch.empty(d * h * w, dtype=torch.float32, device=dev)
     vol_i = torch.empty(d * h * w, dtype=torch.float32, device=dev)
     if keep is not False:
-        kept = keep_prepare(center_vox, cov_inv, rho_r, rho_i, half_shape) if keep is True else keep
+        if keep is True:
+            kept = keep_prepare(center_vox, cov_inv, rho_r, rho_i, half_shape)
+            keep_clear(kept)
+        else:
+            kept = keep
         kept.cols = torch.empty(max(pair_ids.numel(), 1), dtype=torch.int32, device=dev)
         with torch.cuda.device(dev):
             st = lib.gstv_forward_keep(_ptr(kept.ws), kept.ws.numel(), _ptr(tile_ranges), tile_ranges.shape[0],
@@ -280,8 +309,10 @@ def backward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shap
         g_rr = torch.empty(n, dtype=torch.float32, device=dev)
         g_ri = torch.empty(n, dtype=torch.float32, device=dev)
     if kept is not None:
+        if kept.stride != stride:
+            raise ValueError("density layout differs from the kept forward's")
         if not kept.fresh:
-            keep_prepare(center_vox, cov_inv, rho_r, rho_i, half_shape, kept)
+            keep_clear(kept)
         with torch.cuda.device(dev):
             st = lib.gstv_backward_keep(_ptr(kept.ws), kept.ws.numel(), _ptr(cov_inv), _ptr(half_shape),
                                         _ptr(tile_ranges), tile_ranges.shape[0], _ptr(pair_ids), n, _ptr(grad_r),

@@ -233,10 +233,20 @@ __global__ void k_fill_f32(float* __restrict__ p, int64_t n, float v) {
 
 // ----------------------------------------------------------------- binning --
 
+// Record of one Gaussian for the tile kernels (built by k_prep).
+struct __align__(16) GRec {
+  float c0, c1, c2, rr;      // centre (voxel units, axes D,H,W), density real
+  float ri, a00, a11, a22;   // density imag, diagonal of the scaled form
+  float b01, b02, b12, e0;   // doubled off-diagonals of the scaled form; extent D
+  float e1, e2, pad0, pad1;  // extents H, W
+};
+static_assert(sizeof(GRec) == 64, "record must be one 64-byte line");
+
 struct BinGeom {
   int size[3];
   int nt[3];
   int tile;
+  const GRec* rec;  // non-null: pairs only where the records' extent boxes meet the tile
 };
 
 // float -> int32 conversion with truncation, saturation and NaN -> 0 (the
@@ -262,15 +272,37 @@ __device__ __forceinline__ void gauss_tile_box(const float* __restrict__ center,
     hi = min(max(hi, 0), geo.size[a] - 1);
     int tl = min(max(lo / geo.tile, 0), geo.nt[a] - 1);
     int th = min(max(hi / geo.tile, 0), geo.nt[a] - 1);
+    if (geo.rec != nullptr) {
+      // Tiles that the extent box [ceil(c - e), floor(c + e)] of the record
+      // meets.  The tile kernels skip every other pair of the tile lists (the
+      // box test of stage_pair, with the same float bounds), so dropping them
+      // here changes no result; a NaN bound restricts nothing, as there.
+      const GRec& rg = geo.rec[g];
+      const float cc = a == 0 ? rg.c0 : (a == 1 ? rg.c1 : rg.c2);
+      const float ee = a == 0 ? rg.e0 : (a == 1 ? rg.e1 : rg.e2);
+      float blo = ceilf(cc - ee), bhi = floorf(cc + ee);
+      if (!(blo == blo)) blo = -INFINITY;
+      if (!(bhi == bhi)) bhi = INFINITY;
+      const float last = (float)(geo.size[a] - 1);
+      if (blo > bhi || blo > last || bhi < 0.0f) {
+        th = tl - 1;
+      } else {
+        if (blo > 0.0f) tl = max(tl, (int)blo / geo.tile);
+        if (bhi < last) th = min(th, (int)bhi / geo.tile);
+      }
+    }
     tmin[a] = tl;
     tcnt[a] = max(th - tl + 1, 0);  // a negative radius can invert the box: no pairs
   }
 }
 
+// Pairs per Gaussian; counts[n] = 0 so that the scan of n + 1 entries leaves
+// the total in its last entry.
 __global__ __launch_bounds__(256) void k_bin_count(const float* __restrict__ center,
                                                    const float* __restrict__ radius, int n, BinGeom geo,
                                                    int32_t* __restrict__ counts) {
   const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g == n) counts[n] = 0;
   if (g >= n) return;
   int tmin[3], tcnt[3];
   gauss_tile_box(center, radius, g, geo, tmin, tcnt);
@@ -761,13 +793,6 @@ __global__ void k_ranges(const int32_t* __restrict__ keys, int total, int32_t* _
 
 // -------------------------------------------------------- Gaussian records --
 
-struct __align__(16) GRec {
-  float c0, c1, c2, rr;      // centre (voxel units, axes D,H,W), density real
-  float ri, a00, a11, a22;   // density imag, diagonal of the scaled form
-  float b01, b02, b12, e0;   // doubled off-diagonals of the scaled form; extent D
-  float e1, e2, pad0, pad1;  // extents H, W
-};
-static_assert(sizeof(GRec) == 64, "record must be one 64-byte line");
 
 // Record of one Gaussian.  The quadratic form of the reference,
 // q = d^T M d with d = (x - c) / hs, is rewritten in voxel units u = x - c as
@@ -2030,10 +2055,12 @@ size_t gstv_bin_count_workspace_bytes(int32_t n) {
   return align_up(cv.off, 256);
 }
 
-int gstv_bin_count(const float* center_vox, const float* radius_vox, int32_t n, int32_t d, int32_t h,
-                   int32_t w, int32_t tile, int32_t* gauss_offsets, int64_t* total_pairs, void* workspace,
-                   size_t workspace_bytes, void* stream) {
+namespace {
+int bin_count_impl(const GRec* rec, const float* center_vox, const float* radius_vox, int32_t n, int32_t d,
+                   int32_t h, int32_t w, int32_t tile, int32_t* gauss_offsets, int64_t* total_pairs,
+                   void* workspace, size_t workspace_bytes, void* stream) {
   BinGeom geo;
+  geo.rec = rec;
   int64_t num_tiles;
   if (n < 0 || gauss_offsets == nullptr || total_pairs == nullptr ||
       !tile_geometry(d, h, w, tile, geo.nt, &num_tiles))
@@ -2048,15 +2075,18 @@ int gstv_bin_count(const float* center_vox, const float* radius_vox, int32_t n, 
   hipStream_t s = static_cast<hipStream_t>(stream);
   Carve cv(workspace);
   int64_t* sums = cv.take<int64_t>((size_t)ceil_div_l((int64_t)n + 1, kScanTile) + 1);
-  // counts[n] = 0 so the scan of n+1 entries leaves the total in offsets[n].
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, gauss_offsets + n, (int64_t)1, 0);
-  GSTV_LAUNCH_CHECK("bin_count_fill");
-  if (n > 0) {
-    hipLaunchKernelGGL(k_bin_count, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, center_vox, radius_vox, n, geo,
-                       gauss_offsets);
-    GSTV_LAUNCH_CHECK("bin_count");
-  }
+  hipLaunchKernelGGL(k_bin_count, dim3(ceil_div_i(n + 1, 256)), dim3(256), 0, s, center_vox, radius_vox, n, geo,
+                     gauss_offsets);
+  GSTV_LAUNCH_CHECK("bin_count");
   return launch_exclusive_scan(gauss_offsets, (int64_t)n + 1, gauss_offsets, sums, total_pairs, s);
+}
+}  // namespace
+
+int gstv_bin_count(const float* center_vox, const float* radius_vox, int32_t n, int32_t d, int32_t h,
+                   int32_t w, int32_t tile, int32_t* gauss_offsets, int64_t* total_pairs, void* workspace,
+                   size_t workspace_bytes, void* stream) {
+  return bin_count_impl(nullptr, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, total_pairs, workspace,
+                        workspace_bytes, stream);
 }
 
 size_t gstv_bin_fill_workspace_bytes(int32_t n, int64_t total_pairs, int32_t num_tiles) {
@@ -2068,11 +2098,13 @@ size_t gstv_bin_fill_workspace_bytes(int32_t n, int64_t total_pairs, int32_t num
   return fill_ws_layout(total_pairs, num_tiles, cv, &ka, &va, &kb, &vb, &hist, &sums, &first);
 }
 
-int gstv_bin_fill(const float* center_vox, const float* radius_vox, int32_t n, int32_t d, int32_t h,
-                  int32_t w, int32_t tile, const int32_t* gauss_offsets, int64_t total_pairs,
+namespace {
+int bin_fill_impl(const GRec* rec, const float* center_vox, const float* radius_vox, int32_t n, int32_t d,
+                  int32_t h, int32_t w, int32_t tile, const int32_t* gauss_offsets, int64_t total_pairs,
                   int32_t* tile_ranges, int32_t* pair_ids, void* workspace, size_t workspace_bytes,
                   void* stream) {
   BinGeom geo;
+  geo.rec = rec;
   int64_t num_tiles64;
   if (n < 0 || total_pairs < 0 || tile_ranges == nullptr || !tile_geometry(d, h, w, tile, geo.nt, &num_tiles64))
     return GSTV_ERR_INVALID_ARGUMENT;
@@ -2165,6 +2197,15 @@ int gstv_bin_fill(const float* center_vox, const float* radius_vox, int32_t n, i
   hipLaunchKernelGGL(k_ranges_fix, dim3(ceil_div_i(num_tiles, 256)), dim3(256), 0, s, tile_ranges, num_tiles);
   GSTV_LAUNCH_CHECK("ranges_fix");
   return GSTV_OK;
+}
+}  // namespace
+
+int gstv_bin_fill(const float* center_vox, const float* radius_vox, int32_t n, int32_t d, int32_t h,
+                  int32_t w, int32_t tile, const int32_t* gauss_offsets, int64_t total_pairs,
+                  int32_t* tile_ranges, int32_t* pair_ids, void* workspace, size_t workspace_bytes,
+                  void* stream) {
+  return bin_fill_impl(nullptr, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, total_pairs, tile_ranges,
+                       pair_ids, workspace, workspace_bytes, stream);
 }
 
 size_t gstv_forward_workspace_bytes(int32_t n) {
@@ -2323,11 +2364,45 @@ int gstv_keep_prepare(const float* center_vox, const float* cov_inv, const float
   hipStream_t s = static_cast<hipStream_t>(stream);
   KeepLayout ws;
   keep_layout(keep_workspace, n, &ws);
-  int st = launch_prep(center_vox, cov_inv, rho_r, rho_i, rho_stride, half_shape, n, ws.rec, s);
-  if (st != GSTV_OK) return st;
-  launch_clear_acc(ws.gacc, n, s);
-  GSTV_LAUNCH_CHECK("keep_prepare");
+  return launch_prep(center_vox, cov_inv, rho_r, rho_i, rho_stride, half_shape, n, ws.rec, s);
+}
+
+int gstv_keep_clear(void* keep_workspace, size_t keep_workspace_bytes, int32_t n, void* stream) {
+  if (n < 0) return GSTV_ERR_INVALID_ARGUMENT;
+  if (n == 0) return GSTV_OK;
+  if (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n))
+    return GSTV_ERR_WORKSPACE_TOO_SMALL;
+  KeepLayout ws;
+  keep_layout(keep_workspace, n, &ws);
+  launch_clear_acc(ws.gacc, n, static_cast<hipStream_t>(stream));
+  GSTV_LAUNCH_CHECK("keep_clear");
   return GSTV_OK;
+}
+
+int gstv_keep_bin_count(const void* keep_workspace, size_t keep_workspace_bytes, const float* center_vox,
+                        const float* radius_vox, int32_t n, int32_t d, int32_t h, int32_t w, int32_t tile,
+                        int32_t* gauss_offsets, int64_t* total_pairs, void* workspace, size_t workspace_bytes,
+                        void* stream) {
+  if (n < 0) return GSTV_ERR_INVALID_ARGUMENT;
+  if (n > 0 && (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n)))
+    return GSTV_ERR_WORKSPACE_TOO_SMALL;
+  KeepLayout ws{nullptr, nullptr};
+  if (n > 0) keep_layout(const_cast<void*>(keep_workspace), n, &ws);
+  return bin_count_impl(ws.rec, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, total_pairs, workspace,
+                        workspace_bytes, stream);
+}
+
+int gstv_keep_bin_fill(const void* keep_workspace, size_t keep_workspace_bytes, const float* center_vox,
+                       const float* radius_vox, int32_t n, int32_t d, int32_t h, int32_t w, int32_t tile,
+                       const int32_t* gauss_offsets, int64_t total_pairs, int32_t* tile_ranges, int32_t* pair_ids,
+                       void* workspace, size_t workspace_bytes, void* stream) {
+  if (n < 0) return GSTV_ERR_INVALID_ARGUMENT;
+  if (n > 0 && (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n)))
+    return GSTV_ERR_WORKSPACE_TOO_SMALL;
+  KeepLayout ws{nullptr, nullptr};
+  if (n > 0) keep_layout(const_cast<void*>(keep_workspace), n, &ws);
+  return bin_fill_impl(ws.rec, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, total_pairs, tile_ranges,
+                       pair_ids, workspace, workspace_bytes, stream);
 }
 
 int gstv_forward_keep(const void* keep_workspace, size_t keep_workspace_bytes, const int32_t* tile_ranges,

@@ -122,19 +122,22 @@ class _TileVoxelizeFusedHIP(torch.autograd.Function):
         if dens.dtype != torch.complex64:
             dens = dens.to(torch.complex64)
         dv = torch.view_as_real(dens.contiguous())
-        # Binning reads the pair total back to the host once; the Gaussian
-        # records and the adjoint's zeroed accumulator, which need no binning,
-        # are queued behind that read so the GPU builds them while the host
-        # waits and then sizes the pair arrays.
-        offsets, total = _native.bin_count(center, radius, volume_shape, tile_size)
+        # The Gaussian records come first: binning keeps only the pairs whose
+        # tile meets a Gaussian's extent box, the pairs the tile kernels
+        # evaluate (the reference's lists hold about twice as many, which the
+        # kernels would skip).  Binning reads the pair total back to the host
+        # once; the adjoint's accumulator is cleared behind that read, while
+        # the host waits and then sizes the pair arrays.
+        kept = _native.keep_prepare(center, cov, dv[:, 0], dv[:, 1], half_shape)
+        offsets, total = _native.bin_count(center, radius, volume_shape, tile_size, kept=kept)
         total_host = torch.empty(1, dtype=torch.int64, pin_memory=True)
         total_host.copy_(total, non_blocking=True)
         total_ready = torch.cuda.Event()
         total_ready.record()
-        kept = _native.keep_prepare(center, cov, dv[:, 0], dv[:, 1], half_shape)
+        _native.keep_clear(kept)
         total_ready.synchronize()
         tile_ranges, pair_ids = _native.bin_fill(center, radius, volume_shape, tile_size, offsets,
-                                                 int(total_host[0]))
+                                                 int(total_host[0]), kept=kept)
         # The forward keeps its records and per-pair column rows for the
         # adjoint of this call, which then rebuilds neither.
         vol_r, vol_i, ctx.kept = _native.forward(center, cov, dv[:, 0], dv[:, 1], tile_ranges, pair_ids,

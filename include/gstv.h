@@ -141,25 +141,36 @@ int gstv_backward(const float* center_vox, const float* cov_inv,
 
 /* ------------------------------------------- forward and adjoint of one call --
  * No counterpart in the reference: a fast path for callers that run the
- * adjoint of the same call after the forward (the fused autograd call), in
- * three calls that share one caller-allocated keep workspace
+ * adjoint of the same call after the forward (the fused autograd call).  The
+ * calls share one caller-allocated keep workspace
  * (gstv_keep_workspace_bytes(n) bytes; keep it, unchanged, between them):
  *   gstv_keep_prepare   builds the Gaussian records that gstv_forward and
- *                       gstv_backward build internally, and zeroes the
- *                       adjoint's moment accumulator.  It needs no binning,
- *                       so a caller can queue it before reading the pair
- *                       total back to the host.
- *   gstv_forward_keep   gstv_forward on those records; also writes
+ *                       gstv_backward build internally.
+ *   gstv_keep_bin_count, gstv_keep_bin_fill
+ *                       gstv_bin_count / gstv_bin_fill restricted to the pairs
+ *                       whose tile meets the Gaussian's extent box (the box of
+ *                       its q <= 9 ellipsoid in the records, widened so that
+ *                       no voxel with fp32 q <= 9 lies outside): the tile
+ *                       kernels skip every other pair of the reference's
+ *                       lists, so the forward on these lists is bit-identical
+ *                       and the adjoint equal up to the order of fp32 atomic
+ *                       additions.  Same order within each tile (ascending
+ *                       Gaussian id).  Needs the records.
+ *   gstv_keep_clear     zeroes the adjoint's moment accumulator.  Neither it
+ *                       nor the records need binning, so a caller can queue
+ *                       them before reading the pair total back to the host.
+ *   gstv_forward_keep   gstv_forward on the records; also writes
  *                       pair_columns [total pairs] (uint32, parallel to
  *                       pair_ids): for every pair whose extent box touches its
  *                       tile, the D rows that its column bound leaves in each
  *                       4x4 (H, W) column of the tile (7 bits per column; other
  *                       entries are left unwritten).  Required, and written,
- *                       when tile <= 8; ignored for larger tiles.
- *   gstv_backward_keep  gstv_backward on the records, the accumulator and
- *                       pair_columns, for exactly the inputs of that forward.
- *                       It consumes the zeroed accumulator: a second adjoint
- *                       of the same forward needs gstv_keep_prepare again.
+ *                       when tile <= 8; ignored for larger tiles.  Any tile
+ *                       lists may be given (those of gstv_bin_fill too).
+ *   gstv_backward_keep  gstv_backward on the records, the zeroed accumulator
+ *                       and pair_columns, for exactly the inputs and lists of
+ *                       that forward.  It consumes the accumulator: a second
+ *                       adjoint of the same forward needs gstv_keep_clear again.
  * Results equal gstv_forward's bit for bit and gstv_backward's up to the order
  * of fp32 atomic additions.
  * rho_stride (>= 1) is the distance, in floats, between consecutive Gaussians'
@@ -172,6 +183,21 @@ int gstv_keep_prepare(const float* center_vox, const float* cov_inv,
                       const float* rho_r, const float* rho_i, int32_t rho_stride,
                       const float* half_shape, int32_t n,
                       void* keep_workspace, size_t keep_workspace_bytes, void* stream);
+
+int gstv_keep_clear(void* keep_workspace, size_t keep_workspace_bytes, int32_t n, void* stream);
+
+int gstv_keep_bin_count(const void* keep_workspace, size_t keep_workspace_bytes,
+                        const float* center_vox, const float* radius_vox, int32_t n,
+                        int32_t d, int32_t h, int32_t w, int32_t tile,
+                        int32_t* gauss_offsets, int64_t* total_pairs,
+                        void* workspace, size_t workspace_bytes, void* stream);
+
+int gstv_keep_bin_fill(const void* keep_workspace, size_t keep_workspace_bytes,
+                       const float* center_vox, const float* radius_vox, int32_t n,
+                       int32_t d, int32_t h, int32_t w, int32_t tile,
+                       const int32_t* gauss_offsets, int64_t total_pairs,
+                       int32_t* tile_ranges, int32_t* pair_ids,
+                       void* workspace, size_t workspace_bytes, void* stream);
 
 int gstv_forward_keep(const void* keep_workspace, size_t keep_workspace_bytes,
                       const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,

@@ -163,6 +163,74 @@ def test_culling_keeps_every_active_evaluation(name, make):
     assert e_active <= e_culled <= e_total
 
 
+def _touching_lists(ranges, pids, rec, shape, tile):
+    """The reference's tile lists restricted to the pairs that pass the tile
+    kernels' box test: tile [o, o + ext) meets [ceil(c - e), floor(c + e)] on
+    every axis, with c and e (fp32) read from the Gaussian records, and a NaN
+    bound restricting nothing.  Returns (ranges, pair_ids) in the same form."""
+    nt = [-(-s // tile) for s in shape]
+    counts = ranges[:, 1] - ranges[:, 0]
+    tiles = np.repeat(np.arange(len(ranges)), counts)
+    order = np.concatenate([np.arange(a, b) for a, b in ranges]) if len(pids) else np.zeros(0, np.int64)
+    p = pids[order]
+    tz, rem = np.divmod(tiles, nt[1] * nt[2])
+    ty, tx = np.divmod(rem, nt[2])
+    keep = np.ones(len(p), bool)
+    for a, t in enumerate((tz, ty, tx)):
+        c = rec[p, a]
+        e = rec[p, 11 + a]
+        with np.errstate(invalid="ignore"):
+            lo_b, hi_b = np.ceil(c - e), np.floor(c + e)
+        o = (t * tile).astype(np.float32)
+        last = np.minimum(t * tile + tile, shape[a]).astype(np.float32) - 1
+        keep &= np.fmax(lo_b, o) <= np.fmin(hi_b, last)
+    new_counts = np.bincount(tiles[keep], minlength=len(ranges))
+    ends = np.cumsum(new_counts)
+    out = np.stack([ends - new_counts, ends], 1).astype(np.int32)
+    out[new_counts == 0] = 0
+    return out, p[keep].astype(np.int32)
+
+
+@pytest.mark.parametrize("name,make", ALL_CASES, ids=[n for n, _ in ALL_CASES])
+def test_keep_binning_is_the_touching_pairs(name, make):
+    """gstv_keep_bin_count / _fill give exactly the reference's pairs that the
+    tile kernels' box test keeps, in the same order, and the forward on those
+    lists equals the forward on the reference's lists bit for bit."""
+    from gs_tmi_amd import _native
+
+    c = make()
+    d, h, w = c["shape"]
+    tile = c["tile"]
+    _announce(f"keep binning {name} shape={c['shape']} tile={tile} n={len(c['rr'])}")
+    center, radius, cov, hs = _prologue_gpu(c)
+    ranges, pids = _bin_gpu(center, radius, c)
+    rr = torch.from_numpy(c["rr"]).to(DEV)
+    ri = torch.from_numpy(c["ri"]).to(DEV)
+    kept = _native.keep_prepare(center, cov, rr, ri, hs)
+    _native.keep_clear(kept)
+    offs, total = _native.bin_count(center, radius, c["shape"], tile, kept=kept)
+    t_ranges, t_pids = _native.bin_fill(center, radius, c["shape"], tile, offs, int(total.item()), kept=kept)
+    n = center.shape[0]
+    rec = kept.ws[: n * 64].view(torch.float32).view(n, 16).cpu().numpy()
+    e_ranges, e_pids = _touching_lists(ranges, pids, rec, c["shape"], tile)
+    np.testing.assert_array_equal(t_ranges.cpu().numpy(), e_ranges)
+    np.testing.assert_array_equal(t_pids.cpu().numpy(), e_pids)
+    print(f"  pairs: reference {len(pids)}, touching {len(e_pids)}")
+    r_t, p_t = torch.from_numpy(ranges).to(DEV), torch.from_numpy(pids).to(DEV)
+    vol_r, vol_i = _native.forward(center, cov, rr, ri, r_t, p_t, hs, tile, d, h, w)
+    kvol_r, kvol_i, kept = _native.forward(center, cov, rr, ri, t_ranges, t_pids, hs, tile, d, h, w, keep=kept)
+    assert torch.equal(vol_r, kvol_r) and torch.equal(vol_i, kvol_i)
+    rng = np.random.default_rng(10)
+    g_r = torch.from_numpy(rng.normal(size=d * h * w).astype(np.float32)).to(DEV)
+    g_i = torch.from_numpy(rng.normal(size=d * h * w).astype(np.float32)).to(DEV)
+    plain = _native.backward(center, cov, rr, ri, r_t, p_t, hs, g_r, g_i, tile, d, h, w)
+    keep = _native.backward(center, cov, rr, ri, t_ranges, t_pids, hs, g_r, g_i, tile, d, h, w, kept=kept)
+    torch.cuda.synchronize()
+    tol = 2e-5 if name in CASE_TOL else 5e-6
+    for a, b in zip(plain, keep):
+        assert cases.rel_l2(b.cpu().numpy(), a.cpu().numpy()) < tol
+
+
 @pytest.mark.parametrize("name,make", ALL_CASES, ids=[n for n, _ in ALL_CASES])
 def test_keep_path_matches_plain_calls(name, make):
     """gstv_forward_keep / gstv_backward_keep (records and column rows handed
