@@ -46,10 +46,10 @@ TILE = 8
 
 # HBM bytes per k_backward launch at this workload, from two rocprofv3 passes
 # (FETCH_SIZE, WRITE_SIZE) scaled by the same counters' reading of a 512 MiB
-# copy: 659.7 MB read + 778.5 MB written (the writes are mostly the fp32
+# copy: 478.0 MB read + 778.5 MB written (the writes are mostly the fp32
 # moment atomics).  Counters cannot be read from inside this process, so the
 # figure is a measured constant of the current kernels.
-BACKWARD_TRAFFIC_BYTES = 1_438_200_000
+BACKWARD_TRAFFIC_BYTES = 1_256_400_000
 
 # Algorithmic FLOPs per evaluation, counted from the reference kernel
 # (FMA = 2, mul/add/sub/div = 1, exp = 1): forward 26 ungated + 6 gated,
@@ -247,9 +247,9 @@ def main():
         for _ in range(5):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
-            kept = _native.keep_prepare(center, cov.contiguous(), dv[:, 0], dv[:, 1], hs)
-            offs, tot = _native.bin_count(center, radius, SHAPE, TILE, kept=kept)
-            _, kpids = _native.bin_fill(center, radius, SHAPE, TILE, offs, int(tot.item()), kept=kept)
+            kept = _native.keep_prepare(center, cov.contiguous(), dv[:, 0], dv[:, 1], hs, radius, SHAPE, TILE)
+            _, kpids = _native.bin_fill(center, radius, SHAPE, TILE, kept.offsets, int(kept.total.item()),
+                                        kept=kept)
             e1.record()
             torch.cuda.synchronize()
             kbt.append(e0.elapsed_time(e1))
