@@ -44,10 +44,9 @@ PROTOTYPES = {
                       [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32,
                        _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _sz, _c_p]),
     "gstv_keep_workspace_bytes": (ctypes.c_size_t, [_i32]),
-    "gstv_keep_prepare": (ctypes.c_int, [_c_p, _c_p, _c_p, _c_p, _i32, _c_p, _i32, _c_p, _sz, _c_p]),
+    "gstv_keep_prepare": (ctypes.c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _i32, _i32, _i32, _i32, _i32,
+                                         _c_p, _c_p, _c_p, _sz, _c_p]),
     "gstv_keep_clear": (ctypes.c_int, [_c_p, _sz, _i32, _c_p]),
-    "gstv_keep_bin_count": (ctypes.c_int,
-                            [_c_p, _sz, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32, _c_p, _c_p, _c_p, _sz, _c_p]),
     "gstv_keep_bin_fill": (ctypes.c_int,
                            [_c_p, _sz, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32, _c_p, ctypes.c_int64, _c_p, _c_p,
                             _c_p, _sz, _c_p]),
@@ -133,10 +132,8 @@ def num_tiles_for(shape, tile_size: int) -> int:
     return (-(-d // tile_size)) * (-(-h // tile_size)) * (-(-w // tile_size))
 
 
-def bin_count(center_vox: torch.Tensor, radius_vox: torch.Tensor, shape, tile_size: int, kept=None):
-    """Phase one of binning.  Returns (gauss_offsets int32 [n+1], total int64 [1]) on the device.
-    With ``kept`` (a ``Kept`` of these Gaussians' records), only the pairs
-    whose tile meets a Gaussian's extent box are counted."""
+def bin_count(center_vox: torch.Tensor, radius_vox: torch.Tensor, shape, tile_size: int):
+    """Phase one of binning.  Returns (gauss_offsets int32 [n+1], total int64 [1]) on the device."""
     lib = get_lib()
     _require_gpu(center_vox, "center_vox")
     dev = center_vox.device
@@ -149,19 +146,17 @@ def bin_count(center_vox: torch.Tensor, radius_vox: torch.Tensor, shape, tile_si
     ws = _workspace(ws_bytes, dev)
     d, h, w = shape
     with torch.cuda.device(dev):
-        if kept is None:
-            st = lib.gstv_bin_count(_ptr(c), _ptr(r), n, d, h, w, tile_size, _ptr(offsets), _ptr(total), _ptr(ws),
-                                    ws_bytes, _stream(dev))
-        else:
-            st = lib.gstv_keep_bin_count(_ptr(kept.ws), kept.ws.numel(), _ptr(c), _ptr(r), n, d, h, w, tile_size,
-                                         _ptr(offsets), _ptr(total), _ptr(ws), ws_bytes, _stream(dev))
-    _check(st, "gstv_bin_count" if kept is None else "gstv_keep_bin_count")
+        st = lib.gstv_bin_count(_ptr(c), _ptr(r), n, d, h, w, tile_size, _ptr(offsets), _ptr(total), _ptr(ws),
+                                ws_bytes, _stream(dev))
+    _check(st, "gstv_bin_count")
     return offsets, total
 
 
-def bin_fill(center_vox, radius_vox, shape, tile_size, offsets, total_pairs: int, kept=None):
+def bin_fill(center_vox, radius_vox, shape, tile_size, offsets, total_pairs: int, kept=None, tile_ranges=None):
     """Phase two of binning.  Returns (tile_ranges int32 [T,2], pair_ids int32 [P]).
-    ``kept``: as in ``bin_count`` (both phases must agree)."""
+    ``kept``: a ``Kept`` whose ``keep_prepare`` counted the pairs (offsets
+    from it): the pairs whose tile meets a Gaussian's extent box.
+    ``tile_ranges``: an int32 [T, 2] array to fill (allocated here if None)."""
     lib = get_lib()
     dev = center_vox.device
     c = center_vox.detach().contiguous().float()
@@ -170,7 +165,8 @@ def bin_fill(center_vox, radius_vox, shape, tile_size, offsets, total_pairs: int
     num_tiles = num_tiles_for(shape, tile_size)
     if total_pairs >= 2**31 - 1:
         _check(3, "gstv_bin_fill")
-    tile_ranges = torch.empty(num_tiles, 2, dtype=torch.int32, device=dev)
+    if tile_ranges is None:
+        tile_ranges = torch.empty(num_tiles, 2, dtype=torch.int32, device=dev)
     pair_ids = torch.empty(total_pairs, dtype=torch.int32, device=dev)
     ws_bytes = lib.gstv_bin_fill_workspace_bytes(n, total_pairs, num_tiles)
     ws = _workspace(ws_bytes, dev)
@@ -209,7 +205,7 @@ class Kept:
     per-pair column words.  ``fresh``: the accumulator is zero (a second
     adjoint clears it again)."""
 
-    __slots__ = ("ws", "n", "cols", "fresh", "stride")
+    __slots__ = ("ws", "n", "cols", "fresh", "stride", "offsets", "total")
 
     def __init__(self, ws, n, stride):
         self.ws = ws
@@ -217,21 +213,34 @@ class Kept:
         self.cols = None
         self.fresh = False
         self.stride = stride
+        self.offsets = None
+        self.total = None
 
 
-def keep_prepare(center_vox, cov_inv, rho_r, rho_i, half_shape):
+def keep_prepare(center_vox, cov_inv, rho_r, rho_i, half_shape, radius_vox=None, shape=None, tile_size=None):
     """Queues gstv_keep_prepare: the Gaussian records in a new keep workspace.
     rho_r / rho_i may be the columns of an interleaved [n, 2] fp32 array.
-    Returns the ``Kept`` (its accumulator not yet cleared)."""
+    With ``radius_vox``, ``shape`` and ``tile_size`` it also counts the pairs
+    whose tile meets a Gaussian's extent box: ``kept.offsets`` (int32 [n+1])
+    and ``kept.total`` (int64 [1]) on the device, for ``bin_fill(...,
+    kept=kept)``.  Returns the ``Kept`` (its accumulator not yet cleared)."""
     lib = get_lib()
     _require_gpu(center_vox, "center_vox")
     dev = center_vox.device
     n = center_vox.shape[0]
     stride = _rho_stride(rho_r, rho_i, True)
     kept = Kept(_workspace(lib.gstv_keep_workspace_bytes(n), dev), n, stride)
+    d = h = w = 0
+    tile = 0
+    if radius_vox is not None:
+        d, h, w = shape
+        tile = tile_size
+        kept.offsets = torch.empty(n + 1, dtype=torch.int32, device=dev)
+        kept.total = torch.empty(1, dtype=torch.int64, device=dev)
     with torch.cuda.device(dev):
-        st = lib.gstv_keep_prepare(_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i), stride,
-                                   _ptr(half_shape), n, _ptr(kept.ws), kept.ws.numel(), _stream(dev))
+        st = lib.gstv_keep_prepare(_ptr(center_vox), _ptr(radius_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i),
+                                   stride, _ptr(half_shape), n, d, h, w, tile, _ptr(kept.offsets),
+                                   _ptr(kept.total), _ptr(kept.ws), kept.ws.numel(), _stream(dev))
     _check(st, "gstv_keep_prepare")
     return kept
 

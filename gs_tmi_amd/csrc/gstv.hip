@@ -246,8 +246,15 @@ struct BinGeom {
   int size[3];
   int nt[3];
   int tile;
+  int tile_shift;   // log2(tile) for a power of two, else -1
   const GRec* rec;  // non-null: pairs only where the records' extent boxes meet the tile
 };
+
+inline int tile_shift_of(int tile) {
+  int sh = 0;
+  while ((1 << sh) < tile) ++sh;
+  return (1 << sh) == tile ? sh : -1;
+}
 
 // float -> int32 conversion with truncation, saturation and NaN -> 0 (the
 // semantics of a device-side float->int cast, which the reference's GPU path
@@ -259,9 +266,20 @@ __device__ __forceinline__ int sat_f2i(float x) {
   return (int)x;
 }
 
-// Tile box of one Gaussian; gaussian/tile_voxelizer.py:97-116.
-__device__ __forceinline__ void gauss_tile_box(const float* __restrict__ center, const float* __restrict__ radius,
-                                               int g, const BinGeom& geo, int tmin[3], int tcnt[3]) {
+// Non-negative x / tile (a shift for power-of-two tiles).
+__device__ __forceinline__ int tile_of(int x, const BinGeom& geo) {
+  return geo.tile_shift >= 0 ? (x >> geo.tile_shift) : x / geo.tile;
+}
+
+// Tile box of one Gaussian; gaussian/tile_voxelizer.py:97-116.  With tc / te
+// (centre and extents of the Gaussian's record), the box is cut to the tiles
+// that the extent box [ceil(c - e), floor(c + e)] meets: the tile kernels skip
+// every other pair of the tile lists (the box test of stage_pair, with the same
+// float bounds), so dropping them changes no result; a NaN bound restricts
+// nothing, as there.
+__device__ __forceinline__ void tile_box(const float* __restrict__ center, const float* __restrict__ radius, int g,
+                                         const BinGeom& geo, const float* tc, const float* te, int tmin[3],
+                                         int tcnt[3]) {
   const float r = radius[g];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -270,29 +288,34 @@ __device__ __forceinline__ void gauss_tile_box(const float* __restrict__ center,
     int hi = sat_f2i(ceilf(c + r));
     lo = min(max(lo, 0), geo.size[a] - 1);
     hi = min(max(hi, 0), geo.size[a] - 1);
-    int tl = min(max(lo / geo.tile, 0), geo.nt[a] - 1);
-    int th = min(max(hi / geo.tile, 0), geo.nt[a] - 1);
-    if (geo.rec != nullptr) {
-      // Tiles that the extent box [ceil(c - e), floor(c + e)] of the record
-      // meets.  The tile kernels skip every other pair of the tile lists (the
-      // box test of stage_pair, with the same float bounds), so dropping them
-      // here changes no result; a NaN bound restricts nothing, as there.
-      const GRec& rg = geo.rec[g];
-      const float cc = a == 0 ? rg.c0 : (a == 1 ? rg.c1 : rg.c2);
-      const float ee = a == 0 ? rg.e0 : (a == 1 ? rg.e1 : rg.e2);
-      float blo = ceilf(cc - ee), bhi = floorf(cc + ee);
+    int tl = min(tile_of(lo, geo), geo.nt[a] - 1);
+    int th = min(tile_of(hi, geo), geo.nt[a] - 1);
+    if (tc != nullptr) {
+      float blo = ceilf(tc[a] - te[a]), bhi = floorf(tc[a] + te[a]);
       if (!(blo == blo)) blo = -INFINITY;
       if (!(bhi == bhi)) bhi = INFINITY;
       const float last = (float)(geo.size[a] - 1);
       if (blo > bhi || blo > last || bhi < 0.0f) {
         th = tl - 1;
       } else {
-        if (blo > 0.0f) tl = max(tl, (int)blo / geo.tile);
-        if (bhi < last) th = min(th, (int)bhi / geo.tile);
+        if (blo > 0.0f) tl = max(tl, tile_of((int)blo, geo));
+        if (bhi < last) th = min(th, tile_of((int)bhi, geo));
       }
     }
     tmin[a] = tl;
     tcnt[a] = max(th - tl + 1, 0);  // a negative radius can invert the box: no pairs
+  }
+}
+
+// tile_box of Gaussian g, cut to its record's extent box when geo.rec is set.
+__device__ __forceinline__ void gauss_tile_box(const float* __restrict__ center, const float* __restrict__ radius,
+                                               int g, const BinGeom& geo, int tmin[3], int tcnt[3]) {
+  if (geo.rec != nullptr) {
+    const GRec& rg = geo.rec[g];
+    const float tc[3] = {rg.c0, rg.c1, rg.c2}, te[3] = {rg.e0, rg.e1, rg.e2};
+    tile_box(center, radius, g, geo, tc, te, tmin, tcnt);
+  } else {
+    tile_box(center, radius, g, geo, nullptr, nullptr, tmin, tcnt);
   }
 }
 
@@ -804,11 +827,17 @@ __global__ void k_ranges(const int32_t* __restrict__ keys, int total, int32_t* _
 // gets an infinite extent (no culling).  Gaussian g's density is
 // rho_r[g * rho_stride], rho_i[g * rho_stride] (stride 2 with rho_i = rho_r + 1
 // reads an interleaved complex64 array).
+//
+// counts (optional): the Gaussian's pairs for binning cut to its extent box
+// (tile_box with this record; radius and geo as for k_bin_count), and
+// counts[n] = 0 for the scan.
 __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ center, const float* __restrict__ cov,
                                               const float* __restrict__ rho_r, const float* __restrict__ rho_i,
                                               int rho_stride, const float* __restrict__ half_shape, int n,
-                                              GRec* __restrict__ rec) {
+                                              GRec* __restrict__ rec, const float* __restrict__ radius,
+                                              BinGeom geo, int32_t* __restrict__ counts) {
   const int g = blockIdx.x * 256 + threadIdx.x;
+  if (counts != nullptr && g == n) counts[n] = 0;
   if (g >= n) return;
   const float hs0 = half_shape[0], hs1 = half_shape[1], hs2 = half_shape[2];
   const float* m = cov + 9 * (int64_t)g;
@@ -859,6 +888,12 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ center, 
   r.pad0 = 0.0f;
   r.pad1 = 0.0f;
   rec[g] = r;
+  if (counts != nullptr) {
+    const float tc[3] = {r.c0, r.c1, r.c2}, te[3] = {e0, e1, e2};
+    int tmin[3], tcnt[3];
+    tile_box(center, radius, g, geo, tc, te, tmin, tcnt);
+    counts[g] = tcnt[0] * tcnt[1] * tcnt[2];
+  }
 }
 
 // ------------------------------------------------ D extent over a column --
@@ -2072,6 +2107,7 @@ int bin_count_impl(const GRec* rec, const float* center_vox, const float* radius
   geo.size[1] = h;
   geo.size[2] = w;
   geo.tile = tile;
+  geo.tile_shift = tile_shift_of(tile);
   hipStream_t s = static_cast<hipStream_t>(stream);
   Carve cv(workspace);
   int64_t* sums = cv.take<int64_t>((size_t)ceil_div_l((int64_t)n + 1, kScanTile) + 1);
@@ -2114,6 +2150,7 @@ int bin_fill_impl(const GRec* rec, const float* center_vox, const float* radius_
   geo.size[1] = h;
   geo.size[2] = w;
   geo.tile = tile;
+  geo.tile_shift = tile_shift_of(tile);
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int total = (int)total_pairs;
   const int passes = radix_passes(num_tiles);
@@ -2143,8 +2180,6 @@ int bin_fill_impl(const GRec* rec, const float* center_vox, const float* radius_
     return GSTV_OK;
   }
 
-  hipLaunchKernelGGL(k_ranges_preset, dim3(ceil_div_i(num_tiles, 256)), dim3(256), 0, s, tile_ranges, num_tiles);
-  GSTV_LAUNCH_CHECK("ranges_preset");
   hipLaunchKernelGGL(k_block_first, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, gauss_offsets, n, total, first);
   GSTV_LAUNCH_CHECK("block_first");
   const int bits_total = key_bits(num_tiles);
@@ -2176,6 +2211,13 @@ int bin_fill_impl(const GRec* rec, const float* center_vox, const float* radius_
       hipLaunchKernelGGL(k_radix_hist<false>, dim3(nblk), dim3(kRadixThreads), 0, s, src, total, shift, bits, nblk,
                          hist);
     GSTV_LAUNCH_CHECK("radix_hist");
+    if (p == 0) {
+      // Needed only by the last pass; launched once the first long kernel is
+      // queued (the host launches while the GPU works).
+      hipLaunchKernelGGL(k_ranges_preset, dim3(ceil_div_i(num_tiles, 256)), dim3(256), 0, s, tile_ranges,
+                         num_tiles);
+      GSTV_LAUNCH_CHECK("ranges_preset");
+    }
     const int64_t hist_n = (int64_t)(1 << bits) * nblk;
     int st = launch_exclusive_scan(hist, hist_n, hist, sums, nullptr, s);
     if (st != GSTV_OK) return st;
@@ -2223,10 +2265,13 @@ int check_tile_args(const int32_t* tile_ranges, int32_t num_tiles, int32_t n, in
 }
 
 int launch_prep(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
-                int rho_stride, const float* half_shape, int32_t n, GRec* rec, hipStream_t s) {
-  if (n == 0) return GSTV_OK;
-  hipLaunchKernelGGL(k_prep, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, center_vox, cov_inv, rho_r, rho_i,
-                     rho_stride, half_shape, n, rec);
+                int rho_stride, const float* half_shape, int32_t n, GRec* rec, hipStream_t s,
+                const float* radius = nullptr, const BinGeom* geo = nullptr, int32_t* counts = nullptr) {
+  if (n == 0 && counts == nullptr) return GSTV_OK;
+  BinGeom g0{};
+  hipLaunchKernelGGL(k_prep, dim3(ceil_div_i(counts != nullptr ? n + 1 : n, 256)), dim3(256), 0, s, center_vox,
+                     cov_inv, rho_r, rho_i, rho_stride, half_shape, n, rec, radius, geo != nullptr ? *geo : g0,
+                     counts);
   GSTV_LAUNCH_CHECK("prep");
   return GSTV_OK;
 }
@@ -2283,12 +2328,14 @@ void launch_backward(const GRec* rec, float* gacc, const float* cov_inv, const f
 struct KeepLayout {
   GRec* rec;
   float* gacc;
+  int64_t* sums;  // block sums of the pair-count scan
 };
 size_t keep_layout(void* base, int32_t n, KeepLayout* out) {
   Carve cv(base);
   GRec* rec = cv.take<GRec>((size_t)n);
   float* gacc = cv.take<float>((size_t)n * kAccStride);
-  if (out != nullptr) *out = KeepLayout{rec, gacc};
+  int64_t* sums = cv.take<int64_t>((size_t)ceil_div_l((int64_t)n + 1, kScanTile) + 1);
+  if (out != nullptr) *out = KeepLayout{rec, gacc, sums};
   return align_up(cv.off, 256);
 }
 }  // namespace
@@ -2352,19 +2399,36 @@ size_t gstv_keep_workspace_bytes(int32_t n) {
   return keep_layout(nullptr, n, nullptr);
 }
 
-int gstv_keep_prepare(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
-                      int32_t rho_stride, const float* half_shape, int32_t n, void* keep_workspace,
-                      size_t keep_workspace_bytes, void* stream) {
-  if (n < 0 || rho_stride < 1) return GSTV_ERR_INVALID_ARGUMENT;
-  if (n == 0) return GSTV_OK;
-  if (center_vox == nullptr || cov_inv == nullptr || rho_r == nullptr || rho_i == nullptr || half_shape == nullptr)
+int gstv_keep_prepare(const float* center_vox, const float* radius_vox, const float* cov_inv, const float* rho_r,
+                      const float* rho_i, int32_t rho_stride, const float* half_shape, int32_t n, int32_t d,
+                      int32_t h, int32_t w, int32_t tile, int32_t* gauss_offsets, int64_t* total_pairs,
+                      void* keep_workspace, size_t keep_workspace_bytes, void* stream) {
+  if (n < 0 || rho_stride < 1 || ((gauss_offsets == nullptr) != (total_pairs == nullptr)))
     return GSTV_ERR_INVALID_ARGUMENT;
-  if (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n))
+  const bool count = gauss_offsets != nullptr;
+  BinGeom geo{};
+  if (count) {
+    int64_t num_tiles;
+    if (!tile_geometry(d, h, w, tile, geo.nt, &num_tiles)) return GSTV_ERR_INVALID_ARGUMENT;
+    geo.size[0] = d;
+    geo.size[1] = h;
+    geo.size[2] = w;
+    geo.tile = tile;
+    geo.tile_shift = tile_shift_of(tile);
+    if (n > 0 && radius_vox == nullptr) return GSTV_ERR_INVALID_ARGUMENT;
+  }
+  if (n > 0 && (center_vox == nullptr || cov_inv == nullptr || rho_r == nullptr || rho_i == nullptr ||
+                half_shape == nullptr))
+    return GSTV_ERR_INVALID_ARGUMENT;
+  if ((n > 0 || count) && (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n)))
     return GSTV_ERR_WORKSPACE_TOO_SMALL;
   hipStream_t s = static_cast<hipStream_t>(stream);
   KeepLayout ws;
   keep_layout(keep_workspace, n, &ws);
-  return launch_prep(center_vox, cov_inv, rho_r, rho_i, rho_stride, half_shape, n, ws.rec, s);
+  int st = launch_prep(center_vox, cov_inv, rho_r, rho_i, rho_stride, half_shape, n, ws.rec, s, radius_vox, &geo,
+                       count ? gauss_offsets : nullptr);
+  if (st != GSTV_OK || !count) return st;
+  return launch_exclusive_scan(gauss_offsets, (int64_t)n + 1, gauss_offsets, ws.sums, total_pairs, s);
 }
 
 int gstv_keep_clear(void* keep_workspace, size_t keep_workspace_bytes, int32_t n, void* stream) {
@@ -2377,19 +2441,6 @@ int gstv_keep_clear(void* keep_workspace, size_t keep_workspace_bytes, int32_t n
   launch_clear_acc(ws.gacc, n, static_cast<hipStream_t>(stream));
   GSTV_LAUNCH_CHECK("keep_clear");
   return GSTV_OK;
-}
-
-int gstv_keep_bin_count(const void* keep_workspace, size_t keep_workspace_bytes, const float* center_vox,
-                        const float* radius_vox, int32_t n, int32_t d, int32_t h, int32_t w, int32_t tile,
-                        int32_t* gauss_offsets, int64_t* total_pairs, void* workspace, size_t workspace_bytes,
-                        void* stream) {
-  if (n < 0) return GSTV_ERR_INVALID_ARGUMENT;
-  if (n > 0 && (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n)))
-    return GSTV_ERR_WORKSPACE_TOO_SMALL;
-  KeepLayout ws{nullptr, nullptr};
-  if (n > 0) keep_layout(const_cast<void*>(keep_workspace), n, &ws);
-  return bin_count_impl(ws.rec, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, total_pairs, workspace,
-                        workspace_bytes, stream);
 }
 
 int gstv_keep_bin_fill(const void* keep_workspace, size_t keep_workspace_bytes, const float* center_vox,

@@ -128,16 +128,18 @@ class _TileVoxelizeFusedHIP(torch.autograd.Function):
         # kernels would skip).  Binning reads the pair total back to the host
         # once; the adjoint's accumulator is cleared behind that read, while
         # the host waits and then sizes the pair arrays.
-        kept = _native.keep_prepare(center, cov, dv[:, 0], dv[:, 1], half_shape)
-        offsets, total = _native.bin_count(center, radius, volume_shape, tile_size, kept=kept)
+        kept = _native.keep_prepare(center, cov, dv[:, 0], dv[:, 1], half_shape, radius, volume_shape, tile_size)
         total_host = torch.empty(1, dtype=torch.int64, pin_memory=True)
-        total_host.copy_(total, non_blocking=True)
+        total_host.copy_(kept.total, non_blocking=True)
         total_ready = torch.cuda.Event()
         total_ready.record()
         _native.keep_clear(kept)
+        tile_ranges = torch.empty(_native.num_tiles_for(volume_shape, tile_size), 2, dtype=torch.int32,
+                                  device=center.device)
+        total_np = total_host.numpy()
         total_ready.synchronize()
-        tile_ranges, pair_ids = _native.bin_fill(center, radius, volume_shape, tile_size, offsets,
-                                                 int(total_host[0]), kept=kept)
+        tile_ranges, pair_ids = _native.bin_fill(center, radius, volume_shape, tile_size, kept.offsets,
+                                                 int(total_np[0]), kept=kept, tile_ranges=tile_ranges)
         # The forward keeps its records and per-pair column rows for the
         # adjoint of this call, which then rebuilds neither.
         vol_r, vol_i, ctx.kept = _native.forward(center, cov, dv[:, 0], dv[:, 1], tile_ranges, pair_ids,
