@@ -1162,14 +1162,39 @@ __device__ __forceinline__ void batch_masks(Stage& sh, const BlockCtx& b, int bs
 // Tiles of at most 2 sub-bricks per axis test one bit of the staged mask.
 __device__ __forceinline__ int build_list(Stage& sh, int cnt, const BlockCtx& b, int wave, int lane, bool use_mask) {
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int sbi = (b.s0 >> 2) * 4 + (b.s1 >> 2) * 2 + (b.s2 >> 2);
   int nhit = 0;
+  if (use_mask) {
+    // The branch is uniform; kept outside the loop so that the loop holds
+    // only the bit test.
+    const int sbi = (b.s0 >> 2) * 4 + (b.s1 >> 2) * 2 + (b.s2 >> 2);
+    for (int base = 0; base < cnt; base += 256) {
+      // Lane l tests pairs base + 4l .. base + 4l + 3 from one 32-bit read;
+      // their list slots follow those of the lower lanes (ascending pair order).
+      const int k0 = base + 4 * lane;
+      const uint32_t w4 = k0 < cnt ? *reinterpret_cast<const uint32_t*>(&sh.sbm[k0]) : 0u;
+      bool h[4];
+      int before = 0, total = 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        h[j] = k0 + j < cnt && ((w4 >> (8 * j + sbi)) & 1);
+        const uint64_t m = __ballot(h[j]);
+        before += __popcll(m & lt_mask);
+        total += __popcll(m);
+      }
+      int pos = nhit + before;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (h[j]) sh.list[wave][pos] = (uint16_t)(k0 + j);
+        pos += h[j] ? 1 : 0;
+      }
+      nhit += total;
+    }
+    return nhit;
+  }
   for (int base = 0; base < cnt; base += 64) {
     const int k = base + lane;
     bool hit = false;
-    if (k < cnt && use_mask) {
-      hit = (sh.sbm[k] >> sbi) & 1;
-    } else if (k < cnt) {
+    if (k < cnt) {
       const uint32_t bx = sh.box[k];
       const int lo0 = bx & 31, lo1 = (bx >> 5) & 31, lo2 = (bx >> 10) & 31;
       const int hi0 = (bx >> 15) & 31, hi1 = (bx >> 20) & 31, hi2 = (bx >> 25) & 31;

@@ -145,17 +145,17 @@ int gstv_backward(const float* center_vox, const float* cov_inv,
  * calls share one caller-allocated keep workspace
  * (gstv_keep_workspace_bytes(n) bytes; keep it, unchanged, between them):
  *   gstv_keep_prepare   builds the Gaussian records that gstv_forward and
- *                       gstv_backward build internally.
- *   gstv_keep_bin_count, gstv_keep_bin_fill
- *                       gstv_bin_count / gstv_bin_fill restricted to the pairs
- *                       whose tile meets the Gaussian's extent box (the box of
- *                       its q <= 9 ellipsoid in the records, widened so that
- *                       no voxel with fp32 q <= 9 lies outside): the tile
- *                       kernels skip every other pair of the reference's
- *                       lists, so the forward on these lists is bit-identical
- *                       and the adjoint equal up to the order of fp32 atomic
- *                       additions.  Same order within each tile (ascending
- *                       Gaussian id).  Needs the records.
+ *                       gstv_backward build internally and, when
+ *                       gauss_offsets / total_pairs are given, does the work
+ *                       of gstv_bin_count restricted to the pairs whose tile
+ *                       meets the Gaussian's extent box (the box of its q <= 9
+ *                       ellipsoid in the record, widened so that no voxel with
+ *                       fp32 q <= 9 lies outside).  The tile kernels skip every
+ *                       other pair of the reference's lists, so the forward on
+ *                       such lists is bit-identical and the adjoint equal up to
+ *                       the order of fp32 atomic additions.
+ *   gstv_keep_bin_fill  gstv_bin_fill for those counts (same order within each
+ *                       tile: ascending Gaussian id).
  *   gstv_keep_clear     zeroes the adjoint's moment accumulator.  Neither it
  *                       nor the records need binning, so a caller can queue
  *                       them before reading the pair total back to the host.
@@ -179,18 +179,14 @@ int gstv_backward(const float* center_vox, const float* cov_inv,
  * interleaved complex64 density and its gradient. */
 size_t gstv_keep_workspace_bytes(int32_t n);
 
-int gstv_keep_prepare(const float* center_vox, const float* cov_inv,
+int gstv_keep_prepare(const float* center_vox, const float* radius_vox, const float* cov_inv,
                       const float* rho_r, const float* rho_i, int32_t rho_stride,
                       const float* half_shape, int32_t n,
+                      int32_t d, int32_t h, int32_t w, int32_t tile,
+                      int32_t* gauss_offsets, int64_t* total_pairs,
                       void* keep_workspace, size_t keep_workspace_bytes, void* stream);
 
 int gstv_keep_clear(void* keep_workspace, size_t keep_workspace_bytes, int32_t n, void* stream);
-
-int gstv_keep_bin_count(const void* keep_workspace, size_t keep_workspace_bytes,
-                        const float* center_vox, const float* radius_vox, int32_t n,
-                        int32_t d, int32_t h, int32_t w, int32_t tile,
-                        int32_t* gauss_offsets, int64_t* total_pairs,
-                        void* workspace, size_t workspace_bytes, void* stream);
 
 int gstv_keep_bin_fill(const void* keep_workspace, size_t keep_workspace_bytes,
                        const float* center_vox, const float* radius_vox, int32_t n,

@@ -206,10 +206,10 @@ def test_keep_binning_is_the_touching_pairs(name, make):
     ranges, pids = _bin_gpu(center, radius, c)
     rr = torch.from_numpy(c["rr"]).to(DEV)
     ri = torch.from_numpy(c["ri"]).to(DEV)
-    kept = _native.keep_prepare(center, cov, rr, ri, hs)
+    kept = _native.keep_prepare(center, cov, rr, ri, hs, radius, c["shape"], tile)
     _native.keep_clear(kept)
-    offs, total = _native.bin_count(center, radius, c["shape"], tile, kept=kept)
-    t_ranges, t_pids = _native.bin_fill(center, radius, c["shape"], tile, offs, int(total.item()), kept=kept)
+    t_ranges, t_pids = _native.bin_fill(center, radius, c["shape"], tile, kept.offsets, int(kept.total.item()),
+                                        kept=kept)
     n = center.shape[0]
     rec = kept.ws[: n * 64].view(torch.float32).view(n, 16).cpu().numpy()
     e_ranges, e_pids = _touching_lists(ranges, pids, rec, c["shape"], tile)
