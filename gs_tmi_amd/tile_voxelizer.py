@@ -129,10 +129,12 @@ class _TileVoxelizeFusedHIP(torch.autograd.Function):
         # once; the adjoint's accumulator is cleared behind that read, while
         # the host waits and then sizes the pair arrays.
         kept = _native.keep_prepare(center, cov, dv[:, 0], dv[:, 1], half_shape, radius, volume_shape, tile_size)
+        stream = torch.cuda.current_stream(center.device)
         total_host = torch.empty(1, dtype=torch.int64, pin_memory=True)
-        total_host.copy_(kept.total, non_blocking=True)
+        with torch.cuda.stream(stream):
+            total_host.copy_(kept.total, non_blocking=True)
         total_ready = torch.cuda.Event()
-        total_ready.record()
+        total_ready.record(stream)
         _native.keep_clear(kept)
         tile_ranges = torch.empty(_native.num_tiles_for(volume_shape, tile_size), 2, dtype=torch.int32,
                                   device=center.device)
