@@ -463,7 +463,7 @@ __global__ void k_block_first(const int32_t* __restrict__ offsets, int n, int to
 // (uniformly) when more than kGenCap Gaussians span the block, or when the
 // volume has more than 2^20 tiles (the bound of gen_pair's divisions);
 // callers then search per slot.
-__device__ bool gen_setup(GenLds& L, const float* __restrict__ center, const float* __restrict__ radius,
+__device__ __forceinline__ bool gen_setup(GenLds& L, const float* __restrict__ center, const float* __restrict__ radius,
                           const BinGeom& geo, const int32_t* __restrict__ offsets,
                           const int32_t* __restrict__ first, int j0, int j1) {
   const int b = j0 / kRadixTile;
