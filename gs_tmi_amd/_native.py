@@ -55,6 +55,9 @@ PROTOTYPES = {
     "gstv_backward_keep": (ctypes.c_int,
                            [_c_p, _sz, _c_p, _c_p, _c_p, _i32, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32, _i32,
                             _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p]),
+    "gstv_backward_keep_params": (ctypes.c_int,
+                                  [_c_p, _sz, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _i32, _c_p, _c_p, _i32, _i32,
+                                   _i32, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p]),
     "gstv_count_evaluations": (ctypes.c_int,
                                [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32,
                                 _c_p, _c_p, _sz, _c_p]),
@@ -112,6 +115,12 @@ def _check(status: int, what: str):
 
 def _ptr(t: torch.Tensor | None) -> int | None:
     return None if t is None else t.data_ptr()
+
+
+def aligned16(t: torch.Tensor) -> torch.Tensor:
+    """``t`` itself when its data starts on a 16-byte boundary, else a copy
+    (the kernels read quaternions as 16-byte vectors)."""
+    return t if t.data_ptr() % 16 == 0 else t.clone()
 
 
 def _stream(device: torch.device) -> int:
@@ -341,6 +350,37 @@ def backward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shap
     return g_c, g_m, g_rr, g_ri
 
 
+def backward_params(center_vox, cov_inv, scales, rotations, tile_ranges, pair_ids, half_shape, grad_r, grad_i,
+                    tile_size, d, h, w, kept, grad_rho):
+    """Adjoint of the fused call down to the parameters (gstv_backward_keep_params).
+    ``kept`` from ``forward(..., keep=...)``; ``grad_rho`` an fp32 [n, 2] array
+    for the interleaved density gradient.  Returns (grad_positions [n,3],
+    grad_scales [n,3], grad_rotations [n,4]); rotations must be 16-byte aligned."""
+    lib = get_lib()
+    _require_gpu(center_vox, "center_vox")
+    dev = center_vox.device
+    n = center_vox.shape[0]
+    if kept.stride != 2 or grad_rho.shape != (n, 2) or grad_rho.dtype != torch.float32 or \
+            not grad_rho.is_contiguous():
+        raise ValueError("backward_params needs an interleaved density and a contiguous fp32 [n, 2] grad_rho")
+    gp = torch.empty(n, 3, dtype=torch.float32, device=dev)
+    gs = torch.empty(n, 3, dtype=torch.float32, device=dev)
+    gq = torch.empty(n, 4, dtype=torch.float32, device=dev)
+    if not kept.fresh:
+        keep_clear(kept)
+    g_rr, g_ri = grad_rho[:, 0], grad_rho[:, 1]
+    with torch.cuda.device(dev):
+        st = lib.gstv_backward_keep_params(_ptr(kept.ws), kept.ws.numel(), _ptr(cov_inv), _ptr(half_shape),
+                                           _ptr(scales), _ptr(aligned16(rotations)), _ptr(tile_ranges),
+                                           tile_ranges.shape[0],
+                                           _ptr(pair_ids), n, _ptr(grad_r), _ptr(grad_i), tile_size, d, h, w,
+                                           _ptr(gp), _ptr(gs), _ptr(gq), _ptr(g_rr), _ptr(g_ri), 2,
+                                           _ptr(kept.cols), _stream(dev))
+    kept.fresh = False
+    _check(st, "gstv_backward_keep_params")
+    return gp, gs, gq
+
+
 def count_evaluations(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape, tile_size, d, h, w):
     """Exact (E_total, E_active, E_culled) of the tile kernels for this binning (synchronises)."""
     lib = get_lib()
@@ -377,7 +417,7 @@ def prologue(positions, scales, rotations, shape):
     dev = positions.device
     p = positions.detach().contiguous().float()
     s = scales.detach().contiguous().float()
-    q = rotations.detach().contiguous().float()
+    q = aligned16(rotations.detach().contiguous().float())
     n = p.shape[0]
     center = torch.empty(n, 3, dtype=torch.float32, device=dev)
     radius = torch.empty(n, dtype=torch.float32, device=dev)
@@ -401,7 +441,7 @@ def epilogue(scales, rotations, grad_center, grad_cov, shape):
     gq = torch.empty(n, 4, dtype=torch.float32, device=dev)
     d, h, w = shape
     with torch.cuda.device(dev):
-        st = lib.gstv_epilogue(_ptr(scales), _ptr(rotations), _ptr(grad_center.contiguous()),
+        st = lib.gstv_epilogue(_ptr(scales), _ptr(aligned16(rotations)), _ptr(grad_center.contiguous()),
                                _ptr(grad_cov.contiguous()), n, d, h, w, _ptr(gp), _ptr(gs), _ptr(gq), _stream(dev))
     _check(st, "gstv_epilogue")
     return gp, gs, gq

@@ -768,7 +768,7 @@ __device__ __forceinline__ void radix_scatter(const PairSource& src, int total, 
   }
 }
 
-// First pass (pairs generated in the kernel; LDS limits it to 2 blocks per CU).
+// First pass (pairs generated in the kernel; under 40 KB of LDS, 4 blocks per CU).
 template <bool kLast>
 __global__ __launch_bounds__(kRadixThreads, 4) void k_radix_scatter_gen(PairSource src, int total, int shift, int bits,
                                                                     int nblocks,
@@ -1144,7 +1144,7 @@ __device__ __forceinline__ void append_touch(Stage& sh, bool touch, int k, int l
 }
 
 // Sub-brick masks of the batch's touching pairs, spread over the block's
-// threads (the non-touching ~half of the pairs costs nothing here), and, when
+// threads (pairs that do not touch the tile cost nothing here), and, when
 // pair_cols is given, their column words at the pairs' positions bs + k.
 // Called between two barriers.
 __device__ __forceinline__ void batch_masks(Stage& sh, const BlockCtx& b, int bs, uint32_t* __restrict__ pair_cols) {
@@ -1796,17 +1796,13 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
 // Moments -> reference outputs (gaussian/cuda_ext/tile_voxelize_cuda_kernel.cu:237-252):
 //   s = -w t / 2;  grad_rho = sum w g;  grad_c_i = -2/hs_i sum_j M_ij sum(s u_j)/hs_j;
 //   grad_M_ij = sum(s u_i u_j) / (hs_i hs_j).
-__global__ __launch_bounds__(256) void k_finalize(const float* __restrict__ gacc, const float* __restrict__ cov,
-                                                  const float* __restrict__ half_shape, int n,
-                                                  float* __restrict__ g_center, float* __restrict__ g_cov,
-                                                  float* __restrict__ g_rr, float* __restrict__ g_ri,
-                                                  int rho_stride) {
-  const int g = blockIdx.x * 256 + threadIdx.x;
-  if (g >= n) return;
+// One Gaussian's moments -> grad_center (gc), grad_cov_inv (gm), grad_rho.
+__device__ __forceinline__ void finalize_one(const float* __restrict__ gacc, const float* __restrict__ cov,
+                                             const float hs[3], int g, float gc[3], float gm[9], float& grr,
+                                             float& gri) {
   const float* a = gacc + (int64_t)g * kAccStride;
-  const float hs[3] = {half_shape[0], half_shape[1], half_shape[2]};
-  g_rr[(int64_t)g * rho_stride] = a[0];
-  g_ri[(int64_t)g * rho_stride] = a[1];
+  grr = a[0];
+  gri = a[1];
   const float m[3] = {-0.5f * a[2], -0.5f * a[3], -0.5f * a[4]};
   const float mm[3][3] = {{-0.5f * a[5], -0.5f * a[8], -0.5f * a[9]},
                           {-0.5f * a[8], -0.5f * a[6], -0.5f * a[10]},
@@ -1817,10 +1813,28 @@ __global__ __launch_bounds__(256) void k_finalize(const float* __restrict__ gacc
     float acc = 0.0f;
 #pragma unroll
     for (int j = 0; j < 3; ++j) acc = fmaf(M[3 * i + j], m[j] / hs[j], acc);
-    g_center[3 * (int64_t)g + i] = -2.0f * acc / hs[i];
+    gc[i] = -2.0f * acc / hs[i];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) g_cov[9 * (int64_t)g + 3 * i + j] = mm[i][j] / (hs[i] * hs[j]);
+    for (int j = 0; j < 3; ++j) gm[3 * i + j] = mm[i][j] / (hs[i] * hs[j]);
   }
+}
+
+__global__ __launch_bounds__(256) void k_finalize(const float* __restrict__ gacc, const float* __restrict__ cov,
+                                                  const float* __restrict__ half_shape, int n,
+                                                  float* __restrict__ g_center, float* __restrict__ g_cov,
+                                                  float* __restrict__ g_rr, float* __restrict__ g_ri,
+                                                  int rho_stride) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= n) return;
+  const float hs[3] = {half_shape[0], half_shape[1], half_shape[2]};
+  float gc[3], gm[9], grr, gri;
+  finalize_one(gacc, cov, hs, g, gc, gm, grr, gri);
+  g_rr[(int64_t)g * rho_stride] = grr;
+  g_ri[(int64_t)g * rho_stride] = gri;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) g_center[3 * (int64_t)g + i] = gc[i];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) g_cov[9 * (int64_t)g + i] = gm[i];
 }
 
 // ------------------------------------------------------ prologue / epilogue --
@@ -1876,16 +1890,16 @@ __global__ __launch_bounds__(256) void k_prologue(const float* __restrict__ pos,
 // dL/dsi_k = sum_i dL/dT_ik R_ik;  dL/ds_k = -si_k^2 dL/dsi_k;  dL/dq from
 // dL/dR through R(q^) and the normalisation q^ = q / (|q| + 1e-8).  The
 // radius carries no gradient (the reference computes it under no_grad).
-__global__ __launch_bounds__(256) void k_epilogue(const float* __restrict__ scales, const float* __restrict__ rot,
-                                                  const float* __restrict__ g_center, const float* __restrict__ g_cov,
-                                                  int n, float sh0, float sh1, float sh2, float* __restrict__ g_pos,
-                                                  float* __restrict__ g_scales, float* __restrict__ g_rot) {
-  const int g = blockIdx.x * 256 + threadIdx.x;
-  if (g >= n) return;
+// One Gaussian's chain rule through the prologue, from its grad_center (gc)
+// and grad_cov_inv (gm).
+__device__ __forceinline__ void epilogue_one(const float* __restrict__ scales, const float* __restrict__ rot, int g,
+                                             const float gc[3], const float gm[9], float sh0, float sh1, float sh2,
+                                             float* __restrict__ g_pos, float* __restrict__ g_scales,
+                                             float* __restrict__ g_rot) {
   const int64_t g3 = 3 * (int64_t)g;
-  g_pos[g3 + 0] = g_center[g3 + 0] * (0.5f * sh0);
-  g_pos[g3 + 1] = g_center[g3 + 1] * (0.5f * sh1);
-  g_pos[g3 + 2] = g_center[g3 + 2] * (0.5f * sh2);
+  g_pos[g3 + 0] = gc[0] * (0.5f * sh0);
+  g_pos[g3 + 1] = gc[1] * (0.5f * sh1);
+  g_pos[g3 + 2] = gc[2] * (0.5f * sh2);
   const float s[3] = {scales[g3 + 0], scales[g3 + 1], scales[g3 + 2]};
   const float4 q4 = *reinterpret_cast<const float4*>(rot + 4 * (int64_t)g);
   const float nr = sqrtf(q4.x * q4.x + q4.y * q4.y + q4.z * q4.z + q4.w * q4.w);
@@ -1901,7 +1915,7 @@ __global__ __launch_bounds__(256) void k_epilogue(const float* __restrict__ scal
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       T[i][k] = R[i][k] * si[k];
-      G[i][k] = g_cov[9 * (int64_t)g + 3 * i + k];
+      G[i][k] = gm[3 * i + k];
     }
   float GR[3][3], gsi[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -1932,6 +1946,43 @@ __global__ __launch_bounds__(256) void k_epilogue(const float* __restrict__ scal
   out.z = gy / nrm - q4.z * c2;
   out.w = gz / nrm - q4.w * c2;
   *reinterpret_cast<float4*>(g_rot + 4 * (int64_t)g) = out;
+}
+
+
+__global__ __launch_bounds__(256) void k_epilogue(const float* __restrict__ scales, const float* __restrict__ rot,
+                                                  const float* __restrict__ g_center, const float* __restrict__ g_cov,
+                                                  int n, float sh0, float sh1, float sh2, float* __restrict__ g_pos,
+                                                  float* __restrict__ g_scales, float* __restrict__ g_rot) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= n) return;
+  float gc[3], gm[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) gc[i] = g_center[3 * (int64_t)g + i];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) gm[i] = g_cov[9 * (int64_t)g + i];
+  epilogue_one(scales, rot, g, gc, gm, sh0, sh1, sh2, g_pos, g_scales, g_rot);
+}
+
+// k_finalize and k_epilogue in one pass: the adjoint of the fused call from
+// the moments straight to the parameters' gradients (grad_center and
+// grad_cov_inv stay in registers).
+__global__ __launch_bounds__(256) void k_finalize_epilogue(const float* __restrict__ gacc,
+                                                           const float* __restrict__ cov,
+                                                           const float* __restrict__ half_shape,
+                                                           const float* __restrict__ scales,
+                                                           const float* __restrict__ rot, int n, float sh0,
+                                                           float sh1, float sh2, float* __restrict__ g_pos,
+                                                           float* __restrict__ g_scales, float* __restrict__ g_rot,
+                                                           float* __restrict__ g_rr, float* __restrict__ g_ri,
+                                                           int rho_stride) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= n) return;
+  const float hs[3] = {half_shape[0], half_shape[1], half_shape[2]};
+  float gc[3], gm[9], grr, gri;
+  finalize_one(gacc, cov, hs, g, gc, gm, grr, gri);
+  g_rr[(int64_t)g * rho_stride] = grr;
+  g_ri[(int64_t)g * rho_stride] = gri;
+  epilogue_one(scales, rot, g, gc, gm, sh0, sh1, sh2, g_pos, g_scales, g_rot);
 }
 
 // Exact evaluation counts for the roofline accounting: the forward kernel's
@@ -2322,9 +2373,10 @@ void launch_clear_acc(float* gacc, int32_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)ceil_div_l(nacc, 256)), dim3(256), 0, s, gacc, nacc, 0.0f);
 }
 
-// Backward tile kernel into a zeroed moment accumulator, then the moments ->
-// gradients pass.  pair_columns: the forward's column words (tiles of at most
-// 8 voxels per axis), or null to compute the column bounds here.
+// Backward tile kernel into a zeroed moment accumulator, then (when
+// grad_center is given) the moments -> gradients pass.  pair_columns: the
+// forward's column words (tiles of at most 8 voxels per axis), or null to
+// compute the column bounds here.
 void launch_backward(const GRec* rec, float* gacc, const float* cov_inv, const float* half_shape,
                      const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, int32_t n,
                      const float* grad_r, const float* grad_i, int32_t tile, int32_t d, int32_t h, int32_t w,
@@ -2345,8 +2397,9 @@ void launch_backward(const GRec* rec, float* gacc, const float* cov_inv, const f
     hipLaunchKernelGGL(k_backward<0>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, rec, tile_ranges,
                        pair_ids, geo_b, grad_r, grad_i, gacc, cols);
   prof_stop(1, slot, s);
-  hipLaunchKernelGGL(k_finalize, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, (const float*)gacc, cov_inv,
-                     half_shape, n, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, rho_stride);
+  if (grad_center != nullptr)
+    hipLaunchKernelGGL(k_finalize, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, (const float*)gacc, cov_inv,
+                       half_shape, n, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, rho_stride);
 }
 
 // Keep workspace: the records, then the 48-byte moment rows.
@@ -2522,6 +2575,39 @@ int gstv_backward_keep(void* keep_workspace, size_t keep_workspace_bytes, const 
   launch_backward(ws.rec, ws.gacc, cov_inv, half_shape, tile_ranges, num_tiles, pair_ids, n, grad_r, grad_i, tile, d,
                   h, w, nt, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, rho_stride, pair_columns, s);
   GSTV_LAUNCH_CHECK("backward_keep");
+  return GSTV_OK;
+}
+
+int gstv_backward_keep_params(void* keep_workspace, size_t keep_workspace_bytes, const float* cov_inv,
+                              const float* half_shape, const float* scales, const float* rotations,
+                              const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, int32_t n,
+                              const float* grad_r, const float* grad_i, int32_t tile, int32_t d, int32_t h,
+                              int32_t w, float* grad_positions, float* grad_scales, float* grad_rotations,
+                              float* grad_rho_r, float* grad_rho_i, int32_t rho_stride,
+                              const uint32_t* pair_columns, void* stream) {
+  int nt[3];
+  int st = check_tile_args(tile_ranges, num_tiles, n, tile, d, h, w, nt);
+  if (st != GSTV_OK) return st;
+  if (rho_stride < 1) return GSTV_ERR_INVALID_ARGUMENT;
+  if (n == 0) return GSTV_OK;
+  if (cov_inv == nullptr || pair_ids == nullptr || half_shape == nullptr || scales == nullptr ||
+      rotations == nullptr || grad_r == nullptr || grad_i == nullptr || grad_positions == nullptr ||
+      grad_scales == nullptr || grad_rotations == nullptr || grad_rho_r == nullptr || grad_rho_i == nullptr ||
+      (tile <= 8 && pair_columns == nullptr))
+    return GSTV_ERR_INVALID_ARGUMENT;
+  if (reinterpret_cast<uintptr_t>(rotations) % 16 != 0 || reinterpret_cast<uintptr_t>(grad_rotations) % 16 != 0)
+    return GSTV_ERR_INVALID_ARGUMENT;
+  if (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n))
+    return GSTV_ERR_WORKSPACE_TOO_SMALL;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  KeepLayout ws;
+  keep_layout(keep_workspace, n, &ws);
+  launch_backward(ws.rec, ws.gacc, cov_inv, half_shape, tile_ranges, num_tiles, pair_ids, n, grad_r, grad_i, tile, d,
+                  h, w, nt, nullptr, nullptr, nullptr, nullptr, rho_stride, pair_columns, s);
+  hipLaunchKernelGGL(k_finalize_epilogue, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, (const float*)ws.gacc,
+                     cov_inv, half_shape, scales, rotations, n, (float)d, (float)h, (float)w, grad_positions,
+                     grad_scales, grad_rotations, grad_rho_r, grad_rho_i, (int)rho_stride);
+  GSTV_LAUNCH_CHECK("backward_keep_params");
   return GSTV_OK;
 }
 

@@ -114,7 +114,7 @@ class _TileVoxelizeFusedHIP(torch.autograd.Function):
         _load_hip_ext(strict=True)
         d_size, h_size, w_size = volume_shape
         scales_c = scales.detach().contiguous().float()
-        rot_c = rotations.detach().contiguous().float()
+        rot_c = _native.aligned16(rotations.detach().contiguous().float())
         center, radius, cov, half_shape = _native.prologue(positions, scales_c, rot_c, volume_shape)
         # The kernels read the complex64 density interleaved, in place: no
         # real / imag copies here, and one complex gradient in backward.
@@ -163,11 +163,11 @@ class _TileVoxelizeFusedHIP(torch.autograd.Function):
         if grad_vol_i is None:
             grad_vol_i = torch.zeros(shape, dtype=torch.float32, device=center.device)
         g_dv = torch.empty_like(dv)
-        g_c, g_m, _, _ = _native.backward(
-            center, cov, dv[:, 0], dv[:, 1], tile_ranges, pair_ids, half_shape,
+        # Adjoint kernel, then moments -> parameter gradients in one pass.
+        g_p, g_s, g_q = _native.backward_params(
+            center, cov, scales_c, rot_c, tile_ranges, pair_ids, half_shape,
             grad_vol_r.contiguous().float().view(-1), grad_vol_i.contiguous().float().view(-1),
-            ctx.tile_size, d_size, h_size, w_size, kept=ctx.kept, grad_rho=g_dv)
-        g_p, g_s, g_q = _native.epilogue(scales_c, rot_c, g_c, g_m, shape)
+            ctx.tile_size, d_size, h_size, w_size, ctx.kept, g_dv)
         if ctx.density_dtype.is_complex:
             g_d = torch.view_as_complex(g_dv).to(ctx.density_dtype)
         else:

@@ -171,6 +171,12 @@ int gstv_backward(const float* center_vox, const float* cov_inv,
  *                       and pair_columns, for exactly the inputs and lists of
  *                       that forward.  It consumes the accumulator: a second
  *                       adjoint of the same forward needs gstv_keep_clear again.
+ *   gstv_backward_keep_params
+ *                       gstv_backward_keep followed by gstv_epilogue in one
+ *                       pass: gradients of positions, linear scales and
+ *                       rotations (as gstv_epilogue; rotations 16-byte
+ *                       aligned) instead of grad_center / grad_cov_inv, and
+ *                       the density gradient.
  * Results equal gstv_forward's bit for bit and gstv_backward's up to the order
  * of fp32 atomic additions.
  * rho_stride (>= 1) is the distance, in floats, between consecutive Gaussians'
@@ -208,6 +214,16 @@ int gstv_backward_keep(void* keep_workspace, size_t keep_workspace_bytes,
                        float* grad_center, float* grad_cov_inv,
                        float* grad_rho_r, float* grad_rho_i, int32_t rho_stride,
                        const uint32_t* pair_columns, void* stream);
+
+int gstv_backward_keep_params(void* keep_workspace, size_t keep_workspace_bytes,
+                              const float* cov_inv, const float* half_shape,
+                              const float* scales, const float* rotations,
+                              const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
+                              int32_t n, const float* grad_r, const float* grad_i,
+                              int32_t tile, int32_t d, int32_t h, int32_t w,
+                              float* grad_positions, float* grad_scales, float* grad_rotations,
+                              float* grad_rho_r, float* grad_rho_i, int32_t rho_stride,
+                              const uint32_t* pair_columns, void* stream);
 
 /* ------------------------------------------------------ prologue / epilogue --
  * Replaces the torch prologue of TileVoxelizer.forward (gaussian/tile_voxelizer.py:281-290,
