@@ -47,6 +47,8 @@ PROTOTYPES = {
     "gstv_keep_prepare": (ctypes.c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _i32, _i32, _i32, _i32, _i32,
                                          _c_p, _c_p, _c_p, _sz, _c_p]),
     "gstv_keep_clear": (ctypes.c_int, [_c_p, _sz, _i32, _c_p]),
+    "gstv_keep_prologue": (ctypes.c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32, _i32, _c_p,
+                                          _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _sz, _c_p]),
     "gstv_keep_bin_fill": (ctypes.c_int,
                            [_c_p, _sz, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32, _c_p, ctypes.c_int64, _c_p, _c_p,
                             _c_p, _sz, _c_p]),
@@ -252,6 +254,35 @@ def keep_prepare(center_vox, cov_inv, rho_r, rho_i, half_shape, radius_vox=None,
                                    _ptr(kept.total), _ptr(kept.ws), kept.ws.numel(), _stream(dev))
     _check(st, "gstv_keep_prepare")
     return kept
+
+
+def keep_prologue(positions, scales, rotations, rho_r, rho_i, shape, tile_size):
+    """The fused call's first pass (gstv_keep_prologue): prologue, Gaussian
+    records and the extent-box pair counts.  rho_r / rho_i as in
+    ``keep_prepare``.  Returns (center_vox, radius_vox, cov_inv, half_shape,
+    kept), ``kept.offsets`` / ``kept.total`` set."""
+    lib = get_lib()
+    _require_gpu(positions, "positions")
+    dev = positions.device
+    p = positions.detach().contiguous().float()
+    s = scales.detach().contiguous().float()
+    q = aligned16(rotations.detach().contiguous().float())
+    n = p.shape[0]
+    stride = _rho_stride(rho_r, rho_i, True)
+    center = torch.empty(n, 3, dtype=torch.float32, device=dev)
+    radius = torch.empty(n, dtype=torch.float32, device=dev)
+    cov = torch.empty(n, 3, 3, dtype=torch.float32, device=dev)
+    hs = torch.empty(3, dtype=torch.float32, device=dev)
+    kept = Kept(_workspace(lib.gstv_keep_workspace_bytes(n), dev), n, stride)
+    kept.offsets = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    kept.total = torch.empty(1, dtype=torch.int64, device=dev)
+    d, h, w = shape
+    with torch.cuda.device(dev):
+        st = lib.gstv_keep_prologue(_ptr(p), _ptr(s), _ptr(q), _ptr(rho_r), _ptr(rho_i), stride, n, d, h, w,
+                                    tile_size, _ptr(center), _ptr(radius), _ptr(cov), _ptr(hs), _ptr(kept.offsets),
+                                    _ptr(kept.total), _ptr(kept.ws), kept.ws.numel(), _stream(dev))
+    _check(st, "gstv_keep_prologue")
+    return center, radius, cov, hs, kept
 
 
 def keep_clear(kept):

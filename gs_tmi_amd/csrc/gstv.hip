@@ -831,24 +831,18 @@ __global__ void k_ranges(const int32_t* __restrict__ keys, int total, int32_t* _
 // counts (optional): the Gaussian's pairs for binning cut to its extent box
 // (tile_box with this record; radius and geo as for k_bin_count), and
 // counts[n] = 0 for the scan.
-__global__ __launch_bounds__(256) void k_prep(const float* __restrict__ center, const float* __restrict__ cov,
-                                              const float* __restrict__ rho_r, const float* __restrict__ rho_i,
-                                              int rho_stride, const float* __restrict__ half_shape, int n,
-                                              GRec* __restrict__ rec, const float* __restrict__ radius,
-                                              BinGeom geo, int32_t* __restrict__ counts) {
-  const int g = blockIdx.x * 256 + threadIdx.x;
-  if (counts != nullptr && g == n) counts[n] = 0;
-  if (g >= n) return;
-  const float hs0 = half_shape[0], hs1 = half_shape[1], hs2 = half_shape[2];
-  const float* m = cov + 9 * (int64_t)g;
-  float mm[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) mm[i] = m[i];
+// Record (and, with counts, pair count) of Gaussian g from its centre c and
+// matrix mm (the fp32 values of center_vox and cov_inv).
+__device__ __forceinline__ void prep_one(int g, const float c[3], const float mm[9], const float* __restrict__ center,
+                                         float hs0, float hs1, float hs2, const float* __restrict__ rho_r,
+                                         const float* __restrict__ rho_i, int rho_stride, GRec* __restrict__ rec,
+                                         const float* __restrict__ radius, const BinGeom& geo,
+                                         int32_t* __restrict__ counts) {
   const float kq = (float)kQScaleD;
   GRec r;
-  r.c0 = center[3 * (int64_t)g + 0];
-  r.c1 = center[3 * (int64_t)g + 1];
-  r.c2 = center[3 * (int64_t)g + 2];
+  r.c0 = c[0];
+  r.c1 = c[1];
+  r.c2 = c[2];
   r.rr = rho_r[(int64_t)g * rho_stride];
   r.ri = rho_i[(int64_t)g * rho_stride];
   r.a00 = kq * (mm[0] / (hs0 * hs0));
@@ -894,6 +888,23 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ center, 
     tile_box(center, radius, g, geo, tc, te, tmin, tcnt);
     counts[g] = tcnt[0] * tcnt[1] * tcnt[2];
   }
+}
+
+__global__ __launch_bounds__(256) void k_prep(const float* __restrict__ center, const float* __restrict__ cov,
+                                              const float* __restrict__ rho_r, const float* __restrict__ rho_i,
+                                              int rho_stride, const float* __restrict__ half_shape, int n,
+                                              GRec* __restrict__ rec, const float* __restrict__ radius,
+                                              BinGeom geo, int32_t* __restrict__ counts) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (counts != nullptr && g == n) counts[n] = 0;
+  if (g >= n) return;
+  const float* m = cov + 9 * (int64_t)g;
+  float mm[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) mm[i] = m[i];
+  const float c[3] = {center[3 * (int64_t)g + 0], center[3 * (int64_t)g + 1], center[3 * (int64_t)g + 2]};
+  prep_one(g, c, mm, center, half_shape[0], half_shape[1], half_shape[2], rho_r, rho_i, rho_stride, rec, radius, geo,
+           counts);
 }
 
 // ------------------------------------------------ D extent over a column --
@@ -1847,23 +1858,23 @@ __device__ __forceinline__ float max_nan(float a, float b) { return (a != a || a
 //   R = R(q / (|q| + 1e-8));  T = R * (1 / (s + 1e-8)) (column scaling);  M = T T^T.
 // Centre and radius round after every operation (no contraction) so binning
 // sees the same fp32 values the reference's torch ops produce.
-__global__ __launch_bounds__(256) void k_prologue(const float* __restrict__ pos, const float* __restrict__ scales,
-                                                  const float* __restrict__ rot, int n, float sh0, float sh1,
-                                                  float sh2, float* __restrict__ center, float* __restrict__ radius,
-                                                  float* __restrict__ cov, float* __restrict__ half_shape) {
+// Prologue of Gaussian g (gaussian/tile_voxelizer.py:281-290): writes its
+// centre, radius and cov_inv and returns the centre and matrix in c and M.
+// Every operation rounds (no contraction): centre and radius match the
+// reference's fp32 operation order bit for bit.
+__device__ __forceinline__ void prologue_one(int g, const float* __restrict__ pos, const float* __restrict__ scales,
+                                             const float* __restrict__ rot, float sh0, float sh1, float sh2,
+                                             float* __restrict__ center, float* __restrict__ radius,
+                                             float* __restrict__ cov, float c[3], float M[9]) {
 #pragma clang fp contract(off)
-  const int g = blockIdx.x * 256 + threadIdx.x;
   const float hs0 = sh0 * 0.5f, hs1 = sh1 * 0.5f, hs2 = sh2 * 0.5f;
-  if (g == 0) {
-    half_shape[0] = hs0;
-    half_shape[1] = hs1;
-    half_shape[2] = hs2;
-  }
-  if (g >= n) return;
   const int64_t g3 = 3 * (int64_t)g;
-  center[g3 + 0] = ((pos[g3 + 0] + 1.0f) * 0.5f) * sh0 - 0.5f;
-  center[g3 + 1] = ((pos[g3 + 1] + 1.0f) * 0.5f) * sh1 - 0.5f;
-  center[g3 + 2] = ((pos[g3 + 2] + 1.0f) * 0.5f) * sh2 - 0.5f;
+  c[0] = ((pos[g3 + 0] + 1.0f) * 0.5f) * sh0 - 0.5f;
+  c[1] = ((pos[g3 + 1] + 1.0f) * 0.5f) * sh1 - 0.5f;
+  c[2] = ((pos[g3 + 2] + 1.0f) * 0.5f) * sh2 - 0.5f;
+  center[g3 + 0] = c[0];
+  center[g3 + 1] = c[1];
+  center[g3 + 2] = c[2];
   const float s0 = scales[g3 + 0], s1 = scales[g3 + 1], s2 = scales[g3 + 2];
   radius[g] = max_nan(max_nan(s0 * hs0, s1 * hs1), s2 * hs2) * 3.0f;
   const float4 q4 = *reinterpret_cast<const float4*>(rot + 4 * (int64_t)g);
@@ -1878,11 +1889,55 @@ __global__ __launch_bounds__(256) void k_prologue(const float* __restrict__ pos,
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int k = 0; k < 3; ++k) T[i][k] = R[i][k] * si[k];
-  float* M = cov + 9 * (int64_t)g;
+  float* Mg = cov + 9 * (int64_t)g;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) M[3 * i + j] = T[i][0] * T[j][0] + T[i][1] * T[j][1] + T[i][2] * T[j][2];
+    for (int j = 0; j < 3; ++j) {
+      M[3 * i + j] = T[i][0] * T[j][0] + T[i][1] * T[j][1] + T[i][2] * T[j][2];
+      Mg[3 * i + j] = M[3 * i + j];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_prologue(const float* __restrict__ pos, const float* __restrict__ scales,
+                                                  const float* __restrict__ rot, int n, float sh0, float sh1,
+                                                  float sh2, float* __restrict__ center, float* __restrict__ radius,
+                                                  float* __restrict__ cov, float* __restrict__ half_shape) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g == 0) {
+    half_shape[0] = sh0 * 0.5f;
+    half_shape[1] = sh1 * 0.5f;
+    half_shape[2] = sh2 * 0.5f;
+  }
+  if (g >= n) return;
+  float c[3], M[9];
+  prologue_one(g, pos, scales, rot, sh0, sh1, sh2, center, radius, cov, c, M);
+}
+
+// k_prologue and k_prep in one pass (the fused call): centre, radius and
+// cov_inv written as k_prologue writes them, and the record (and pair count)
+// built from the same fp32 values as k_prep builds it from memory.
+__global__ __launch_bounds__(256) void k_prologue_prep(const float* __restrict__ pos,
+                                                       const float* __restrict__ scales,
+                                                       const float* __restrict__ rot, int n, float sh0, float sh1,
+                                                       float sh2, float* __restrict__ center,
+                                                       float* __restrict__ radius, float* __restrict__ cov,
+                                                       float* __restrict__ half_shape,
+                                                       const float* __restrict__ rho_r,
+                                                       const float* __restrict__ rho_i, int rho_stride,
+                                                       GRec* __restrict__ rec, BinGeom geo,
+                                                       int32_t* __restrict__ counts) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g == 0) {
+    half_shape[0] = sh0 * 0.5f;
+    half_shape[1] = sh1 * 0.5f;
+    half_shape[2] = sh2 * 0.5f;
+  }
+  if (counts != nullptr && g == n) counts[n] = 0;
+  if (g >= n) return;
+  float c[3], M[9];
+  prologue_one(g, pos, scales, rot, sh0, sh1, sh2, center, radius, cov, c, M);
+  prep_one(g, c, M, center, sh0 * 0.5f, sh1 * 0.5f, sh2 * 0.5f, rho_r, rho_i, rho_stride, rec, radius, geo, counts);
 }
 
 // Chain rule of the prologue: from (dL/dc, dL/dM) to positions, linear scales
@@ -2506,6 +2561,37 @@ int gstv_keep_prepare(const float* center_vox, const float* radius_vox, const fl
   int st = launch_prep(center_vox, cov_inv, rho_r, rho_i, rho_stride, half_shape, n, ws.rec, s, radius_vox, &geo,
                        count ? gauss_offsets : nullptr);
   if (st != GSTV_OK || !count) return st;
+  return launch_exclusive_scan(gauss_offsets, (int64_t)n + 1, gauss_offsets, ws.sums, total_pairs, s);
+}
+
+int gstv_keep_prologue(const float* positions, const float* scales, const float* rotations, const float* rho_r,
+                       const float* rho_i, int32_t rho_stride, int32_t n, int32_t d, int32_t h, int32_t w,
+                       int32_t tile, float* center_vox, float* radius_vox, float* cov_inv, float* half_shape,
+                       int32_t* gauss_offsets, int64_t* total_pairs, void* keep_workspace,
+                       size_t keep_workspace_bytes, void* stream) {
+  BinGeom geo{};
+  int64_t num_tiles;
+  if (n < 0 || rho_stride < 1 || half_shape == nullptr || gauss_offsets == nullptr || total_pairs == nullptr ||
+      !tile_geometry(d, h, w, tile, geo.nt, &num_tiles))
+    return GSTV_ERR_INVALID_ARGUMENT;
+  if (n > 0 && (positions == nullptr || scales == nullptr || rotations == nullptr || rho_r == nullptr ||
+                rho_i == nullptr || center_vox == nullptr || radius_vox == nullptr || cov_inv == nullptr))
+    return GSTV_ERR_INVALID_ARGUMENT;
+  if (reinterpret_cast<uintptr_t>(rotations) % 16 != 0) return GSTV_ERR_INVALID_ARGUMENT;
+  if (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n))
+    return GSTV_ERR_WORKSPACE_TOO_SMALL;
+  geo.size[0] = d;
+  geo.size[1] = h;
+  geo.size[2] = w;
+  geo.tile = tile;
+  geo.tile_shift = tile_shift_of(tile);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  KeepLayout ws;
+  keep_layout(keep_workspace, n, &ws);
+  hipLaunchKernelGGL(k_prologue_prep, dim3(ceil_div_i(n + 1, 256)), dim3(256), 0, s, positions, scales, rotations, n,
+                     (float)d, (float)h, (float)w, center_vox, radius_vox, cov_inv, half_shape, rho_r, rho_i,
+                     (int)rho_stride, ws.rec, geo, gauss_offsets);
+  GSTV_LAUNCH_CHECK("keep_prologue");
   return launch_exclusive_scan(gauss_offsets, (int64_t)n + 1, gauss_offsets, ws.sums, total_pairs, s);
 }
 

@@ -115,20 +115,21 @@ class _TileVoxelizeFusedHIP(torch.autograd.Function):
         d_size, h_size, w_size = volume_shape
         scales_c = scales.detach().contiguous().float()
         rot_c = _native.aligned16(rotations.detach().contiguous().float())
-        center, radius, cov, half_shape = _native.prologue(positions, scales_c, rot_c, volume_shape)
         # The kernels read the complex64 density interleaved, in place: no
         # real / imag copies here, and one complex gradient in backward.
         dens = density.detach().reshape(-1).resolve_conj()
         if dens.dtype != torch.complex64:
             dens = dens.to(torch.complex64)
         dv = torch.view_as_real(dens.contiguous())
-        # The Gaussian records come first: binning keeps only the pairs whose
-        # tile meets a Gaussian's extent box, the pairs the tile kernels
-        # evaluate (the reference's lists hold about twice as many, which the
-        # kernels would skip).  Binning reads the pair total back to the host
-        # once; the adjoint's accumulator is cleared behind that read, while
-        # the host waits and then sizes the pair arrays.
-        kept = _native.keep_prepare(center, cov, dv[:, 0], dv[:, 1], half_shape, radius, volume_shape, tile_size)
+        # One pass makes the prologue's outputs and the Gaussian records, and
+        # counts the pairs whose tile meets a Gaussian's extent box: binning
+        # keeps only the pairs the tile kernels evaluate (the reference's lists
+        # hold about twice as many, which the kernels would skip).  Binning
+        # reads the pair total back to the host once; the adjoint's
+        # accumulator is cleared behind that read, while the host waits and
+        # then sizes the pair arrays.
+        center, radius, cov, half_shape, kept = _native.keep_prologue(positions, scales_c, rot_c, dv[:, 0],
+                                                                      dv[:, 1], volume_shape, tile_size)
         stream = torch.cuda.current_stream(center.device)
         total_host = torch.empty(1, dtype=torch.int64, pin_memory=True)
         with torch.cuda.stream(stream):

@@ -154,6 +154,8 @@ int gstv_backward(const float* center_vox, const float* cov_inv,
  *                       other pair of the reference's lists, so the forward on
  *                       such lists is bit-identical and the adjoint equal up to
  *                       the order of fp32 atomic additions.
+ *   gstv_keep_prologue  gstv_prologue and gstv_keep_prepare (with the counts)
+ *                       in one pass; the same outputs.
  *   gstv_keep_bin_fill  gstv_bin_fill for those counts (same order within each
  *                       tile: ascending Gaussian id).
  *   gstv_keep_clear     zeroes the adjoint's moment accumulator.  Neither it
@@ -191,6 +193,13 @@ int gstv_keep_prepare(const float* center_vox, const float* radius_vox, const fl
                       int32_t d, int32_t h, int32_t w, int32_t tile,
                       int32_t* gauss_offsets, int64_t* total_pairs,
                       void* keep_workspace, size_t keep_workspace_bytes, void* stream);
+
+int gstv_keep_prologue(const float* positions, const float* scales, const float* rotations,
+                       const float* rho_r, const float* rho_i, int32_t rho_stride, int32_t n,
+                       int32_t d, int32_t h, int32_t w, int32_t tile,
+                       float* center_vox, float* radius_vox, float* cov_inv, float* half_shape,
+                       int32_t* gauss_offsets, int64_t* total_pairs,
+                       void* keep_workspace, size_t keep_workspace_bytes, void* stream);
 
 int gstv_keep_clear(void* keep_workspace, size_t keep_workspace_bytes, int32_t n, void* stream);
 

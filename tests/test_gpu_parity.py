@@ -249,6 +249,28 @@ def test_keep_binning_is_the_touching_pairs(name, make):
 
 
 @pytest.mark.parametrize("name,make", ALL_CASES, ids=[n for n, _ in ALL_CASES])
+def test_keep_prologue_matches_separate_passes(name, make):
+    """gstv_keep_prologue (one pass) gives gstv_prologue's outputs and
+    gstv_keep_prepare's records and pair counts bit for bit."""
+    from gs_tmi_amd import _native
+
+    c = make()
+    tile = c["tile"]
+    _announce(f"keep prologue {name} shape={c['shape']} tile={tile} n={len(c['rr'])}")
+    t = lambda a: torch.from_numpy(a).to(DEV)  # noqa: E731
+    pos, scales, rot, rr, ri = t(c["pos"]), t(c["scales"]), t(c["rot"]), t(c["rr"]), t(c["ri"])
+    center, radius, cov, hs = _native.prologue(pos, scales, rot, c["shape"])
+    kept = _native.keep_prepare(center, cov, rr, ri, hs, radius, c["shape"], tile)
+    center2, radius2, cov2, hs2, kept2 = _native.keep_prologue(pos, scales, rot, rr, ri, c["shape"], tile)
+    torch.cuda.synchronize()
+    for a, b in ((center, center2), (radius, radius2), (cov, cov2), (hs, hs2), (kept.offsets, kept2.offsets),
+                 (kept.total, kept2.total)):
+        assert torch.equal(a, b)
+    n = center.shape[0]
+    assert torch.equal(kept.ws[: n * 64], kept2.ws[: n * 64])
+
+
+@pytest.mark.parametrize("name,make", ALL_CASES, ids=[n for n, _ in ALL_CASES])
 def test_keep_path_matches_plain_calls(name, make):
     """gstv_forward_keep / gstv_backward_keep (records and column rows handed
     from the forward to the adjoint) give the plain calls' results: the same
