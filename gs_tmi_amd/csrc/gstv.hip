@@ -277,13 +277,11 @@ __device__ __forceinline__ int tile_of(int x, const BinGeom& geo) {
 // every other pair of the tile lists (the box test of stage_pair, with the same
 // float bounds), so dropping them changes no result; a NaN bound restricts
 // nothing, as there.
-__device__ __forceinline__ void tile_box(const float* __restrict__ center, const float* __restrict__ radius, int g,
-                                         const BinGeom& geo, const float* tc, const float* te, int tmin[3],
-                                         int tcnt[3]) {
-  const float r = radius[g];
+__device__ __forceinline__ void tile_box_v(const float cref[3], float r, const BinGeom& geo, const float* tc,
+                                           const float* te, int tmin[3], int tcnt[3]) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float c = center[3 * (int64_t)g + a];
+    const float c = cref[a];
     int lo = sat_f2i(floorf(c - r));
     int hi = sat_f2i(ceilf(c + r));
     lo = min(max(lo, 0), geo.size[a] - 1);
@@ -305,6 +303,14 @@ __device__ __forceinline__ void tile_box(const float* __restrict__ center, const
     tmin[a] = tl;
     tcnt[a] = max(th - tl + 1, 0);  // a negative radius can invert the box: no pairs
   }
+}
+
+// tile_box_v of Gaussian g read from center_vox / radius_vox.
+__device__ __forceinline__ void tile_box(const float* __restrict__ center, const float* __restrict__ radius, int g,
+                                         const BinGeom& geo, const float* tc, const float* te, int tmin[3],
+                                         int tcnt[3]) {
+  const float cref[3] = {center[3 * (int64_t)g + 0], center[3 * (int64_t)g + 1], center[3 * (int64_t)g + 2]};
+  tile_box_v(cref, radius[g], geo, tc, te, tmin, tcnt);
 }
 
 // tile_box of Gaussian g, cut to its record's extent box when geo.rec is set.
@@ -833,10 +839,9 @@ __global__ void k_ranges(const int32_t* __restrict__ keys, int total, int32_t* _
 // counts[n] = 0 for the scan.
 // Record (and, with counts, pair count) of Gaussian g from its centre c and
 // matrix mm (the fp32 values of center_vox and cov_inv).
-__device__ __forceinline__ void prep_one(int g, const float c[3], const float mm[9], const float* __restrict__ center,
-                                         float hs0, float hs1, float hs2, const float* __restrict__ rho_r,
-                                         const float* __restrict__ rho_i, int rho_stride, GRec* __restrict__ rec,
-                                         const float* __restrict__ radius, const BinGeom& geo,
+__device__ __forceinline__ void prep_one(int g, const float c[3], const float mm[9], float hs0, float hs1, float hs2,
+                                         const float* __restrict__ rho_r, const float* __restrict__ rho_i,
+                                         int rho_stride, GRec* __restrict__ rec, float rad, const BinGeom& geo,
                                          int32_t* __restrict__ counts) {
   const float kq = (float)kQScaleD;
   GRec r;
@@ -885,7 +890,7 @@ __device__ __forceinline__ void prep_one(int g, const float c[3], const float mm
   if (counts != nullptr) {
     const float tc[3] = {r.c0, r.c1, r.c2}, te[3] = {e0, e1, e2};
     int tmin[3], tcnt[3];
-    tile_box(center, radius, g, geo, tc, te, tmin, tcnt);
+    tile_box_v(c, rad, geo, tc, te, tmin, tcnt);
     counts[g] = tcnt[0] * tcnt[1] * tcnt[2];
   }
 }
@@ -903,8 +908,8 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ center, 
 #pragma unroll
   for (int i = 0; i < 9; ++i) mm[i] = m[i];
   const float c[3] = {center[3 * (int64_t)g + 0], center[3 * (int64_t)g + 1], center[3 * (int64_t)g + 2]};
-  prep_one(g, c, mm, center, half_shape[0], half_shape[1], half_shape[2], rho_r, rho_i, rho_stride, rec, radius, geo,
-           counts);
+  prep_one(g, c, mm, half_shape[0], half_shape[1], half_shape[2], rho_r, rho_i, rho_stride, rec,
+           counts != nullptr ? radius[g] : 0.0f, geo, counts);
 }
 
 // ------------------------------------------------ D extent over a column --
@@ -1865,7 +1870,7 @@ __device__ __forceinline__ float max_nan(float a, float b) { return (a != a || a
 __device__ __forceinline__ void prologue_one(int g, const float* __restrict__ pos, const float* __restrict__ scales,
                                              const float* __restrict__ rot, float sh0, float sh1, float sh2,
                                              float* __restrict__ center, float* __restrict__ radius,
-                                             float* __restrict__ cov, float c[3], float M[9]) {
+                                             float* __restrict__ cov, float c[3], float M[9], float& rad) {
 #pragma clang fp contract(off)
   const float hs0 = sh0 * 0.5f, hs1 = sh1 * 0.5f, hs2 = sh2 * 0.5f;
   const int64_t g3 = 3 * (int64_t)g;
@@ -1876,7 +1881,8 @@ __device__ __forceinline__ void prologue_one(int g, const float* __restrict__ po
   center[g3 + 1] = c[1];
   center[g3 + 2] = c[2];
   const float s0 = scales[g3 + 0], s1 = scales[g3 + 1], s2 = scales[g3 + 2];
-  radius[g] = max_nan(max_nan(s0 * hs0, s1 * hs1), s2 * hs2) * 3.0f;
+  rad = max_nan(max_nan(s0 * hs0, s1 * hs1), s2 * hs2) * 3.0f;
+  radius[g] = rad;
   const float4 q4 = *reinterpret_cast<const float4*>(rot + 4 * (int64_t)g);
   const float nrm = sqrtf(q4.x * q4.x + q4.y * q4.y + q4.z * q4.z + q4.w * q4.w) + 1e-8f;
   const float w = q4.x / nrm, x = q4.y / nrm, y = q4.z / nrm, z = q4.w / nrm;
@@ -1910,8 +1916,8 @@ __global__ __launch_bounds__(256) void k_prologue(const float* __restrict__ pos,
     half_shape[2] = sh2 * 0.5f;
   }
   if (g >= n) return;
-  float c[3], M[9];
-  prologue_one(g, pos, scales, rot, sh0, sh1, sh2, center, radius, cov, c, M);
+  float c[3], M[9], rad;
+  prologue_one(g, pos, scales, rot, sh0, sh1, sh2, center, radius, cov, c, M, rad);
 }
 
 // k_prologue and k_prep in one pass (the fused call): centre, radius and
@@ -1935,9 +1941,9 @@ __global__ __launch_bounds__(256) void k_prologue_prep(const float* __restrict__
   }
   if (counts != nullptr && g == n) counts[n] = 0;
   if (g >= n) return;
-  float c[3], M[9];
-  prologue_one(g, pos, scales, rot, sh0, sh1, sh2, center, radius, cov, c, M);
-  prep_one(g, c, M, center, sh0 * 0.5f, sh1 * 0.5f, sh2 * 0.5f, rho_r, rho_i, rho_stride, rec, radius, geo, counts);
+  float c[3], M[9], rad;
+  prologue_one(g, pos, scales, rot, sh0, sh1, sh2, center, radius, cov, c, M, rad);
+  prep_one(g, c, M, sh0 * 0.5f, sh1 * 0.5f, sh2 * 0.5f, rho_r, rho_i, rho_stride, rec, rad, geo, counts);
 }
 
 // Chain rule of the prologue: from (dL/dc, dL/dM) to positions, linear scales
