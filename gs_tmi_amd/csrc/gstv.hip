@@ -441,7 +441,10 @@ struct GenLds {
   int g0, ng;
 };
 static_assert((4 * kGenCap * sizeof(int32_t)) % 16 == 0, "gidx must be 16-byte aligned");
-static_assert(kRadixTile == kRadixThreads * 16, "the owner scan gives each thread 16 slots");
+// The owner scan gives each thread kRadixTile / kRadixThreads consecutive
+// slots, read as 16-byte vectors of 8 slots.
+constexpr int kOwnerVecs = kRadixTile / kRadixThreads / 8;
+static_assert(kOwnerVecs >= 1 && kRadixTile == kRadixThreads * 8 * kOwnerVecs, "slots per thread: a multiple of 8");
 
 struct ScatterLds {
   int32_t key[kRadixTile];
@@ -481,8 +484,8 @@ __device__ __forceinline__ bool gen_setup(GenLds& L, const float* __restrict__ c
   const int nthw = geo.nt[1] * geo.nt[2];
   uint4* g16 = reinterpret_cast<uint4*>(L.gidx);
   const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
-  g16[2 * threadIdx.x] = zero4;
-  g16[2 * threadIdx.x + 1] = zero4;
+#pragma unroll
+  for (int q = 0; q < kOwnerVecs; ++q) g16[kOwnerVecs * threadIdx.x + q] = zero4;
   for (int i = threadIdx.x; i < ng; i += kRadixThreads) {
     int tmin[3], tcnt[3];
     gauss_tile_box(center, radius, g0 + i, geo, tmin, tcnt);
@@ -502,13 +505,20 @@ __device__ __forceinline__ bool gen_setup(GenLds& L, const float* __restrict__ c
     if (o < next && o < j1) L.gidx[o - j0] = (uint16_t)i;  // o > j0: slot j0 belongs to Gaussian 0
   }
   __syncthreads();
-  // Each thread scans its 16 consecutive slots (two per 32-bit word), then the
+  // Each thread scans its consecutive slots (two per 32-bit word), then the
   // running maxima of the threads are combined across the wave and the waves.
-  const uint4 a = g16[2 * threadIdx.x], c = g16[2 * threadIdx.x + 1];
-  uint32_t v[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
+  uint32_t v[4 * kOwnerVecs];
+#pragma unroll
+  for (int q = 0; q < kOwnerVecs; ++q) {
+    const uint4 a = g16[kOwnerVecs * threadIdx.x + q];
+    v[4 * q + 0] = a.x;
+    v[4 * q + 1] = a.y;
+    v[4 * q + 2] = a.z;
+    v[4 * q + 3] = a.w;
+  }
   uint32_t run = 0;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < 4 * kOwnerVecs; ++k) {
     const uint32_t lo = max(v[k] & 0xffffu, run);
     const uint32_t hi = max(v[k] >> 16, lo);
     v[k] = lo | (hi << 16);
@@ -527,13 +537,14 @@ __device__ __forceinline__ bool gen_setup(GenLds& L, const float* __restrict__ c
   __syncthreads();
   for (int w = 0; w < wid; ++w) excl = max(excl, L.wmax[w]);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < 4 * kOwnerVecs; ++k) {
     // Both halves are at most 0xffff: a per-half max of v[k] and excl.
     const uint32_t lo = max(v[k] & 0xffffu, excl), hi = max(v[k] >> 16, excl);
     v[k] = lo | (hi << 16);
   }
-  g16[2 * threadIdx.x] = make_uint4(v[0], v[1], v[2], v[3]);
-  g16[2 * threadIdx.x + 1] = make_uint4(v[4], v[5], v[6], v[7]);
+#pragma unroll
+  for (int q = 0; q < kOwnerVecs; ++q)
+    g16[kOwnerVecs * threadIdx.x + q] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
   __syncthreads();
   return true;
 }
