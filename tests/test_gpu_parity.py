@@ -193,9 +193,10 @@ def _touching_lists(ranges, pids, rec, shape, tile):
 
 @pytest.mark.parametrize("name,make", ALL_CASES, ids=[n for n, _ in ALL_CASES])
 def test_keep_binning_is_the_touching_pairs(name, make):
-    """gstv_keep_bin_count / _fill give exactly the reference's pairs that the
-    tile kernels' box test keeps, in the same order, and the forward on those
-    lists equals the forward on the reference's lists bit for bit."""
+    """gstv_keep_prepare's counts and gstv_keep_bin_fill give exactly the
+    reference's pairs that the tile kernels' box test keeps, in the same order,
+    and the forward on those lists equals the forward on the reference's lists
+    bit for bit."""
     from gs_tmi_amd import _native
 
     c = make()
