@@ -275,7 +275,8 @@ __device__ __forceinline__ int tile_of(int x, const BinGeom& geo) {
 // (centre and extents of the Gaussian's record), the box is cut to the tiles
 // that the extent box [ceil(c - e), floor(c + e)] meets: the tile kernels skip
 // every other pair of the tile lists (the box test of stage_pair, with the same
-// float bounds), so dropping them changes no result; a NaN bound restricts
+// float bounds), so dropping them changes no evaluation (only how a tile's
+// pairs fall into the forward's 512-pair batches); a NaN bound restricts
 // nothing, as there.
 __device__ __forceinline__ void tile_box_v(const float cref[3], float r, const BinGeom& geo, const float* tc,
                                            const float* te, int tmin[3], int tcnt[3]) {

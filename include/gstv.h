@@ -151,9 +151,10 @@ int gstv_backward(const float* center_vox, const float* cov_inv,
  *                       meets the Gaussian's extent box (the box of its q <= 9
  *                       ellipsoid in the record, widened so that no voxel with
  *                       fp32 q <= 9 lies outside).  The tile kernels skip every
- *                       other pair of the reference's lists, so the forward on
- *                       such lists is bit-identical and the adjoint equal up to
- *                       the order of fp32 atomic additions.
+ *                       other pair of the reference's lists, so the forward and
+ *                       the adjoint on such lists make the same evaluations;
+ *                       the forward's fp32 sums differ only in order where a
+ *                       tile's list spans more than one batch of 512 pairs.
  *   gstv_keep_prologue  gstv_prologue and gstv_keep_prepare (with the counts)
  *                       in one pass; the same outputs.
  *   gstv_keep_bin_fill  gstv_bin_fill for those counts (same order within each
@@ -179,8 +180,8 @@ int gstv_backward(const float* center_vox, const float* cov_inv,
  *                       rotations (as gstv_epilogue; rotations 16-byte
  *                       aligned) instead of grad_center / grad_cov_inv, and
  *                       the density gradient.
- * Results equal gstv_forward's bit for bit and gstv_backward's up to the order
- * of fp32 atomic additions.
+ * On the same tile lists, results equal gstv_forward's bit for bit and
+ * gstv_backward's up to the order of fp32 atomic additions.
  * rho_stride (>= 1) is the distance, in floats, between consecutive Gaussians'
  * values in rho_r / rho_i and in grad_rho_r / grad_rho_i: 1 for planar arrays,
  * 2 with rho_i = rho_r + 1 (and grad_rho_i = grad_rho_r + 1) for an

@@ -232,3 +232,39 @@ def test_fused_call_backward_twice():
         e = cases.rel_l2(b.cpu().numpy(), a.cpu().numpy())
         print(f"  second backward {nm}: rel L2 {e:.2e}")
         assert e < 5e-6, (nm, e)
+
+
+def test_full_size_fused_call_matches_reference_lists():
+    """Voxelizer microbenchmark size: the fused call (extent-box tile lists)
+    gives the volume of the forward kernel on the reference's tile lists with
+    the same prologue up to fp32 summation order (the lists split into
+    512-pair batches differently), and the same density gradient up to the
+    order of fp32 atomic additions."""
+    from gs_tmi_amd import TileVoxelizer, _native, build_tile_pairs
+
+    shape = (256, 256, 128)
+    d, h, w = shape
+    _announce("full-size fused call vs reference lists 256x256x128, 1M Gaussians")
+    c = cases.microbench_case(n=1_000_000, shape=shape, seed=3)
+    t = lambda a: torch.from_numpy(a).to(DEV)  # noqa: E731
+    pos, sc, rot, rr, ri = t(c["pos"]), t(c["scales"]), t(c["rot"]), t(c["rr"]), t(c["ri"])
+    center, radius, cov, hs = _native.prologue(pos, sc, rot, shape)
+    ranges, pids = build_tile_pairs(center, radius, shape, 8)
+    vr, vi = _native.forward(center, cov, rr, ri, ranges, pids, hs, 8, d, h, w)
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    g_r = torch.randn(d * h * w, generator=gen).to(DEV)
+    g_i = torch.randn(d * h * w, generator=gen).to(DEV)
+    _, _, grr, gri = _native.backward(center, cov, rr, ri, ranges, pids, hs, g_r, g_i, 8, d, h, w)
+    dens = torch.complex(rr, ri).requires_grad_()
+    fr, fi = TileVoxelizer(shape, tile_size=8, device=DEV)(pos, sc, rot, dens)
+    ((fr.reshape(-1) * g_r).sum() + (fi.reshape(-1) * g_i).sum()).backward()
+    torch.cuda.synchronize()
+    ev = cases.rel_l2(torch.stack([fr.reshape(-1), fi.reshape(-1)]).detach().cpu().numpy(),
+                      torch.stack([vr, vi]).cpu().numpy())
+    dmax = float(torch.maximum((fr.reshape(-1) - vr).abs().max(), (fi.reshape(-1) - vi).abs().max()))
+    print(f"  volume rel L2 {ev:.2e}, max abs difference {dmax:.2e}")
+    assert ev < 1e-6
+    gd = dens.grad
+    e = cases.rel_l2(torch.stack([gd.real, gd.imag]).cpu().numpy(), torch.stack([grr, gri]).cpu().numpy())
+    print(f"  pairs: reference {pids.numel()}; density gradient rel L2 {e:.2e}")
+    assert e < 1e-6

@@ -196,7 +196,8 @@ def test_keep_binning_is_the_touching_pairs(name, make):
     """gstv_keep_prepare's counts and gstv_keep_bin_fill give exactly the
     reference's pairs that the tile kernels' box test keeps, in the same order,
     and the forward on those lists equals the forward on the reference's lists
-    bit for bit."""
+    bit for bit (these cases' tile lists fit one 512-pair batch, so the sums
+    run in the same order)."""
     from gs_tmi_amd import _native
 
     c = make()
