@@ -2499,8 +2499,7 @@ int gstv_forward(const float* center_vox, const float* cov_inv, const float* rho
   int st = check_tile_args(tile_ranges, num_tiles, n, tile, d, h, w, nt);
   if (st != GSTV_OK) return st;
   if (vol_r == nullptr || vol_i == nullptr || half_shape == nullptr) return GSTV_ERR_INVALID_ARGUMENT;
-  if (n > 0 && (center_vox == nullptr || cov_inv == nullptr || rho_r == nullptr || rho_i == nullptr ||
-                pair_ids == nullptr))
+  if (n > 0 && (center_vox == nullptr || cov_inv == nullptr || rho_r == nullptr || rho_i == nullptr))
     return GSTV_ERR_INVALID_ARGUMENT;
   if (n > 0 && (workspace == nullptr || workspace_bytes < gstv_forward_workspace_bytes(n)))
     return GSTV_ERR_WORKSPACE_TOO_SMALL;
@@ -2527,7 +2526,7 @@ int gstv_backward(const float* center_vox, const float* cov_inv, const float* rh
   int st = check_tile_args(tile_ranges, num_tiles, n, tile, d, h, w, nt);
   if (st != GSTV_OK) return st;
   if (n == 0) return GSTV_OK;
-  if (center_vox == nullptr || cov_inv == nullptr || rho_r == nullptr || rho_i == nullptr || pair_ids == nullptr ||
+  if (center_vox == nullptr || cov_inv == nullptr || rho_r == nullptr || rho_i == nullptr ||
       half_shape == nullptr || grad_r == nullptr || grad_i == nullptr || grad_center == nullptr ||
       grad_cov_inv == nullptr || grad_rho_r == nullptr || grad_rho_i == nullptr)
     return GSTV_ERR_INVALID_ARGUMENT;
@@ -2645,7 +2644,7 @@ int gstv_forward_keep(const void* keep_workspace, size_t keep_workspace_bytes, c
   int st = check_tile_args(tile_ranges, num_tiles, n, tile, d, h, w, nt);
   if (st != GSTV_OK) return st;
   if (vol_r == nullptr || vol_i == nullptr) return GSTV_ERR_INVALID_ARGUMENT;
-  if (n > 0 && (pair_ids == nullptr || (tile <= 8 && pair_columns == nullptr))) return GSTV_ERR_INVALID_ARGUMENT;
+  if (n > 0 && tile <= 8 && pair_columns == nullptr) return GSTV_ERR_INVALID_ARGUMENT;
   if (n > 0 && (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n)))
     return GSTV_ERR_WORKSPACE_TOO_SMALL;
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -2667,7 +2666,7 @@ int gstv_backward_keep(void* keep_workspace, size_t keep_workspace_bytes, const 
   if (st != GSTV_OK) return st;
   if (rho_stride < 1) return GSTV_ERR_INVALID_ARGUMENT;
   if (n == 0) return GSTV_OK;
-  if (cov_inv == nullptr || pair_ids == nullptr || half_shape == nullptr || grad_r == nullptr ||
+  if (cov_inv == nullptr || half_shape == nullptr || grad_r == nullptr ||
       grad_i == nullptr || grad_center == nullptr || grad_cov_inv == nullptr || grad_rho_r == nullptr ||
       grad_rho_i == nullptr || (tile <= 8 && pair_columns == nullptr))
     return GSTV_ERR_INVALID_ARGUMENT;
@@ -2694,7 +2693,7 @@ int gstv_backward_keep_params(void* keep_workspace, size_t keep_workspace_bytes,
   if (st != GSTV_OK) return st;
   if (rho_stride < 1) return GSTV_ERR_INVALID_ARGUMENT;
   if (n == 0) return GSTV_OK;
-  if (cov_inv == nullptr || pair_ids == nullptr || half_shape == nullptr || scales == nullptr ||
+  if (cov_inv == nullptr || half_shape == nullptr || scales == nullptr ||
       rotations == nullptr || grad_r == nullptr || grad_i == nullptr || grad_positions == nullptr ||
       grad_scales == nullptr || grad_rotations == nullptr || grad_rho_r == nullptr || grad_rho_i == nullptr ||
       (tile <= 8 && pair_columns == nullptr))
@@ -2723,8 +2722,7 @@ int gstv_count_evaluations(const float* center_vox, const float* cov_inv, const 
   int st = check_tile_args(tile_ranges, num_tiles, n, tile, d, h, w, nt);
   if (st != GSTV_OK) return st;
   if (counts == nullptr || half_shape == nullptr) return GSTV_ERR_INVALID_ARGUMENT;
-  if (n > 0 && (center_vox == nullptr || cov_inv == nullptr || rho_r == nullptr || rho_i == nullptr ||
-                pair_ids == nullptr))
+  if (n > 0 && (center_vox == nullptr || cov_inv == nullptr || rho_r == nullptr || rho_i == nullptr))
     return GSTV_ERR_INVALID_ARGUMENT;
   if (n > 0 && (workspace == nullptr || workspace_bytes < gstv_forward_workspace_bytes(n)))
     return GSTV_ERR_WORKSPACE_TOO_SMALL;

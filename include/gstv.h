@@ -106,6 +106,9 @@ int gstv_bin_fill(const float* center_vox, const float* radius_vox, int32_t n,
  * contributes), so the outputs need no prior clearing.  Deterministic.
  *   n           number of Gaussians (rows of center_vox / cov_inv / rho)
  *   num_tiles   rows of tile_ranges; must equal nTD*nTH*nTW for (d, h, w, tile)
+ *   pair_ids    read only through tile_ranges: may be null when no tile has
+ *               pairs (the data pointer of an empty array), here and in every
+ *               other entry point that takes tile lists
  */
 size_t gstv_forward_workspace_bytes(int32_t n);
 

@@ -268,3 +268,24 @@ def test_full_size_fused_call_matches_reference_lists():
     e = cases.rel_l2(torch.stack([gd.real, gd.imag]).cpu().numpy(), torch.stack([grr, gri]).cpu().numpy())
     print(f"  pairs: reference {pids.numel()}; density gradient rel L2 {e:.2e}")
     assert e < 1e-6
+
+
+def test_fused_call_without_gaussians_or_pairs():
+    """The fused call with no Gaussians, and with Gaussians whose extent boxes
+    miss every tile (no pairs): zero volumes and zero (or empty) gradients."""
+    from gs_tmi_amd import TileVoxelizer
+
+    shape = (16, 8, 12)
+    _announce(f"fused call without Gaussians / pairs shape={shape}")
+    vox = TileVoxelizer(shape, tile_size=8, device=DEV)
+    for n, far in ((0, False), (5, True)):
+        pos = torch.full((n, 3), 9.0 if far else 0.0, device=DEV, requires_grad=True)
+        log_s = torch.full((n, 3), -4.0, device=DEV, requires_grad=True)
+        rot = torch.tensor([[1.0, 0.0, 0.0, 0.0]] * n, device=DEV).reshape(n, 4).requires_grad_()
+        dens = torch.ones(n, dtype=torch.complex64, device=DEV, requires_grad=True)
+        vr, vi = vox(pos, torch.exp(log_s), rot, dens)
+        assert vr.shape == shape and float(vr.abs().sum() + vi.abs().sum()) == 0.0
+        (vr.sum() + vi.sum()).backward()
+        for t in (pos, log_s, rot, dens):
+            assert t.grad is not None and t.grad.shape == t.shape
+            assert float(t.grad.abs().sum()) == 0.0
