@@ -284,7 +284,7 @@ def test_fused_call_without_gaussians_or_pairs():
         rot = torch.tensor([[1.0, 0.0, 0.0, 0.0]] * n, device=DEV).reshape(n, 4).requires_grad_()
         dens = torch.ones(n, dtype=torch.complex64, device=DEV, requires_grad=True)
         vr, vi = vox(pos, torch.exp(log_s), rot, dens)
-        assert vr.shape == shape and float(vr.abs().sum() + vi.abs().sum()) == 0.0
+        assert vr.shape == shape and float(vr.detach().abs().sum() + vi.detach().abs().sum()) == 0.0
         (vr.sum() + vi.sum()).backward()
         for t in (pos, log_s, rot, dens):
             assert t.grad is not None and t.grad.shape == t.shape
