@@ -185,10 +185,13 @@ def test_gemm_nt_k_split_invariance():
     assert np.all(diff <= bound), float(np.max(diff - bound))
 
 
-@pytest.mark.parametrize("m,n,k", [(64, 64, 64), (65, 63, 130), (256, 128, 200), (4096, 512, 1024)])
+@pytest.mark.parametrize("m,n,k", [(64, 64, 64), (65, 63, 130), (256, 128, 200), (4096, 512, 1024),
+                                   (4100, 4097, 96)])
 def test_gemm_nt_tile_variants(m, n, k):
-    """Both block-tile variants (64x64 below one block per CU, 128x128 above)
-    against FP64 of the rounded operands, with bias, ReLU and the bf16 copy."""
+    """The block-tile variants (64x64 below one 128x128 block per CU, 128x128
+    above, 256x256 from one 256x256 block per CU: the last case, with ragged
+    edges) against FP64 of the rounded operands, with bias, ReLU and the bf16
+    copy."""
     mf = _mfma()
     print(f"[gemm variant] m={m} n={n} k={k}", flush=True)
     g = torch.Generator(device="cpu").manual_seed(m + n + k)
