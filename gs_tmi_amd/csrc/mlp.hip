@@ -25,7 +25,6 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 constexpr int kBK = 64;                 // k per LDS stage
 constexpr int kPad = 8;                 // bf16 elements of padding per LDS row (16 B): conflict-free b128 reads
 constexpr int kLdk = kBK + kPad;
-constexpr int kThreads = 256;           // 4 waves in a 2 x 2 grid (the 64x64 tile)
 
 thread_local char g_detail[256] = "";
 
@@ -213,7 +212,8 @@ int gstv_mlp_gemm_nt_bf16(const void* a, int64_t lda, const void* b, int64_t ldb
   // The largest block tile that still gives at least one block per CU:
   // 256 x 256 (16 waves of 64 x 64; 1.0 PFLOP/s at 4096^3 against 0.68 for
   // 128 x 128 with 4 waves), 128 x 128 (8 waves of 32 x 64; 474 against 410
-  // TFLOP/s with 4 waves at 16384 x 512 x 1024), else 64 x 64 (4 waves).
+  // TFLOP/s with 4 waves at 16384 x 512 x 1024), else 64 x 64 (8 waves of
+  // 32 x 16; 158 against 137 TFLOP/s with 4 waves at 2048 x 512 x 1024).
   const int64_t huge_blocks = (int64_t)((n + 255) / 256) * ((m + 255) / 256);
   const int64_t big_blocks = (int64_t)((n + 127) / 128) * ((m + 127) / 128);
   const hipStream_t st = static_cast<hipStream_t>(stream);
@@ -230,7 +230,7 @@ int gstv_mlp_gemm_nt_bf16(const void* a, int64_t lda, const void* b, int64_t ldb
                        ldc, pcb, ldcb, vec_a, vec_b);
   } else {
     dim3 grid((n + 63) / 64, (m + 63) / 64);
-    hipLaunchKernelGGL((k_gemm_nt<64, 64>), grid, dim3(kThreads), 0, st, pa, lda, pb, ldb, bias, act, m, n, k, c,
+    hipLaunchKernelGGL((k_gemm_nt<64, 64, 2, 4>), grid, dim3(512), 0, st, pa, lda, pb, ldb, bias, act, m, n, k, c,
                        ldc, pcb, ldcb, vec_a, vec_b);
   }
   return hip_status(hipGetLastError(), "gemm_nt_bf16");
