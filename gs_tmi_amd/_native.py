@@ -200,6 +200,11 @@ def _rho_stride(rho_r, rho_i, keep):
     [n, 2] (complex64) array."""
     if rho_r.dim() != 1 or rho_i.dim() != 1:
         raise ValueError("rho_r and rho_i must be 1-D")
+    if not keep and rho_r.numel() <= 1 and rho_i.numel() <= 1:
+        # torch calls a one-element view contiguous whatever its stride (the
+        # real part of a one-element complex tensor keeps stride 2); the one
+        # address read does not depend on it.
+        return 1
     st = rho_r.stride(0)
     if rho_i.stride(0) != st:
         raise ValueError("rho_r and rho_i must have the same stride")

@@ -42,9 +42,27 @@ def test_tile_voxelizer_param_gradients_vs_fp64(reading):
     loss.backward()
     torch.cuda.synchronize()
 
-    # FP64 reference with the reference's binning of the fp32 prologue.
-    # The oracle bins the exact fp32 scales the GPU prologue saw (exp on the GPU).
-    s_gpu = torch.exp(log_s).detach().cpu().numpy()
+    got = dict(pos=pos.grad, log_s=log_s.grad, rot=rot.grad, dr=dr.grad, di=di.grad)
+    ref, ok, vol64 = _fp64_reference(c, torch.exp(log_s).detach().cpu().numpy(), g_r, g_i)
+    for k in got:
+        e = cases.rel_l2(got[k].cpu().numpy()[ok], ref[k][ok])
+        print(f"  d/d{k}: rel L2 {e:.3e}")
+        assert e <= 1e-4, (k, e)
+    # Volume parity through the full call.
+    e_vol = cases.rel_l2(np.concatenate([vol_r.detach().cpu().numpy().ravel(), vol_i.detach().cpu().numpy().ravel()]),
+                         vol64)
+    print(f"  volume rel L2 {e_vol:.3e}")
+    assert e_vol <= 1e-5
+
+
+def _fp64_reference(c, s_gpu, g_r, g_i):
+    """FP64 torch autograd of the whole call on case ``c`` with upstream
+    gradient (g_r, g_i), binned as the reference bins the fp32 prologue of the
+    exact fp32 scales the GPU saw (``s_gpu``, exp on the GPU).  Returns the
+    parameter gradients (pos, log_s, rot, dr, di), the mask of Gaussians with no
+    voxel within 1e-4 of the q = 9 cutoff (elsewhere fp32 and fp64 may disagree
+    on a voxel's inclusion), and the volume (real then imaginary, flattened)."""
+    shape, tile = c["shape"], c["tile"]
     c32, r32, cov64, hs = vo.prologue(c["pos"], s_gpu, c["rot"], shape)
     ranges, pids = vo.bin_tiles(c32, r32, shape, tile)
     *_, amb = vo.backward(c32, cov64, c["rr"], c["ri"], ranges, pids, hs, g_r, g_i, shape, tile, band=1e-4)
@@ -54,17 +72,8 @@ def test_tile_voxelizer_param_gradients_vs_fp64(reading):
     loss64 = (er * torch.from_numpy(g_r.ravel().astype(np.float64))).sum() + \
              (ei * torch.from_numpy(g_i.ravel().astype(np.float64))).sum()
     loss64.backward()
-    ok = ~amb
-    got = dict(pos=pos.grad, log_s=log_s.grad, rot=rot.grad, dr=dr.grad, di=di.grad)
-    for k in got:
-        e = cases.rel_l2(got[k].cpu().numpy()[ok], t[k].grad.numpy()[ok])
-        print(f"  d/d{k}: rel L2 {e:.3e}")
-        assert e <= 1e-4, (k, e)
-    # Volume parity through the full call.
-    e_vol = cases.rel_l2(np.concatenate([vol_r.detach().cpu().numpy().ravel(), vol_i.detach().cpu().numpy().ravel()]),
-                         np.concatenate([er.detach().numpy(), ei.detach().numpy()]))
-    print(f"  volume rel L2 {e_vol:.3e}")
-    assert e_vol <= 1e-5
+    ref = {k: (v.grad.numpy() if v.grad is not None else np.zeros(v.shape)) for k, v in t.items()}
+    return ref, ~amb, np.concatenate([er.detach().numpy(), ei.detach().numpy()])
 
 
 def _microbench_inputs(n, shape, seed=0):
@@ -148,19 +157,33 @@ def test_full_size_properties():
         assert e < 1e-6, (name, e)
 
 
-@pytest.mark.parametrize("name", ["adjoint_voxel", "partial_tiles", "tile5", "outside"])
+# The two prologues give cov_inv within fp32 rounding of each other (not bit for
+# bit), which the quadratic form's conditioning amplifies for needle-shaped and
+# strongly anisotropic Gaussians (axis ratios up to ~600).  Tolerances
+# (volume, parameter gradients) between the two calls, then against FP64.
+FUSED_TOL = {"needles": (5e-6, 2e-5), "extreme_aniso": (5e-5, 5e-4)}
+FP64_TOL = {"extreme_aniso": (5e-5, 1e-3)}
+
+
+@pytest.mark.parametrize("name", ["adjoint_voxel"] + cases.EDGE_KINDS + [f"random{i}" for i in range(4)])
 def test_fused_call_matches_unfused_composition(name):
     """TileVoxelizer.forward (fused HIP prologue and epilogue) against the
     reference's composition: torch prologue + binning + 11-argument Function."""
     from gs_tmi_amd import TileVoxelizer
 
-    c = cases.adjoint_check_case(0, "voxel") if name == "adjoint_voxel" else cases.edge_case(name)
+    if name == "adjoint_voxel":
+        c = cases.adjoint_check_case(0, "voxel")
+    elif name.startswith("random"):
+        c = cases.random_case(int(name[6:]))
+    else:
+        c = cases.edge_case(name)
     shape, tile = c["shape"], c["tile"]
     _announce(f"fused vs unfused {name} shape={shape} tile={tile}")
     vox = TileVoxelizer(shape, tile_size=tile, device=DEV)
     rng = np.random.default_rng(4)
-    g_r = torch.from_numpy(rng.normal(size=shape).astype(np.float32)).to(DEV)
-    g_i = torch.from_numpy(rng.normal(size=shape).astype(np.float32)).to(DEV)
+    g_r_np = rng.normal(size=shape).astype(np.float32)
+    g_i_np = rng.normal(size=shape).astype(np.float32)
+    g_r, g_i = torch.from_numpy(g_r_np).to(DEV), torch.from_numpy(g_i_np).to(DEV)
     out = {}
     for mode in ("fused", "unfused"):
         leaves = [torch.from_numpy(np.ascontiguousarray(a)).to(DEV).requires_grad_()
@@ -170,11 +193,26 @@ def test_fused_call_matches_unfused_composition(name):
         vr, vi = fn(pos, torch.exp(log_s), rot, torch.complex(dr, di))
         ((vr * g_r).sum() + (vi * g_i).sum()).backward()
         out[mode] = [vr.detach(), vi.detach()] + [t.grad for t in leaves]
+        if mode == "fused":
+            s_gpu = torch.exp(log_s).detach().cpu().numpy()
     names = ["vol_r", "vol_i", "pos", "log_s", "rot", "dr", "di"]
+    vtol, gtol = FUSED_TOL.get(name, (2e-6, 2e-5))
     for nm, a, b in zip(names, out["fused"], out["unfused"]):
         e = cases.rel_l2(a.cpu().numpy(), b.cpu().numpy())
         print(f"  {nm}: fused vs unfused rel L2 {e:.2e}")
-        assert e < 2e-6, (nm, e)
+        assert e < (vtol if nm.startswith("vol") else gtol), (nm, e)
+    # Both against FP64 autograd of the reference's call (the fused call's own
+    # parity, not only its agreement with the composition).
+    ref, ok, vol64 = _fp64_reference(c, s_gpu, g_r_np, g_i_np)
+    vtol, gtol = FP64_TOL.get(name, (1e-5, 1e-4))
+    e = cases.rel_l2(np.concatenate([out["fused"][0].cpu().numpy().ravel(), out["fused"][1].cpu().numpy().ravel()]),
+                     vol64)
+    print(f"  volume: fused vs FP64 rel L2 {e:.2e}")
+    assert e <= vtol, ("volume", e)
+    for nm, a in zip(names[2:], out["fused"][2:]):
+        e = cases.rel_l2(a.cpu().numpy()[ok], ref[nm][ok])
+        print(f"  d/d{nm}: fused vs FP64 rel L2 {e:.2e}")
+        assert e <= gtol, (nm, e)
 
 
 def test_fused_call_density_layouts():

@@ -78,3 +78,23 @@ def test_pair_count_limit_is_refused_before_any_allocation():
 
     with pytest.raises(RuntimeError, match="2\\^31"):
         _native.bin_fill(torch.zeros(1, 3), torch.zeros(1), (8, 8, 8), 8, torch.zeros(2, dtype=torch.int32), 2**31)
+
+
+def test_density_stride_of_views():
+    """Density views the entry points accept (host logic only, CPU tensors)."""
+    import torch
+
+    planar = torch.zeros(5)
+    assert _native._rho_stride(planar, planar, keep=False) == 1
+    inter = torch.view_as_real(torch.zeros(5, dtype=torch.complex64))
+    assert _native._rho_stride(inter[:, 0], inter[:, 1], keep=True) == 2
+    with pytest.raises(ValueError, match="strided"):
+        _native._rho_stride(inter[:, 0], inter[:, 1], keep=False)
+    # One element: torch calls the stride-2 real part of a one-element complex
+    # tensor contiguous, and the one address read does not depend on the stride.
+    one = torch.zeros(1, dtype=torch.complex64)
+    re_part, im_part = one.real.contiguous(), one.imag.contiguous()
+    assert re_part.stride(0) == 2
+    assert _native._rho_stride(re_part, im_part, keep=False) == 1
+    with pytest.raises(ValueError, match="1-D"):
+        _native._rho_stride(torch.zeros(2, 2), torch.zeros(2, 2), keep=False)
