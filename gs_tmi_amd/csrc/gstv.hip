@@ -893,6 +893,16 @@ __device__ __forceinline__ void prep_one(int g, const float c[3], const float mm
       e2 = (float)x2;
     }
   }
+  // A non-finite centre gives q = NaN or +inf at every voxel (the form is
+  // positive semi-definite), so the reference's cutoff test skips all of
+  // them: no contribution and zero gradients.  Such a record gets a finite
+  // centre and inverted extents, which every box test reads as empty, so no
+  // kernel evaluates it (voxel offsets from a non-finite centre would turn
+  // the adjoint's zero-weight moments into NaN).
+  if (!(isfinite(r.c0) && isfinite(r.c1) && isfinite(r.c2))) {
+    r.c0 = r.c1 = r.c2 = 0.0f;
+    e0 = e1 = e2 = -1.0f;
+  }
   r.e0 = e0;
   r.e1 = e1;
   r.e2 = e2;
@@ -1845,6 +1855,10 @@ __device__ __forceinline__ void finalize_one(const float* __restrict__ gacc, con
 #pragma unroll
     for (int j = 0; j < 3; ++j) gm[3 * i + j] = mm[i][j] / (hs[i] * hs[j]);
   }
+  // The reference accumulates the centre gradient over the voxels that pass
+  // the cutoff only: with none (zero first moments) it is exactly zero, also
+  // where cov_inv is not finite (then no voxel passes).
+  if (a[2] == 0.0f && a[3] == 0.0f && a[4] == 0.0f) gc[0] = gc[1] = gc[2] = 0.0f;
 }
 
 __global__ __launch_bounds__(256) void k_finalize(const float* __restrict__ gacc, const float* __restrict__ cov,

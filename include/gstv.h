@@ -29,6 +29,11 @@
  *     normalised by half_shape), density planes rho_r [N], rho_i [N].
  *   - half_shape is a device array of 3 floats (D/2, H/2, W/2 for the reference
  *     caller); the kernels read it on the device as the reference kernel does.
+ *   - A Gaussian with a non-finite centre coordinate or cov_inv fails the
+ *     reference's cutoff test (mahal <= 9, tile_voxelize_cuda_kernel.cu:130,233)
+ *     at every voxel: it adds nothing and its gradients are exactly zero.  A
+ *     non-finite density reaches every voxel the kernels evaluate for its
+ *     Gaussian (the reference's only those within the cutoff).
  */
 #ifndef GSTV_H
 #define GSTV_H

@@ -339,6 +339,107 @@ def test_empty_and_degenerate_inputs():
     assert all(t.numel() == 0 for t in out)
 
 
+BAD = [5, 6, 7, 8, 9]  # NaN, +inf, -inf centre coordinate; NaN cov_inv (whole, one entry)
+
+
+@pytest.mark.parametrize("path", ["plain", "keep"])
+def test_non_finite_gaussians_contribute_nothing(path):
+    """Gaussians with a non-finite centre coordinate or cov_inv fail the
+    reference's cutoff test (mahal <= 9, cuda_ext/tile_voxelize_cuda_kernel.cu:130,233)
+    at every voxel: they add nothing to the volume and get exactly zero
+    gradients, and the other Gaussians' results are those of the same call
+    without them.  Binning of the non-finite centres is bit-exact with the
+    oracle's (device float -> int cast semantics)."""
+    from gs_tmi_amd import _native, build_tile_pairs
+
+    c = cases.edge_case("partial_tiles")
+    shape, tile = c["shape"], c["tile"]
+    d, h, w = shape
+    _announce(f"non-finite Gaussians, {path} entry points, shape={shape} tile={tile}")
+    center, radius, cov, hs = _prologue_gpu(c)
+    n = center.shape[0]
+    center, cov = center.clone(), cov.clone().reshape(n, 9)
+    center[5, 0], center[6, 1], center[7, 2] = float("nan"), float("inf"), -float("inf")
+    cov[8] = float("nan")
+    cov[9, 0] = float("nan")
+    good = torch.ones(n, dtype=torch.bool, device=DEV)
+    good[BAD] = False
+    rr = torch.from_numpy(c["rr"]).to(DEV)
+    ri = torch.from_numpy(c["ri"]).to(DEV)
+    rng = np.random.default_rng(11)
+    g_r = torch.from_numpy(rng.normal(size=d * h * w).astype(np.float32)).to(DEV)
+    g_i = torch.from_numpy(rng.normal(size=d * h * w).astype(np.float32)).to(DEV)
+
+    def run(cen, rad, cv, a, b):
+        if path == "plain":
+            ranges, pids = build_tile_pairs(cen, rad, shape, tile)
+            vol = _native.forward(cen, cv, a, b, ranges, pids, hs, tile, d, h, w)
+            grads = _native.backward(cen, cv, a, b, ranges, pids, hs, g_r, g_i, tile, d, h, w)
+            return ranges, pids, vol, grads
+        kept = _native.keep_prepare(cen, cv, a, b, hs, rad, shape, tile)
+        ranges, pids = _native.bin_fill(cen, rad, shape, tile, kept.offsets, int(kept.total.item()), kept=kept)
+        vr, vi, kept = _native.forward(cen, cv, a, b, ranges, pids, hs, tile, d, h, w, keep=kept)
+        grads = _native.backward(cen, cv, a, b, ranges, pids, hs, g_r, g_i, tile, d, h, w, kept=kept)
+        return ranges, pids, (vr, vi), grads
+
+    ranges, pids, vol, grads = run(center, radius, cov, rr, ri)
+    _, _, vol_ref, grads_ref = run(center[good], radius[good], cov[good], rr[good], ri[good])
+    torch.cuda.synchronize()
+    if path == "plain":
+        r_exp, p_exp = vo.bin_tiles(center.cpu().numpy(), radius.cpu().numpy(), shape, tile)
+        np.testing.assert_array_equal(ranges.cpu().numpy(), r_exp)
+        np.testing.assert_array_equal(pids.cpu().numpy(), p_exp)
+    for a, b in zip(vol, vol_ref):
+        assert bool(torch.isfinite(a).all())
+        e = cases.rel_l2(a.cpu().numpy(), b.cpu().numpy())
+        print(f"  volume against the call without them: rel L2 {e:.2e}")
+        assert e < 1e-6
+    for nm, a, b in zip(["center", "cov_inv", "rho_r", "rho_i"], grads, grads_ref):
+        bad_rows = a[~good]
+        assert int(torch.count_nonzero(bad_rows)) == 0, (nm, bad_rows)
+        e = cases.rel_l2(a[good].cpu().numpy(), b.cpu().numpy())
+        print(f"  d/d{nm}: others against the call without them: rel L2 {e:.2e}")
+        assert e < 5e-6, (nm, e)
+
+
+def test_fused_call_with_non_finite_positions():
+    """The fused call with NaN / inf positions: those Gaussians add nothing and
+    every parameter gradient of theirs is zero (as through the reference's
+    call, where the kernel's cutoff test fails at every voxel)."""
+    from gs_tmi_amd import TileVoxelizer
+
+    c = cases.edge_case("partial_tiles")
+    shape, tile = c["shape"], c["tile"]
+    _announce(f"fused call, non-finite positions, shape={shape} tile={tile}")
+    vox = TileVoxelizer(shape, tile_size=tile, device=DEV)
+    n = len(c["rr"])
+    pos = c["pos"].copy()
+    pos[5, 0], pos[6, 1], pos[7, 2] = np.nan, np.inf, -np.inf
+    bad = np.zeros(n, bool)
+    bad[[5, 6, 7]] = True
+    rng = np.random.default_rng(12)
+    g_r = torch.from_numpy(rng.normal(size=shape).astype(np.float32)).to(DEV)
+    g_i = torch.from_numpy(rng.normal(size=shape).astype(np.float32)).to(DEV)
+    out = []
+    for keep in (np.ones(n, bool), ~bad):
+        leaves = [torch.from_numpy(np.ascontiguousarray(a[keep])).to(DEV).requires_grad_()
+                  for a in (pos, np.log(c["scales"]), c["rot"], c["rr"], c["ri"])]
+        vr, vi = vox(leaves[0], torch.exp(leaves[1]), leaves[2], torch.complex(leaves[3], leaves[4]))
+        ((vr * g_r).sum() + (vi * g_i).sum()).backward()
+        out.append([vr.detach(), vi.detach()] + [t.grad for t in leaves])
+    for a, b in zip(out[0][:2], out[1][:2]):
+        assert bool(torch.isfinite(a).all())
+        e = cases.rel_l2(a.cpu().numpy(), b.cpu().numpy())
+        print(f"  volume against the call without them: rel L2 {e:.2e}")
+        assert e < 1e-6
+    for nm, a, b in zip(["pos", "log_s", "rot", "dr", "di"], out[0][2:], out[1][2:]):
+        a = a.cpu().numpy()
+        assert np.all(a[bad] == 0.0), (nm, a[bad])
+        e = cases.rel_l2(a[~bad], b.cpu().numpy())
+        print(f"  d/d{nm}: others against the call without them: rel L2 {e:.2e}")
+        assert e < 1e-5, (nm, e)
+
+
 @pytest.mark.parametrize("shape,tile,frac", [((64, 64, 64), 8, 0.97), ((64, 64, 64), 8, 0.85),
                                              ((20, 20, 20), 1, 0.85), ((8, 8, 8), 8, 0.85),
                                              ((48, 48, 40), 1, 0.5)])
