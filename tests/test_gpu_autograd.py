@@ -327,3 +327,4 @@ def test_fused_call_without_gaussians_or_pairs():
         for t in (pos, log_s, rot, dens):
             assert t.grad is not None and t.grad.shape == t.shape
             assert float(t.grad.abs().sum()) == 0.0
+

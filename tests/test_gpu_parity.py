@@ -339,6 +339,75 @@ def test_empty_and_degenerate_inputs():
     assert all(t.numel() == 0 for t in out)
 
 
+@pytest.mark.parametrize("keep", [False, True], ids=["plain", "keep"])
+def test_degenerate_scales_vs_fp64(keep):
+    """Gaussians with a normalised scale of 1e-12 on every axis (cov_inv ~1e16 I,
+    below the prologue's 1e-8 guard, about 1e-6 voxels wide) next to ordinary
+    ones, most of them centred on voxels: the tile kernels against FP64 on the
+    same fp32 centres and forms (volume, cov_inv and density gradients of all,
+    centre gradients of the ordinary ones).  Only a voxel within ~1e-6 of a degenerate
+    centre can pass the cutoff, and its fp64 extent box holds no other.  (A
+    single degenerate axis next to ordinary ones leaves a thin tilted pancake
+    whose fp32 form cancels catastrophically within a voxel of its plane, in
+    the reference's arithmetic as in any fp32 evaluation: no parity there.)"""
+    from gs_tmi_amd import _native
+
+    shape, tile, n = (24, 20, 28), 8, 90
+    d, h, w = shape
+    rng = np.random.default_rng(21)
+    hs = np.asarray(shape, np.float64) / 2
+    vox = rng.integers(0, shape, size=(n, 3)).astype(np.float64)
+    pos = ((vox + 0.5) / hs - 1.0).astype(np.float32)
+    pos[n // 4: n // 2] = rng.uniform(-0.9, 0.9, size=(n // 2 - n // 4, 3)).astype(np.float32)
+    scales = (rng.uniform(0.7, 2.0, size=(n, 3)) / hs).astype(np.float32)
+    scales[: n // 2] = 1e-12
+    q = rng.normal(size=(n, 4)).astype(np.float32)
+    rot = (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(np.float32)
+    c = dict(shape=shape, tile=tile, pos=pos, scales=scales, rot=rot,
+             rr=rng.normal(size=n).astype(np.float32), ri=rng.normal(size=n).astype(np.float32))
+    _announce(f"degenerate scales, {'keep' if keep else 'plain'} entry points, shape={shape} tile={tile} n={n}")
+    center, cov, rr, ri, ranges, pids, hs_t = _function_inputs(c)
+    args = (center, cov, rr, ri, ranges, pids, hs_t)
+    g_r = rng.normal(size=d * h * w).astype(np.float32)
+    g_i = rng.normal(size=d * h * w).astype(np.float32)
+    gr_t, gi_t = torch.from_numpy(g_r).to(DEV), torch.from_numpy(g_i).to(DEV)
+    if keep:
+        vol_r, vol_i, kept = _native.forward(*args, tile, d, h, w, keep=True)
+        gc, gm, grr, gri = _native.backward(*args, gr_t, gi_t, tile, d, h, w, kept=kept)
+    else:
+        vol_r, vol_i = _native.forward(*args, tile, d, h, w)
+        gc, gm, grr, gri = _native.backward(*args, gr_t, gi_t, tile, d, h, w)
+    torch.cuda.synchronize()
+    nargs = (center.cpu().numpy(), cov.cpu().numpy(), c["rr"], c["ri"], ranges.cpu().numpy(), pids.cpu().numpy(),
+             hs_t.cpu().numpy())
+    er, ei, amb = vo.forward(*nargs, shape, tile, band=BAND)
+    ok = ~amb
+    assert np.isfinite(vol_r.cpu().numpy()).all() and np.isfinite(vol_i.cpu().numpy()).all()
+    e_fwd = cases.rel_l2(np.concatenate([vol_r.cpu().numpy()[ok], vol_i.cpu().numpy()[ok]]),
+                         np.concatenate([er[ok], ei[ok]]))
+    print(f"  forward rel L2 = {e_fwd:.3e} (ambiguous voxels {int(amb.sum())})")
+    assert e_fwd <= TOL
+    dc, dm, drr, dri, gamb = vo.backward(*nargs, g_r, g_i, shape, tile, band=BAND)
+    okg = ~gamb
+    # The adjoint forms a column's H moment sum(w t u1) as u10 sum(w t) +
+    # sum(y w t): absolute error ~ ulp(u10) sum|w t|, relative ~1e-7 voxel / sigma
+    # in the centre gradient (times cov_inv ~ 1 / sigma^2).  For these ~1e-6
+    # voxel wide Gaussians that is O(1), so the centre gradient is checked on
+    # the ordinary ones.
+    ordinary = okg.copy()
+    ordinary[: n // 2] = False
+    print(f"  centre gradient of the degenerate Gaussians: rel L2 "
+          f"{cases.rel_l2(gc.cpu().numpy()[: n // 2], dc[: n // 2]):.2e}")
+    errs = {
+        "center": cases.rel_l2(gc.cpu().numpy()[ordinary], dc[ordinary]),
+        "cov_inv": cases.rel_l2(gm.cpu().numpy().reshape(-1, 3, 3)[okg], dm[okg]),
+        "rho": cases.rel_l2(np.stack([grr.cpu().numpy(), gri.cpu().numpy()], 1)[okg], np.stack([drr, dri], 1)[okg]),
+    }
+    print(f"  backward rel L2 = {errs} (ambiguous Gaussians {int(gamb.sum())})")
+    for k, e in errs.items():
+        assert e <= TOL, (k, e)
+
+
 BAD = [5, 6, 7, 8, 9]  # NaN, +inf, -inf centre coordinate; NaN cov_inv (whole, one entry)
 
 
