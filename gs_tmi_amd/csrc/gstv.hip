@@ -1506,10 +1506,12 @@ __device__ __forceinline__ void bwd_column(LdsFloat* __restrict__ sh_g, int tp, 
   bool ok_lane = g_ok;
   if (kPartial) ok_lane = ok_lane && (s2 + xi < b.e2);
   float m = 0.0f, m0 = 0.0f, m1 = 0.0f;
-  float u1v[kSub];
+  float u1v[kSub], u1q[kSub];
 #pragma unroll
-  for (int y = 0; y < kSub; ++y) u1v[y] = (float)(b.o1 + s1 + y) - c1;
-  const float u10 = u1v[0];
+  for (int y = 0; y < kSub; ++y) {
+    u1v[y] = (float)(b.o1 + s1 + y) - c1;
+    u1q[y] = u1v[y] * u1v[y];
+  }
   // The next row's upstream gradient is loaded from LDS while the current row
   // is evaluated; rows past the lane's own re-read its last row.
   const int zlast_off = (z0 + own_rows - 1) * ZS;
@@ -1546,14 +1548,14 @@ __device__ __forceinline__ void bwd_column(LdsFloat* __restrict__ sh_g, int tp, 
       acc.di = fmaf(w, g.y, acc.di);
       wt[y] = w * fmaf(ri, g.y, rr * g.x);
     }
-    // Row moments with u1 = u10 + y:  t0 = sum wt,  t1 = sum wt u1 = u10 t0 + s1,
-    // t2 = sum wt u1^2 = u10^2 t0 + 2 u10 s1 + s2 = u10 (t1 + s1) + s2, where
-    // s1 = sum y wt_y and s2 = sum y^2 wt_y.
+    // Row moments t0 = sum wt, t1 = sum wt u1, t2 = sum wt u1^2 from each
+    // voxel's own offset (exact in fp32) and its square: forming them from
+    // the first row's offset, u1 = u10 + y, would leave an absolute error of
+    // ~ulp(u10) sum|wt| that cov_inv magnifies in the centre gradient of
+    // Gaussians much narrower than a voxel.
     const float t0 = (wt[0] + wt[1]) + (wt[2] + wt[3]);
-    const float s1y = fmaf(3.0f, wt[3], fmaf(2.0f, wt[2], wt[1]));
-    const float s2y = fmaf(9.0f, wt[3], fmaf(4.0f, wt[2], wt[1]));
-    const float t1 = fmaf(u10, t0, s1y);
-    const float t2 = fmaf(u10, t1 + s1y, s2y);
+    const float t1 = fmaf(wt[3], u1v[3], fmaf(wt[2], u1v[2], fmaf(wt[1], u1v[1], wt[0] * u1v[0])));
+    const float t2 = fmaf(wt[3], u1q[3], fmaf(wt[2], u1q[2], fmaf(wt[1], u1q[1], wt[0] * u1q[0])));
     m += t0;
     const float tz = t0 * u0;
     m0 += tz;
