@@ -418,11 +418,13 @@ def backward_params(center_vox, cov_inv, scales, rotations, tile_ranges, pair_id
 
 
 def count_evaluations(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape, tile_size, d, h, w):
-    """Exact (E_total, E_active, E_culled) of the tile kernels for this binning (synchronises)."""
+    """Exact (E_total, E_active, E_culled, E_adjoint) of the tile kernels for
+    this binning (synchronises): the reference kernel's evaluations, those with
+    q <= 9, and those the forward and the backward kernel perform."""
     lib = get_lib()
     dev = center_vox.device
     n = center_vox.shape[0]
-    counts = torch.zeros(3, dtype=torch.int64, device=dev)
+    counts = torch.zeros(4, dtype=torch.int64, device=dev)
     ws_bytes = lib.gstv_forward_workspace_bytes(n)
     ws = _workspace(ws_bytes, dev)
     with torch.cuda.device(dev):

@@ -119,6 +119,9 @@ __device__ __forceinline__ int xcd_remap(int b, int grid) {
   return (b & 7) * per + (b >> 3);
 }
 
+// A value that is NaN or +-inf (one v_cmp_class).
+__device__ __forceinline__ bool non_finite(float x) { return !__builtin_isfinite(x); }
+
 // ---------------------------------------------------------------- scanning --
 
 __device__ __forceinline__ int64_t wave_inclusive_scan(int64_t v, int lane) {
@@ -233,11 +236,16 @@ __global__ void k_fill_f32(float* __restrict__ p, int64_t n, float v) {
 
 // ----------------------------------------------------------------- binning --
 
-// Record of one Gaussian for the tile kernels (built by k_prep).
+// Record of one Gaussian for the tile kernels (built by k_prep).  The
+// exponent q' = -log2(e)/2 * q of voxel offset u (axes D, H, W = u0, u1, u2)
+// is kept as squares of linear forms, eliminated in the kernels' loop order
+// (H innermost, then D, W fixed per lane):
+//   q' = -(y1^2 + y2^2 + y3w u2^2),  y1 = y1h u1 + y1d u0 + y1w u2,
+//                                     y2 = y2d u0 + y2w u2.
 struct __align__(16) GRec {
   float c0, c1, c2, rr;      // centre (voxel units, axes D,H,W), density real
-  float ri, a00, a11, a22;   // density imag, diagonal of the scaled form
-  float b01, b02, b12, e0;   // doubled off-diagonals of the scaled form; extent D
+  float ri, y1h, y1d, y1w;   // density imag; first linear form
+  float y2d, y2w, y3w, e0;   // second linear form, u2^2 coefficient; extent D
   float e1, e2, pad0, pad1;  // extents H, W
 };
 static_assert(sizeof(GRec) == 64, "record must be one 64-byte line");
@@ -834,45 +842,92 @@ __global__ void k_ranges(const int32_t* __restrict__ keys, int total, int32_t* _
 
 // -------------------------------------------------------- Gaussian records --
 
+struct ZAxis {
+  float al, ga, ta, rho;
+};
+struct ZCoefs {
+  ZAxis h, w;  // projections on (D, H) and (D, W)
+  bool ok;     // false: no bound (extents infinite, or the form is not definite)
+};
+
+// The two projections of one Gaussian, built once per call by k_prep (32 B);
+// h.rho < 0 stands for "no bound".
+struct __align__(16) ZRec {
+  ZAxis h, w;
+};
+static_assert(sizeof(ZRec) == 32, "column-bound record must be 32 bytes");
+
+__device__ __forceinline__ ZCoefs z_load(const ZRec* __restrict__ zt, int g) {
+  const ZRec z = zt[g];
+  return ZCoefs{z.h, z.w, z.h.rho > 0.0f};
+}
+
+
 
 // Record of one Gaussian.  The quadratic form of the reference,
-// q = d^T M d with d = (x - c) / hs, is rewritten in voxel units u = x - c as
-// q = sum_ij A_ij u_i u_j with A = sym(M) / (hs hs^T), and pre-scaled by
-// -log2(e)/2 so that w = exp2(q').  The extent e_a bounds |u_a| on the
-// ellipsoid q <= 9: e_a = 3 hs_a sqrt((S^-1)_aa), S = sym(M), evaluated in
-// fp64 and widened by 2% + 0.01 voxel so that no voxel whose fp32 q passes the
-// cutoff is culled.  A matrix that is not positive definite, or not finite,
-// gets an infinite extent (no culling).  Gaussian g's density is
-// rho_r[g * rho_stride], rho_i[g * rho_stride] (stride 2 with rho_i = rho_r + 1
-// reads an interleaved complex64 array).
+// q = d^T M d with d = (x - c) / hs, is q = u^T A u in voxel units u = x - c
+// with A = sym(M) / (hs hs^T).  The record keeps q' = -log2(e)/2 q (so that
+// w = exp2(q')) as squares of linear forms -- a Cholesky factorisation of
+// P = log2(e)/2 A in the order (u1, u0, u2) -- whose fp32 evaluation stays
+// accurate for strongly anisotropic Gaussians: an expanded sum
+// sum_ij A_ij u_i u_j cancels terms as large as the condition number times q.
+// The factors are formed in fp64 from the fp32 M and rounded once.  A first
+// pivot P11 is a sum of squares (never negative); a second pivot that is not
+// positive (a form that is not positive definite, from near-degenerate fp32
+// M) is clamped to zero, its cross term dropped.  A non-finite M gives NaN
+// factors (q = NaN: no voxel passes, as in the reference).
+//
+// The extent e_a bounds |u_a| on the ellipsoid q <= 9: e_a = 3 hs_a
+// sqrt((S^-1)_aa), S = sym(M), evaluated in fp64 and widened by 2% + 0.01
+// voxel so that no voxel whose fp32 q passes the cutoff is culled.  A matrix
+// that is not positive definite, or not finite, gets an infinite extent (no
+// culling).  The column-bound projections (ZRec, below) come from the same
+// fp64 form.  Gaussian g's density is rho_r[g * rho_stride],
+// rho_i[g * rho_stride] (stride 2 with rho_i = rho_r + 1 reads an interleaved
+// complex64 array).
 //
 // counts (optional): the Gaussian's pairs for binning cut to its extent box
 // (tile_box with this record; radius and geo as for k_bin_count), and
 // counts[n] = 0 for the scan.
-// Record (and, with counts, pair count) of Gaussian g from its centre c and
-// matrix mm (the fp32 values of center_vox and cov_inv).
+// Record, column-bound record (and, with counts, pair count) of Gaussian g
+// from its centre c and matrix mm (the fp32 values of center_vox and cov_inv).
 __device__ __forceinline__ void prep_one(int g, const float c[3], const float mm[9], float hs0, float hs1, float hs2,
                                          const float* __restrict__ rho_r, const float* __restrict__ rho_i,
-                                         int rho_stride, GRec* __restrict__ rec, float rad, const BinGeom& geo,
-                                         int32_t* __restrict__ counts) {
-  const float kq = (float)kQScaleD;
+                                         int rho_stride, GRec* __restrict__ rec, ZRec* __restrict__ zt, float rad,
+                                         const BinGeom& geo, int32_t* __restrict__ counts) {
   GRec r;
   r.c0 = c[0];
   r.c1 = c[1];
   r.c2 = c[2];
   r.rr = rho_r[(int64_t)g * rho_stride];
   r.ri = rho_i[(int64_t)g * rho_stride];
-  r.a00 = kq * (mm[0] / (hs0 * hs0));
-  r.a11 = kq * (mm[4] / (hs1 * hs1));
-  r.a22 = kq * (mm[8] / (hs2 * hs2));
-  r.b01 = kq * ((mm[1] + mm[3]) / (hs0 * hs1));
-  r.b02 = kq * ((mm[2] + mm[6]) / (hs0 * hs2));
-  r.b12 = kq * ((mm[5] + mm[7]) / (hs1 * hs2));
-  // Extents in fp64.
+  // Voxel-unit form, scaled by log2(e)/2 (P) for the factors.
+  const double h0 = hs0, h1 = hs1, h2 = hs2;
   const double s00 = mm[0], s11 = mm[4], s22 = mm[8];
   const double s01 = 0.5 * ((double)mm[1] + mm[3]);
   const double s02 = 0.5 * ((double)mm[2] + mm[6]);
   const double s12 = 0.5 * ((double)mm[5] + mm[7]);
+  const double A00 = s00 / (h0 * h0), A11 = s11 / (h1 * h1), A22 = s22 / (h2 * h2);
+  const double A01 = s01 / (h0 * h1), A02 = s02 / (h0 * h2), A12 = s12 / (h1 * h2);
+  {
+    const double k = -kQScaleD;
+    const double p11 = k * A11, p01 = k * A01, p12 = k * A12, p00 = k * A00, p02 = k * A02, p22 = k * A22;
+    const double y1h = p11 > 0.0 ? sqrt(p11) : 0.0;
+    const double y1d = y1h > 0.0 ? p01 / y1h : 0.0, y1w = y1h > 0.0 ? p12 / y1h : 0.0;
+    const double q00 = p00 - y1d * y1d, q02 = p02 - y1d * y1w, q22 = p22 - y1w * y1w;
+    const double y2d = q00 > 0.0 ? sqrt(q00) : 0.0;
+    const double y2w = y2d > 0.0 ? q02 / y2d : 0.0;
+    const double y3w = q22 - y2w * y2w;
+    const bool fin = isfinite(p00) && isfinite(p11) && isfinite(p22) && isfinite(p01) && isfinite(p02) &&
+                     isfinite(p12);
+    r.y1h = fin ? (float)y1h : NAN;
+    r.y1d = fin ? (float)y1d : NAN;
+    r.y1w = fin ? (float)y1w : NAN;
+    r.y2d = fin ? (float)y2d : NAN;
+    r.y2w = fin ? (float)y2w : NAN;
+    r.y3w = fin ? (float)y3w : NAN;
+  }
+  // Extents in fp64.
   const double cof00 = s11 * s22 - s12 * s12;
   const double cof11 = s00 * s22 - s02 * s02;
   const double cof22 = s00 * s11 - s01 * s01;
@@ -909,6 +964,30 @@ __device__ __forceinline__ void prep_one(int g, const float c[3], const float mm
   r.pad0 = 0.0f;
   r.pad1 = 0.0f;
   rec[g] = r;
+  // Column-bound projections in voxel units (C = A^-1, cofactors of A):
+  //   al = C_0i / C_ii = cof_0i / cof_ii,  ga = sqrt(C_00 C_ii - C_0i^2) / C_ii
+  //   = sqrt(A_jj det A) / cof_ii (Jacobi), rho_i = e_i - 0.01 = R sqrt(C_ii),
+  //   ta = al rho_i^2 / rho_0; for i = H (j = W) and i = W (j = H).
+  {
+    const double c00 = A11 * A22 - A12 * A12, c11 = A00 * A22 - A02 * A02, c22 = A00 * A11 - A01 * A01;
+    const double c01 = A02 * A12 - A01 * A22, c02 = A01 * A12 - A02 * A11;
+    const double dA = A00 * c00 + A01 * c01 + A02 * c02;
+    const double r0 = (double)(e0 - 0.01f), r1 = (double)(e1 - 0.01f), r2 = (double)(e2 - 0.01f);
+    ZRec z;
+    z.h.al = (float)(c01 / c11);
+    z.h.ga = (float)(sqrt(A22 * dA) / c11);
+    z.h.rho = (float)r1;
+    z.h.ta = (float)(c01 / c11 * (r1 * r1 / r0));
+    z.w.al = (float)(c02 / c22);
+    z.w.ga = (float)(sqrt(A11 * dA) / c22);
+    z.w.rho = (float)r2;
+    z.w.ta = (float)(c02 / c22 * (r2 * r2 / r0));
+    const bool ok = c11 > 0.0 && c22 > 0.0 && dA > 0.0 && r0 > 0.0 && r1 > 0.0 && r2 > 0.0 && isfinite(r0) &&
+                    isfinite(r1) && isfinite(r2) && isfinite(z.h.al) && isfinite(z.h.ga) && isfinite(z.h.ta) &&
+                    isfinite(z.w.al) && isfinite(z.w.ga) && isfinite(z.w.ta);
+    if (!ok) z.h.rho = z.w.rho = -1.0f;
+    zt[g] = z;
+  }
   if (counts != nullptr) {
     const float tc[3] = {r.c0, r.c1, r.c2}, te[3] = {e0, e1, e2};
     int tmin[3], tcnt[3];
@@ -920,8 +999,9 @@ __device__ __forceinline__ void prep_one(int g, const float c[3], const float mm
 __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ center, const float* __restrict__ cov,
                                               const float* __restrict__ rho_r, const float* __restrict__ rho_i,
                                               int rho_stride, const float* __restrict__ half_shape, int n,
-                                              GRec* __restrict__ rec, const float* __restrict__ radius,
-                                              BinGeom geo, int32_t* __restrict__ counts) {
+                                              GRec* __restrict__ rec, ZRec* __restrict__ zt,
+                                              const float* __restrict__ radius, BinGeom geo,
+                                              int32_t* __restrict__ counts) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (counts != nullptr && g == n) counts[n] = 0;
   if (g >= n) return;
@@ -930,7 +1010,7 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ center, 
 #pragma unroll
   for (int i = 0; i < 9; ++i) mm[i] = m[i];
   const float c[3] = {center[3 * (int64_t)g + 0], center[3 * (int64_t)g + 1], center[3 * (int64_t)g + 2]};
-  prep_one(g, c, mm, half_shape[0], half_shape[1], half_shape[2], rho_r, rho_i, rho_stride, rec,
+  prep_one(g, c, mm, half_shape[0], half_shape[1], half_shape[2], rho_r, rho_i, rho_stride, rec, zt,
            counts != nullptr ? radius[g] : 0.0f, geo, counts);
 }
 
@@ -953,42 +1033,6 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ center, 
 //     al = cof_0i / cof_ii,  ga = sqrt(A'_jj det A') / cof_ii.
 // The determinant cancels badly for needle-shaped Gaussians, so cofactors and
 // determinant are formed in fp64 from the fp32 form the kernels evaluate.
-struct ZAxis {
-  float al, ga, ta, rho;
-};
-struct ZCoefs {
-  ZAxis h, w;  // projections on (D, H) and (D, W)
-  bool ok;     // false: no bound (extents infinite, or the form is not definite)
-};
-
-__device__ __forceinline__ ZCoefs z_coefs(float a00, float a11, float a22, float b01, float b02, float b12, float e0,
-                                          float e1, float e2) {
-  const double s00 = a00, s11 = a11, s22 = a22;
-  const double s01 = 0.5 * (double)b01, s02 = 0.5 * (double)b02, s12 = 0.5 * (double)b12;
-  const double cof00 = s11 * s22 - s12 * s12;
-  const double cof11 = s00 * s22 - s02 * s02;
-  const double cof22 = s00 * s11 - s01 * s01;
-  const double cof01 = s02 * s12 - s01 * s22;
-  const double cof02 = s01 * s12 - s02 * s11;
-  const double det = s00 * cof00 + s01 * cof01 + s02 * cof02;
-  const float dt = (float)det;
-  const float i11 = __builtin_amdgcn_rcpf((float)cof11), i22 = __builtin_amdgcn_rcpf((float)cof22);
-  const float r0 = e0 - 0.01f, r1 = e1 - 0.01f, r2 = e2 - 0.01f;
-  ZCoefs z;
-  z.h.al = (float)cof01 * i11;
-  z.h.ga = __builtin_amdgcn_sqrtf(a22 * dt) * i11;
-  z.h.rho = r1;
-  z.h.ta = z.h.al * (r1 * r1 * __builtin_amdgcn_rcpf(r0));
-  z.w.al = (float)cof02 * i22;
-  z.w.ga = __builtin_amdgcn_sqrtf(a11 * dt) * i22;
-  z.w.rho = r2;
-  z.w.ta = z.w.al * (r2 * r2 * __builtin_amdgcn_rcpf(r0));
-  z.ok = cof11 > 0.0 && cof22 > 0.0 && det < 0.0 && r0 > 0.0f && r1 > 0.0f && r2 > 0.0f && isfinite(r0) &&
-         isfinite(r1) && isfinite(r2) && isfinite(z.h.al) && isfinite(z.h.ga) && isfinite(z.h.ta) &&
-         isfinite(z.w.al) && isfinite(z.w.ga) && isfinite(z.w.ta);
-  return z;
-}
-
 // D extent [zmin, zmax] (relative to the centre) of the projected ellipse over
 // the strip u_i in [lo, hi].  Clamps are single med3 instructions: with a
 // non-finite strip (non-finite centre) the result is arbitrary, which is
@@ -1023,13 +1067,21 @@ struct VoxGeom {
 
 struct Stage {
   float c0[kBatch], c1[kBatch], c2[kBatch], rr[kBatch], ri[kBatch];
-  float a00[kBatch], a11[kBatch], a22[kBatch], b01[kBatch], b02[kBatch], b12[kBatch];
-  float e0[kBatch], e1[kBatch], e2[kBatch];
+  float y1h[kBatch], y1d[kBatch], y1w[kBatch], y2d[kBatch], y2w[kBatch], y3w[kBatch];
+  int32_t gid[kBatch];  // Gaussian id of each staged pair
   uint32_t box[kBatch];
   uint8_t sbm[kBatch];  // sub-bricks touched, bit 4*j0 + 2*j1 + j2 (tiles of at most 2x2x2 sub-bricks)
   uint16_t list[kWaves][kBatch];
   uint16_t touch[kBatch];  // pairs whose box touches the tile (any order)
   int n_touch;
+  // Staged pairs (touching the tile) whose density is not finite: their
+  // staged density is zeroed and their own density kept here.  The count
+  // alternates between two slots by batch parity, so one slot is cleared
+  // while the other is read.
+  uint16_t bad_k[kBatch];
+  float bad_rr[kBatch], bad_ri[kBatch];
+  int n_bad[2];
+  float bad_acc_r[kTileThreads], bad_acc_i[kTileThreads];  // their sums, one slot per thread's voxel
 };
 
 struct BlockCtx {
@@ -1081,21 +1133,19 @@ __device__ __forceinline__ BlockCtx make_ctx(int logical, const VoxGeom& geo, co
 // exact extent clipped to the tile, in tile-local voxel indices, packed as
 // 6 x 5 bits (lo0, lo1, lo2, hi0, hi1, hi2); kEmptyBox if it misses the tile.
 // Returns whether the box touches the tile.
-__device__ __forceinline__ bool stage_pair(Stage& sh, int k, const GRec& r, const BlockCtx& b) {
+__device__ __forceinline__ bool stage_pair(Stage& sh, int k, const GRec& r, int gid, const BlockCtx& b) {
   sh.c0[k] = r.c0;
   sh.c1[k] = r.c1;
   sh.c2[k] = r.c2;
   sh.rr[k] = r.rr;
   sh.ri[k] = r.ri;
-  sh.a00[k] = r.a00;
-  sh.a11[k] = r.a11;
-  sh.a22[k] = r.a22;
-  sh.b01[k] = r.b01;
-  sh.b02[k] = r.b02;
-  sh.b12[k] = r.b12;
-  sh.e0[k] = r.e0;
-  sh.e1[k] = r.e1;
-  sh.e2[k] = r.e2;
+  sh.y1h[k] = r.y1h;
+  sh.y1d[k] = r.y1d;
+  sh.y1w[k] = r.y1w;
+  sh.y2d[k] = r.y2d;
+  sh.y2w[k] = r.y2w;
+  sh.y3w[k] = r.y3w;
+  sh.gid[k] = gid;
   // fmaxf / fminf return the non-NaN operand, so a NaN centre keeps the whole
   // tile (its q is NaN and fails the cutoff, as in the reference).
   const float lo0 = fmaxf(ceilf(r.c0 - r.e0), (float)b.o0);
@@ -1129,14 +1179,14 @@ __device__ __forceinline__ uint32_t col_field(int z_lo, int z_hi) {
 // staged pair whose box touches a tile of at most 2 sub-brick per axis: the
 // box's (H, W) column blocks, each with the D rows its column bound leaves.
 // The same rows are returned as a column word in `cols`.
-__device__ __forceinline__ uint32_t pair_mask(const Stage& sh, int k, const BlockCtx& b, uint32_t& cols) {
+__device__ __forceinline__ uint32_t pair_mask(const Stage& sh, int k, const BlockCtx& b, const ZRec* __restrict__ zt,
+                                              uint32_t& cols) {
   const float c0 = sh.c0[k], c1 = sh.c1[k], c2 = sh.c2[k];
   const uint32_t bx = sh.box[k];
   const int l0 = bx & 31, l1 = (bx >> 5) & 31, l2 = (bx >> 10) & 31;
   const int h0 = (bx >> 15) & 31, h1 = (bx >> 20) & 31, h2 = (bx >> 25) & 31;
   const float lo0 = (float)(b.o0 + l0), hi0 = (float)(b.o0 + h0);
-  const ZCoefs zc = z_coefs(sh.a00[k], sh.a11[k], sh.a22[k], sh.b01[k], sh.b02[k], sh.b12[k], sh.e0[k], sh.e1[k],
-                            sh.e2[k]);
+  const ZCoefs zc = z_load(zt, sh.gid[k]);
   float hmn[2], hmx[2], wmn[2], wmx[2];
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
@@ -1185,12 +1235,13 @@ __device__ __forceinline__ void append_touch(Stage& sh, bool touch, int k, int l
 // threads (pairs that do not touch the tile cost nothing here), and, when
 // pair_cols is given, their column words at the pairs' positions bs + k.
 // Called between two barriers.
-__device__ __forceinline__ void batch_masks(Stage& sh, const BlockCtx& b, int bs, uint32_t* __restrict__ pair_cols) {
+__device__ __forceinline__ void batch_masks(Stage& sh, const BlockCtx& b, int bs, const ZRec* __restrict__ zt,
+                                            uint32_t* __restrict__ pair_cols) {
   const int nt = sh.n_touch;
   for (int i = threadIdx.x; i < nt; i += kTileThreads) {
     const int k = sh.touch[i];
     uint32_t cols;
-    sh.sbm[k] = (uint8_t)pair_mask(sh, k, b, cols);
+    sh.sbm[k] = (uint8_t)pair_mask(sh, k, b, zt, cols);
     if (pair_cols != nullptr) pair_cols[bs + k] = cols;
   }
 }
@@ -1269,38 +1320,46 @@ __device__ __forceinline__ int row16_max(int v) {
   return v;
 }
 
+// One lane's view of a Gaussian: the W offset u2 of its column is fixed, so
+// the u2 terms of both linear forms and of the u2^2 term are per-lane values.
 struct GaussLane {
-  float c0, c1, rr, ri, a00, a11, b01;
-  float k0, k1, k2;  // u2-dependent terms of the form
+  float c0, rr, ri, y1h, y1d, y2d;
+  float g1, h2, c3;  // y1w u2, y2w u2, y3w u2^2
   float u1[kSub];    // H offsets of the 4 sub-brick rows
-  float u2;
 };
 
 struct GaussRaw {
-  float c0, c1, c2, rr, ri, a00, a11, a22, b01, b02, b12;
+  float c0, c1, c2, rr, ri, y1h, y1d, y1w, y2d, y2w, y3w;
 };
 
 __device__ __forceinline__ GaussRaw load_raw(const Stage& sh, int k) {
-  return GaussRaw{sh.c0[k], sh.c1[k], sh.c2[k], sh.rr[k], sh.ri[k], sh.a00[k], sh.a11[k], sh.a22[k], sh.b01[k],
-                  sh.b02[k], sh.b12[k]};
+  return GaussRaw{sh.c0[k], sh.c1[k], sh.c2[k], sh.rr[k], sh.ri[k], sh.y1h[k], sh.y1d[k], sh.y1w[k], sh.y2d[k],
+                  sh.y2w[k], sh.y3w[k]};
 }
 
 __device__ __forceinline__ GaussLane lane_terms(const GaussRaw& r, const BlockCtx& b, int xi) {
   GaussLane g;
   g.c0 = r.c0;
-  g.c1 = r.c1;
   g.rr = r.rr;
   g.ri = r.ri;
-  g.a00 = r.a00;
-  g.a11 = r.a11;
-  g.b01 = r.b01;
-  g.u2 = (float)(b.o2 + b.s2 + xi) - r.c2;
-  g.k0 = r.a22 * g.u2 * g.u2;
-  g.k1 = r.b02 * g.u2;
-  g.k2 = r.b12 * g.u2;
+  g.y1h = r.y1h;
+  g.y1d = r.y1d;
+  g.y2d = r.y2d;
+  const float u2 = (float)(b.o2 + b.s2 + xi) - r.c2;
+  g.g1 = r.y1w * u2;
+  g.h2 = r.y2w * u2;
+  g.c3 = r.y3w * u2 * u2;
 #pragma unroll
-  for (int y = 0; y < kSub; ++y) g.u1[y] = (float)(b.o1 + b.s1 + y) - g.c1;
+  for (int y = 0; y < kSub; ++y) g.u1[y] = (float)(b.o1 + b.s1 + y) - r.c1;
   return g;
+}
+
+// Per D row u0: the H-independent part r = -(y2^2 + y3w u2^2) of q' and the
+// part t1 of y1 without its H term; per voxel q' = -(y1h u1 + t1)^2 + r.
+__device__ __forceinline__ void row_terms(const GaussLane& g, float u0, float& t1, float& r) {
+  t1 = fmaf(g.y1d, u0, g.g1);
+  const float y2 = fmaf(g.y2d, u0, g.h2);
+  r = fmaf(-y2, y2, -g.c3);
 }
 
 // Evaluates the wave's compacted Gaussians 16 at a time over its sub-brick.
@@ -1318,13 +1377,13 @@ __device__ __forceinline__ void fwd_groups(const Stage& sh, int nhit, const Bloc
     for (int z = 0; z < kSub; ++z) {
       if (!kPartial || z < b.n0) {
         const float u0 = (float)(b.o0 + b.s0 + z) - g.c0;
-        const float A = fmaf(fmaf(g.a00, u0, g.k1), u0, g.k0);
-        const float B = fmaf(g.b01, u0, g.k2);
+        float t1, r;
+        row_terms(g, u0, t1, r);
 #pragma unroll
         for (int y = 0; y < kSub; ++y) {
           if (!kPartial || y < b.n1) {
-            const float u1 = g.u1[y];
-            const float q = fmaf(fmaf(g.a11, u1, B), u1, A);
+            const float y1 = fmaf(g.y1h, g.u1[y], t1);
+            const float q = fmaf(-y1, y1, r);
             const bool ok = g_ok && (q >= kThr);
             const float w = ok ? __builtin_amdgcn_exp2f(q) : 0.0f;
             acc_r[z * kSub + y] = fmaf(w, g.rr, acc_r[z * kSub + y]);
@@ -1336,8 +1395,38 @@ __device__ __forceinline__ void fwd_groups(const Stage& sh, int nhit, const Bloc
   }
 }
 
-__global__ __launch_bounds__(kTileThreads) void k_forward(const GRec* __restrict__ rec,
-                                                          const int32_t* __restrict__ ranges,
+// Contributions of the batch's staged Gaussians whose density is not finite
+// (their staged density is zero, so the groups above add nothing for them).
+// Thread (wave, gl, xi) evaluates the voxel (z, y, x) = (gl / 4, gl % 4, xi)
+// of the wave's sub-brick -- the voxel it writes -- and adds w rho to its own
+// LDS slot only where the cutoff test passes, as the reference does (0 * inf
+// would be NaN).  The quadratic form is evaluated as in fwd_groups.
+__device__ __forceinline__ void fwd_non_finite(Stage& sh, int nbad, const BlockCtx& b, int gl, int xi) {
+  const int z = gl >> 2, y = gl & 3;
+  if (!(z < b.n0 && y < b.n1 && xi < b.n2)) return;
+  for (int j = 0; j < nbad; ++j) {
+    const int k = sh.bad_k[j];
+    const float u2 = (float)(b.o2 + b.s2 + xi) - sh.c2[k];
+    const float u0 = (float)(b.o0 + b.s0 + z) - sh.c0[k];
+    const float u1 = (float)(b.o1 + b.s1 + y) - sh.c1[k];
+    const float t1 = fmaf(sh.y1d[k], u0, sh.y1w[k] * u2);
+    const float y2 = fmaf(sh.y2d[k], u0, sh.y2w[k] * u2);
+    const float r = fmaf(-y2, y2, -(sh.y3w[k] * u2 * u2));
+    const float y1 = fmaf(sh.y1h[k], u1, t1);
+    const float q = fmaf(-y1, y1, r);
+    if (q >= kThr) {
+      const float w = __builtin_amdgcn_exp2f(q);
+      sh.bad_acc_r[threadIdx.x] = fmaf(w, sh.bad_rr[j], sh.bad_acc_r[threadIdx.x]);
+      sh.bad_acc_i[threadIdx.x] = fmaf(w, sh.bad_ri[j], sh.bad_acc_i[threadIdx.x]);
+    }
+  }
+}
+
+// At most 128 VGPRs (4 waves per SIMD, two blocks per CU); the few values that
+// do not fit are reloaded once per batch, outside the evaluation loops.
+__global__ __launch_bounds__(kTileThreads, 4) void k_forward(const GRec* __restrict__ rec,
+                                                             const ZRec* __restrict__ zt,
+                                                             const int32_t* __restrict__ ranges,
                                                           const int32_t* __restrict__ pair_ids, VoxGeom geo,
                                                           float* __restrict__ vol_r, float* __restrict__ vol_i,
                                                           uint32_t* __restrict__ pair_cols) {
@@ -1362,28 +1451,62 @@ __global__ __launch_bounds__(kTileThreads) void k_forward(const GRec* __restrict
   // Records of the next batch are loaded into registers while the current
   // batch is evaluated (loads stay in flight across the LDS barriers).
   GRec nxt = {};
-  if ((int)threadIdx.x < min(kBatch, b.p_end - b.p_begin)) nxt = rec[pair_ids[b.p_begin + threadIdx.x]];
+  int nxt_id = 0;
+  if ((int)threadIdx.x < min(kBatch, b.p_end - b.p_begin)) {
+    nxt_id = pair_ids[b.p_begin + threadIdx.x];
+    nxt = rec[nxt_id];
+  }
   const bool use_mask = geo.nsb <= 2;
-  if (threadIdx.x == 0) sh.n_touch = 0;
-  for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
+  if (threadIdx.x == 0) {
+    sh.n_touch = 0;
+    sh.n_bad[0] = sh.n_bad[1] = 0;
+  }
+  sh.bad_acc_r[threadIdx.x] = 0.0f;
+  sh.bad_acc_i[threadIdx.x] = 0.0f;
+  bool any_bad = false;  // block-uniform
+  int par = 0;
+  for (int bs = b.p_begin; bs < b.p_end; bs += kBatch, par ^= 1) {
     const int cnt = min(kBatch, b.p_end - bs);
     __syncthreads();
-    const bool touch = (int)threadIdx.x < cnt && stage_pair(sh, threadIdx.x, nxt, b);
+    const bool touch = (int)threadIdx.x < cnt && stage_pair(sh, threadIdx.x, nxt, nxt_id, b);
+    if (touch && (non_finite(nxt.rr) || non_finite(nxt.ri))) {
+      // Zero weights times a non-finite density would be NaN: such a
+      // Gaussian is evaluated on its own below.
+      sh.rr[threadIdx.x] = 0.0f;
+      sh.ri[threadIdx.x] = 0.0f;
+      const int j = atomicAdd(&sh.n_bad[par], 1);
+      sh.bad_k[j] = (uint16_t)threadIdx.x;
+      sh.bad_rr[j] = nxt.rr;
+      sh.bad_ri[j] = nxt.ri;
+    }
     const int bn = bs + kBatch;
-    if (bn < b.p_end && (int)threadIdx.x < min(kBatch, b.p_end - bn)) nxt = rec[pair_ids[bn + threadIdx.x]];
+    if (bn < b.p_end && (int)threadIdx.x < min(kBatch, b.p_end - bn)) {
+      nxt_id = pair_ids[bn + threadIdx.x];
+      nxt = rec[nxt_id];
+    }
+    int nbad;
     if (use_mask) {
       append_touch(sh, touch, threadIdx.x, lane);
       __syncthreads();
-      batch_masks(sh, b, bs, pair_cols);
+      nbad = sh.n_bad[par];
+      batch_masks(sh, b, bs, zt, pair_cols);
     }
     __syncthreads();
-    if (threadIdx.x == 0) sh.n_touch = 0;
+    if (!use_mask) nbad = sh.n_bad[par];
+    if (threadIdx.x == 0) {
+      sh.n_touch = 0;
+      sh.n_bad[par ^ 1] = 0;  // the next batch's count (every wave has read the last batch's)
+    }
     if (!b.active) continue;
     const int nhit = build_list(sh, cnt, b, wave, lane, use_mask);
     if (full)
       fwd_groups<false>(sh, nhit, b, wave, gl, xi, x_ok, acc_r, acc_i);
     else
       fwd_groups<true>(sh, nhit, b, wave, gl, xi, x_ok, acc_r, acc_i);
+    if (nbad > 0) {
+      any_bad = true;
+      fwd_non_finite(sh, nbad, b, gl, xi);
+    }
   }
 
   // Sum the 16 Gaussian slots of each W column (a row of 16 lanes).
@@ -1398,6 +1521,10 @@ __global__ __launch_bounds__(kTileThreads) void k_forward(const GRec* __restrict
   for (int i = 1; i < kSub * kSub; ++i) {
     out_r = (gl == i) ? acc_r[i] : out_r;
     out_i = (gl == i) ? acc_i[i] : out_i;
+  }
+  if (any_bad) {
+    out_r += sh.bad_acc_r[threadIdx.x];
+    out_i += sh.bad_acc_i[threadIdx.x];
   }
   const int z = gl >> 2, y = gl & 3;
   if (b.active && x_ok && z < b.n0 && y < b.n1) {
@@ -1465,7 +1592,7 @@ __host__ __device__ constexpr int grad_zs(int t) { return t * (t + 2) + 2; }
 template <int MC>
 struct BwdStage {
   float c0[kBatch], c1[kBatch], c2[kBatch], rr[kBatch], ri[kBatch];
-  float a00[kBatch], a11[kBatch], a22[kBatch], b01[kBatch], b02[kBatch], b12[kBatch];
+  float y1h[kBatch], y1d[kBatch], y1w[kBatch], y2d[kBatch], y2w[kBatch], y3w[kBatch];
   int32_t gid[kBatch];            // Gaussian id of each staged pair
   uint16_t cols[kBatch][MC];      // live columns in (H, W) block order
   uint16_t key[kBatch];           // (columns - 1) * 16 + largest row count - 1, or kNoKey
@@ -1493,15 +1620,19 @@ typedef __attribute__((address_space(3))) const float LdsFloat;
 
 __device__ __forceinline__ float2 lds_g(LdsFloat* __restrict__ g, int i) { return make_float2(g[2 * i], g[2 * i + 1]); }
 
-template <int TT, bool kPartial>
+// kSafe: the group's densities or the tile's upstream gradient hold a
+// non-finite value; contributions are then selected by the cutoff test
+// instead of weighted by a zero (0 * inf is NaN), so, as in the reference,
+// only evaluations with q <= 9 add anything.
+template <int TT, bool kPartial, bool kSafe>
 __device__ __forceinline__ void bwd_column(LdsFloat* __restrict__ sh_g, int tp, const BlockCtx& b, int z0,
                                            int rows, int own_rows, int s1, int s2, int xi, bool g_ok, float c0,
-                                           float c1, float c2, float rr, float ri, float a00, float a11, float a22,
-                                           float b01, float b02, float b12, BwdAcc& acc) {
+                                           float c1, float c2, float rr, float ri, float y1h, float y1d, float y1w,
+                                           float y2d, float y2w, float y3w, BwdAcc& acc) {
   const int T = TT > 0 ? TT : tp;
   const int YS = grad_ys(T), ZS = grad_zs(T);
   const float u2 = (float)(b.o2 + s2 + xi) - c2;
-  const float k0 = a22 * u2 * u2, k1 = b02 * u2, k2 = b12 * u2;
+  const float g1 = y1w * u2, h2 = y2w * u2, c3 = y3w * u2 * u2;
   const int gbase = s1 * YS + s2 + xi;
   bool ok_lane = g_ok;
   if (kPartial) ok_lane = ok_lane && (s2 + xi < b.e2);
@@ -1522,9 +1653,12 @@ __device__ __forceinline__ void bwd_column(LdsFloat* __restrict__ sh_g, int tp, 
   float u0 = (float)(b.o0 + z0) - c0;
 #pragma unroll 2
   for (int r = 0; r < rows; ++r) {
-    float A = fmaf(fmaf(a00, u0, k1), u0, k0);
-    A = r < own_rows ? A : -INFINITY;
-    const float B = fmaf(b01, u0, k2);
+    // q' = -(y1h u1 + p1)^2 + rz: rz = -(y2^2 + y3w u2^2), and -inf on rows
+    // past the lane's own (no voxel passes).
+    const float p1 = fmaf(y1d, u0, g1);
+    const float y2 = fmaf(y2d, u0, h2);
+    float rz = fmaf(-y2, y2, -c3);
+    rz = r < own_rows ? rz : -INFINITY;
     float2 gc[kSub];
 #pragma unroll
     for (int y = 0; y < kSub; ++y) gc[y] = gn[y];
@@ -1537,16 +1671,22 @@ __device__ __forceinline__ void bwd_column(LdsFloat* __restrict__ sh_g, int tp, 
     float wt[kSub];
 #pragma unroll
     for (int y = 0; y < kSub; ++y) {
-      const float u1 = u1v[y];
-      float Ay = A;
-      if (kPartial) Ay = (s1 + y < b.e1) ? A : -INFINITY;
-      const float q = fmaf(fmaf(a11, u1, B), u1, Ay);
+      float ry = rz;
+      if (kPartial) ry = (s1 + y < b.e1) ? rz : -INFINITY;
+      const float y1 = fmaf(y1h, u1v[y], p1);
+      const float q = fmaf(-y1, y1, ry);
       const bool ok = ok_lane && (q >= kThr);
       const float w = ok ? __builtin_amdgcn_exp2f(q) : 0.0f;
       const float2 g = gc[y];
-      acc.dr = fmaf(w, g.x, acc.dr);
-      acc.di = fmaf(w, g.y, acc.di);
-      wt[y] = w * fmaf(ri, g.y, rr * g.x);
+      if (kSafe) {
+        acc.dr = ok ? fmaf(w, g.x, acc.dr) : acc.dr;
+        acc.di = ok ? fmaf(w, g.y, acc.di) : acc.di;
+        wt[y] = ok ? w * fmaf(ri, g.y, rr * g.x) : 0.0f;
+      } else {
+        acc.dr = fmaf(w, g.x, acc.dr);
+        acc.di = fmaf(w, g.y, acc.di);
+        wt[y] = w * fmaf(ri, g.y, rr * g.x);
+      }
     }
     // Row moments t0 = sum wt, t1 = sum wt u1, t2 = sum wt u1^2 from each
     // voxel's own offset (exact in fp32) and its square: forming them from
@@ -1578,6 +1718,7 @@ __device__ __forceinline__ void bwd_column(LdsFloat* __restrict__ sh_g, int tp, 
 // column words of the same pairs, used instead of recomputing the bound.
 template <int TT>
 __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restrict__ rec,
+                                                           const ZRec* __restrict__ zt,
                                                            const int32_t* __restrict__ ranges,
                                                            const int32_t* __restrict__ pair_ids, VoxGeom geo,
                                                            const float* __restrict__ grad_r,
@@ -1603,16 +1744,21 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
   if (!b.valid) return;
   const bool partial = b.e1 < T || b.e2 < T;  // D rows are always clipped to the tile
 
+  int g_bad = 0;
   for (int i = tid; i < T * T * T; i += kTileThreads) {
     const int z = i / (T * T), y = (i / T) % T, x = i % T;
     float2 g = make_float2(0.0f, 0.0f);
     if (z < b.e0 && y < b.e1 && x < b.e2) {
       const int64_t flat = ((int64_t)(b.o0 + z) * geo.H + (b.o1 + y)) * geo.W + (b.o2 + x);
       g = make_float2(grad_r[flat], grad_i[flat]);
+      g_bad |= (non_finite(g.x) || non_finite(g.y)) ? 1 : 0;
     }
     sh_g[z * grad_zs(T) + y * grad_ys(T) + x] = g;
   }
   for (int i = tid; i < kKeys; i += kTileThreads) sh.key_cnt[i] = 0;
+  // A non-finite upstream gradient anywhere in the tile sends every group of
+  // the tile down the select path (block-uniform).
+  const bool tile_bad = __syncthreads_or(g_bad) != 0;
 
   // Records of the next batch are loaded into registers while the current
   // batch is evaluated (loads stay in flight across the LDS barriers).
@@ -1638,12 +1784,12 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
       sh.c2[tid] = r.c2;
       sh.rr[tid] = r.rr;
       sh.ri[tid] = r.ri;
-      sh.a00[tid] = r.a00;
-      sh.a11[tid] = r.a11;
-      sh.a22[tid] = r.a22;
-      sh.b01[tid] = r.b01;
-      sh.b02[tid] = r.b02;
-      sh.b12[tid] = r.b12;
+      sh.y1h[tid] = r.y1h;
+      sh.y1d[tid] = r.y1d;
+      sh.y1w[tid] = r.y1w;
+      sh.y2d[tid] = r.y2d;
+      sh.y2w[tid] = r.y2w;
+      sh.y3w[tid] = r.y3w;
       uint16_t key = kNoKey;
       const float lo0 = fmaxf(ceilf(r.c0 - r.e0), (float)b.o0);
       const float lo1 = fmaxf(ceilf(r.c1 - r.e1), (float)b.o1);
@@ -1676,7 +1822,7 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
         const int y_lo = (int)lo1 - b.o1, y_hi = (int)hi1 - b.o1;  // tile-local box
         const int x_lo = (int)lo2 - b.o2, x_hi = (int)hi2 - b.o2;
         const int l1 = y_lo >> 2, h1 = y_hi >> 2, l2 = x_lo >> 2, h2 = x_hi >> 2;
-        const ZCoefs zc = z_coefs(r.a00, r.a11, r.a22, r.b01, r.b02, r.b12, r.e0, r.e1, r.e2);
+        const ZCoefs zc = z_load(zt, nxt_id);
         float hmn[NB], hmx[NB], wmn[NB], wmx[NB];
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
@@ -1789,21 +1935,33 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
         const bool slot_ok = pos >= seg_begin;
         const int k = sh.order[slot_ok ? pos : top];
         const float c0 = sh.c0[k], c1 = sh.c1[k], c2 = sh.c2[k], rr = sh.rr[k], ri = sh.ri[k];
-        const float a00 = sh.a00[k], a11 = sh.a11[k], a22 = sh.a22[k];
-        const float b01 = sh.b01[k], b02 = sh.b02[k], b12 = sh.b12[k];
+        const float y1h = sh.y1h[k], y1d = sh.y1d[k], y1w = sh.y1w[k];
+        const float y2d = sh.y2d[k], y2w = sh.y2w[k], y3w = sh.y3w[k];
         BwdAcc acc = {};
+        // Lanes of empty slots repeat the group's top Gaussian, so the
+        // wave-wide test covers every density the group multiplies.
+        const bool safe = tile_bad || __ballot(non_finite(rr) || non_finite(ri)) != 0;
         for (int j = 0; j < cc; ++j) {
           const uint16_t cw = sh.cols[k][j];
           const int own_rows = col_rows(cw);
           // The group's largest row count for its j-th columns (over the 16 slots).
           const int rows = row16_max(own_rows);
           const int j1 = cw & 3, j2 = (cw >> 2) & 3, z0 = (cw >> 4) & 15;
-          if (partial)
-            bwd_column<TT, true>((LdsFloat*)sh_g, T, b, z0, rows, own_rows, j1 * kSub, j2 * kSub, xi, slot_ok, c0, c1, c2, rr,
-                                 ri, a00, a11, a22, b01, b02, b12, acc);
-          else
-            bwd_column<TT, false>((LdsFloat*)sh_g, T, b, z0, rows, own_rows, j1 * kSub, j2 * kSub, xi, slot_ok, c0, c1, c2,
-                                  rr, ri, a00, a11, a22, b01, b02, b12, acc);
+          LdsFloat* lg = (LdsFloat*)sh_g;
+          if (safe) {
+            if (partial)
+              bwd_column<TT, true, true>(lg, T, b, z0, rows, own_rows, j1 * kSub, j2 * kSub, xi, slot_ok, c0, c1, c2,
+                                         rr, ri, y1h, y1d, y1w, y2d, y2w, y3w, acc);
+            else
+              bwd_column<TT, false, true>(lg, T, b, z0, rows, own_rows, j1 * kSub, j2 * kSub, xi, slot_ok, c0, c1,
+                                          c2, rr, ri, y1h, y1d, y1w, y2d, y2w, y3w, acc);
+          } else if (partial) {
+            bwd_column<TT, true, false>(lg, T, b, z0, rows, own_rows, j1 * kSub, j2 * kSub, xi, slot_ok, c0, c1, c2,
+                                        rr, ri, y1h, y1d, y1w, y2d, y2w, y3w, acc);
+          } else {
+            bwd_column<TT, false, false>(lg, T, b, z0, rows, own_rows, j1 * kSub, j2 * kSub, xi, slot_ok, c0, c1, c2,
+                                         rr, ri, y1h, y1d, y1w, y2d, y2w, y3w, acc);
+          }
         }
         const float v[kAccStride] = {acc.dr,  acc.di,  acc.m0,  acc.m1,  acc.m2,  acc.m00,
                                      acc.m11, acc.m22, acc.m01, acc.m02, acc.m12, 0.0f};
@@ -1837,30 +1995,34 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
 //   s = -w t / 2;  grad_rho = sum w g;  grad_c_i = -2/hs_i sum_j M_ij sum(s u_j)/hs_j;
 //   grad_M_ij = sum(s u_i u_j) / (hs_i hs_j).
 // One Gaussian's moments -> grad_center (gc), grad_cov_inv (gm), grad_rho.
+// In fp64: for strongly anisotropic Gaussians M (m / hs) cancels -- M's large
+// eigenvalue multiplies the small component of m along its eigenvector -- and
+// an fp32 product would add its own rounding to that of the fp32 moments.
 __device__ __forceinline__ void finalize_one(const float* __restrict__ gacc, const float* __restrict__ cov,
-                                             const float hs[3], int g, float gc[3], float gm[9], float& grr,
+                                             const float hs[3], int g, double gc[3], double gm[9], float& grr,
                                              float& gri) {
   const float* a = gacc + (int64_t)g * kAccStride;
   grr = a[0];
   gri = a[1];
-  const float m[3] = {-0.5f * a[2], -0.5f * a[3], -0.5f * a[4]};
-  const float mm[3][3] = {{-0.5f * a[5], -0.5f * a[8], -0.5f * a[9]},
-                          {-0.5f * a[8], -0.5f * a[6], -0.5f * a[10]},
-                          {-0.5f * a[9], -0.5f * a[10], -0.5f * a[7]}};
+  const double m[3] = {-0.5 * a[2], -0.5 * a[3], -0.5 * a[4]};
+  const double mm[3][3] = {{-0.5 * a[5], -0.5 * a[8], -0.5 * a[9]},
+                           {-0.5 * a[8], -0.5 * a[6], -0.5 * a[10]},
+                           {-0.5 * a[9], -0.5 * a[10], -0.5 * a[7]}};
+  const double h[3] = {hs[0], hs[1], hs[2]};
   const float* M = cov + 9 * (int64_t)g;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    float acc = 0.0f;
+    double acc = 0.0;
 #pragma unroll
-    for (int j = 0; j < 3; ++j) acc = fmaf(M[3 * i + j], m[j] / hs[j], acc);
-    gc[i] = -2.0f * acc / hs[i];
+    for (int j = 0; j < 3; ++j) acc = fma((double)M[3 * i + j], m[j] / h[j], acc);
+    gc[i] = -2.0 * acc / h[i];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) gm[3 * i + j] = mm[i][j] / (hs[i] * hs[j]);
+    for (int j = 0; j < 3; ++j) gm[3 * i + j] = mm[i][j] / (h[i] * h[j]);
   }
   // The reference accumulates the centre gradient over the voxels that pass
   // the cutoff only: with none (zero first moments) it is exactly zero, also
   // where cov_inv is not finite (then no voxel passes).
-  if (a[2] == 0.0f && a[3] == 0.0f && a[4] == 0.0f) gc[0] = gc[1] = gc[2] = 0.0f;
+  if (a[2] == 0.0f && a[3] == 0.0f && a[4] == 0.0f) gc[0] = gc[1] = gc[2] = 0.0;
 }
 
 __global__ __launch_bounds__(256) void k_finalize(const float* __restrict__ gacc, const float* __restrict__ cov,
@@ -1871,14 +2033,15 @@ __global__ __launch_bounds__(256) void k_finalize(const float* __restrict__ gacc
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= n) return;
   const float hs[3] = {half_shape[0], half_shape[1], half_shape[2]};
-  float gc[3], gm[9], grr, gri;
+  double gc[3], gm[9];
+  float grr, gri;
   finalize_one(gacc, cov, hs, g, gc, gm, grr, gri);
   g_rr[(int64_t)g * rho_stride] = grr;
   g_ri[(int64_t)g * rho_stride] = gri;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) g_center[3 * (int64_t)g + i] = gc[i];
+  for (int i = 0; i < 3; ++i) g_center[3 * (int64_t)g + i] = (float)gc[i];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) g_cov[9 * (int64_t)g + i] = gm[i];
+  for (int i = 0; i < 9; ++i) g_cov[9 * (int64_t)g + i] = (float)gm[i];
 }
 
 // ------------------------------------------------------ prologue / epilogue --
@@ -1959,7 +2122,7 @@ __global__ __launch_bounds__(256) void k_prologue_prep(const float* __restrict__
                                                        float* __restrict__ half_shape,
                                                        const float* __restrict__ rho_r,
                                                        const float* __restrict__ rho_i, int rho_stride,
-                                                       GRec* __restrict__ rec, BinGeom geo,
+                                                       GRec* __restrict__ rec, ZRec* __restrict__ zt, BinGeom geo,
                                                        int32_t* __restrict__ counts) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g == 0) {
@@ -1971,7 +2134,7 @@ __global__ __launch_bounds__(256) void k_prologue_prep(const float* __restrict__
   if (g >= n) return;
   float c[3], M[9], rad;
   prologue_one(g, pos, scales, rot, sh0, sh1, sh2, center, radius, cov, c, M, rad);
-  prep_one(g, c, M, sh0 * 0.5f, sh1 * 0.5f, sh2 * 0.5f, rho_r, rho_i, rho_stride, rec, rad, geo, counts);
+  prep_one(g, c, M, sh0 * 0.5f, sh1 * 0.5f, sh2 * 0.5f, rho_r, rho_i, rho_stride, rec, zt, rad, geo, counts);
 }
 
 // Chain rule of the prologue: from (dL/dc, dL/dM) to positions, linear scales
@@ -1980,25 +2143,30 @@ __global__ __launch_bounds__(256) void k_prologue_prep(const float* __restrict__
 // dL/dR through R(q^) and the normalisation q^ = q / (|q| + 1e-8).  The
 // radius carries no gradient (the reference computes it under no_grad).
 // One Gaussian's chain rule through the prologue, from its grad_center (gc)
-// and grad_cov_inv (gm).
+// and grad_cov_inv (gm), in fp64 (N-sized work; the products of large
+// 1/scale factors with gradient terms of both signs cancel for strongly
+// anisotropic Gaussians).  The parameters are the fp32 values the forward
+// read; the derivative is that of the prologue's mathematics at them.
 __device__ __forceinline__ void epilogue_one(const float* __restrict__ scales, const float* __restrict__ rot, int g,
-                                             const float gc[3], const float gm[9], float sh0, float sh1, float sh2,
+                                             const double gc[3], const double gm[9], float sh0, float sh1, float sh2,
                                              float* __restrict__ g_pos, float* __restrict__ g_scales,
                                              float* __restrict__ g_rot) {
   const int64_t g3 = 3 * (int64_t)g;
-  g_pos[g3 + 0] = gc[0] * (0.5f * sh0);
-  g_pos[g3 + 1] = gc[1] * (0.5f * sh1);
-  g_pos[g3 + 2] = gc[2] * (0.5f * sh2);
-  const float s[3] = {scales[g3 + 0], scales[g3 + 1], scales[g3 + 2]};
+  g_pos[g3 + 0] = (float)(gc[0] * (0.5 * sh0));
+  g_pos[g3 + 1] = (float)(gc[1] * (0.5 * sh1));
+  g_pos[g3 + 2] = (float)(gc[2] * (0.5 * sh2));
+  const double s[3] = {scales[g3 + 0], scales[g3 + 1], scales[g3 + 2]};
   const float4 q4 = *reinterpret_cast<const float4*>(rot + 4 * (int64_t)g);
-  const float nr = sqrtf(q4.x * q4.x + q4.y * q4.y + q4.z * q4.z + q4.w * q4.w);
-  const float nrm = nr + 1e-8f;
-  const float w = q4.x / nrm, x = q4.y / nrm, y = q4.z / nrm, z = q4.w / nrm;
-  const float R[3][3] = {{1.0f - 2.0f * y * y - 2.0f * z * z, 2.0f * x * y - 2.0f * z * w, 2.0f * x * z + 2.0f * y * w},
-                         {2.0f * x * y + 2.0f * z * w, 1.0f - 2.0f * x * x - 2.0f * z * z, 2.0f * y * z - 2.0f * x * w},
-                         {2.0f * x * z - 2.0f * y * w, 2.0f * y * z + 2.0f * x * w, 1.0f - 2.0f * x * x - 2.0f * y * y}};
-  const float si[3] = {1.0f / (s[0] + 1e-8f), 1.0f / (s[1] + 1e-8f), 1.0f / (s[2] + 1e-8f)};
-  float T[3][3], G[3][3];
+  const double qw = q4.x, qx = q4.y, qy = q4.z, qz = q4.w;
+  const double nr = sqrt(qw * qw + qx * qx + qy * qy + qz * qz);
+  const double nrm = nr + 1e-8;
+  const double inv = 1.0 / nrm;
+  const double w = qw * inv, x = qx * inv, y = qy * inv, z = qz * inv;
+  const double R[3][3] = {{1.0 - 2.0 * y * y - 2.0 * z * z, 2.0 * x * y - 2.0 * z * w, 2.0 * x * z + 2.0 * y * w},
+                          {2.0 * x * y + 2.0 * z * w, 1.0 - 2.0 * x * x - 2.0 * z * z, 2.0 * y * z - 2.0 * x * w},
+                          {2.0 * x * z - 2.0 * y * w, 2.0 * y * z + 2.0 * x * w, 1.0 - 2.0 * x * x - 2.0 * y * y}};
+  const double si[3] = {1.0 / (s[0] + 1e-8), 1.0 / (s[1] + 1e-8), 1.0 / (s[2] + 1e-8)};
+  double T[3][3], G[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -2006,34 +2174,34 @@ __device__ __forceinline__ void epilogue_one(const float* __restrict__ scales, c
       T[i][k] = R[i][k] * si[k];
       G[i][k] = gm[3 * i + k];
     }
-  float GR[3][3], gsi[3] = {0.0f, 0.0f, 0.0f};
+  double GR[3][3], gsi[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-      float gt = 0.0f;
+      double gt = 0.0;
 #pragma unroll
       for (int j = 0; j < 3; ++j) gt += (G[a][j] + G[j][a]) * T[j][b];
       GR[a][b] = gt * si[b];
       gsi[b] += gt * R[a][b];
     }
 #pragma unroll
-  for (int k = 0; k < 3; ++k) g_scales[g3 + k] = -si[k] * si[k] * gsi[k];
-  const float gw = 2.0f * (x * (GR[2][1] - GR[1][2]) + y * (GR[0][2] - GR[2][0]) + z * (GR[1][0] - GR[0][1]));
-  const float gx = 2.0f * (y * (GR[0][1] + GR[1][0]) + z * (GR[0][2] + GR[2][0]) + w * (GR[2][1] - GR[1][2]) -
-                           2.0f * x * (GR[1][1] + GR[2][2]));
-  const float gy = 2.0f * (x * (GR[0][1] + GR[1][0]) + z * (GR[1][2] + GR[2][1]) + w * (GR[0][2] - GR[2][0]) -
-                           2.0f * y * (GR[0][0] + GR[2][2]));
-  const float gz = 2.0f * (x * (GR[0][2] + GR[2][0]) + y * (GR[1][2] + GR[2][1]) + w * (GR[1][0] - GR[0][1]) -
-                           2.0f * z * (GR[0][0] + GR[1][1]));
+  for (int k = 0; k < 3; ++k) g_scales[g3 + k] = (float)(-si[k] * si[k] * gsi[k]);
+  const double gw = 2.0 * (x * (GR[2][1] - GR[1][2]) + y * (GR[0][2] - GR[2][0]) + z * (GR[1][0] - GR[0][1]));
+  const double gx = 2.0 * (y * (GR[0][1] + GR[1][0]) + z * (GR[0][2] + GR[2][0]) + w * (GR[2][1] - GR[1][2]) -
+                           2.0 * x * (GR[1][1] + GR[2][2]));
+  const double gy = 2.0 * (x * (GR[0][1] + GR[1][0]) + z * (GR[1][2] + GR[2][1]) + w * (GR[0][2] - GR[2][0]) -
+                           2.0 * y * (GR[0][0] + GR[2][2]));
+  const double gz = 2.0 * (x * (GR[0][2] + GR[2][0]) + y * (GR[1][2] + GR[2][1]) + w * (GR[1][0] - GR[0][1]) -
+                           2.0 * z * (GR[0][0] + GR[1][1]));
   // Through q^ = q / (|q| + eps): dL/dq = g / (n+eps) - q (g . q) / (n (n+eps)^2); no second term at n = 0.
-  const float dot = gw * q4.x + gx * q4.y + gy * q4.z + gz * q4.w;
-  const float c2 = nr > 0.0f ? dot / (nr * nrm * nrm) : 0.0f;
+  const double dot = gw * qw + gx * qx + gy * qy + gz * qz;
+  const double c2 = nr > 0.0 ? dot / (nr * nrm * nrm) : 0.0;
   float4 out;
-  out.x = gw / nrm - q4.x * c2;
-  out.y = gx / nrm - q4.y * c2;
-  out.z = gy / nrm - q4.z * c2;
-  out.w = gz / nrm - q4.w * c2;
+  out.x = (float)(gw * inv - qw * c2);
+  out.y = (float)(gx * inv - qx * c2);
+  out.z = (float)(gy * inv - qy * c2);
+  out.w = (float)(gz * inv - qz * c2);
   *reinterpret_cast<float4*>(g_rot + 4 * (int64_t)g) = out;
 }
 
@@ -2044,7 +2212,7 @@ __global__ __launch_bounds__(256) void k_epilogue(const float* __restrict__ scal
                                                   float* __restrict__ g_scales, float* __restrict__ g_rot) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= n) return;
-  float gc[3], gm[9];
+  double gc[3], gm[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i) gc[i] = g_center[3 * (int64_t)g + i];
 #pragma unroll
@@ -2067,16 +2235,64 @@ __global__ __launch_bounds__(256) void k_finalize_epilogue(const float* __restri
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= n) return;
   const float hs[3] = {half_shape[0], half_shape[1], half_shape[2]};
-  float gc[3], gm[9], grr, gri;
+  double gc[3], gm[9];
+  float grr, gri;
   finalize_one(gacc, cov, hs, g, gc, gm, grr, gri);
   g_rr[(int64_t)g * rho_stride] = grr;
   g_ri[(int64_t)g * rho_stride] = gri;
   epilogue_one(scales, rot, g, gc, gm, sh0, sh1, sh2, g_pos, g_scales, g_rot);
 }
 
+// Evaluations the backward makes for one pair (record r, tile of b): over
+// the live columns of its box -- the 4x4 (H, W) column blocks with the D rows
+// their column bound leaves, as k_backward stages them -- rows x the column's
+// valid H x W voxels.  nb: column blocks per tile axis.
+__device__ __forceinline__ unsigned int bwd_pair_evals(const GRec& r, const ZCoefs& zc, const BlockCtx& b, int nb) {
+  const float lo0 = fmaxf(ceilf(r.c0 - r.e0), (float)b.o0);
+  const float lo1 = fmaxf(ceilf(r.c1 - r.e1), (float)b.o1);
+  const float lo2 = fmaxf(ceilf(r.c2 - r.e2), (float)b.o2);
+  const float hi0 = fminf(floorf(r.c0 + r.e0), (float)(b.o0 + b.e0 - 1));
+  const float hi1 = fminf(floorf(r.c1 + r.e1), (float)(b.o1 + b.e1 - 1));
+  const float hi2 = fminf(floorf(r.c2 + r.e2), (float)(b.o2 + b.e2 - 1));
+  if (!(lo0 <= hi0 && lo1 <= hi1 && lo2 <= hi2)) return 0u;
+  const int y_lo = (int)lo1 - b.o1, y_hi = (int)hi1 - b.o1;
+  const int x_lo = (int)lo2 - b.o2, x_hi = (int)hi2 - b.o2;
+  const int l1 = y_lo >> 2, h1 = y_hi >> 2, l2 = x_lo >> 2, h2 = x_hi >> 2;
+  float hmn[kMaxBlk], hmx[kMaxBlk], wmn[kMaxBlk], wmx[kMaxBlk];
+#pragma unroll
+  for (int j = 0; j < kMaxBlk; ++j) {
+    hmn[j] = wmn[j] = -INFINITY;
+    hmx[j] = wmx[j] = INFINITY;
+    if (zc.ok) {
+      const int y0 = max(4 * j, y_lo), y1 = min(4 * j + 3, y_hi);
+      const int x0 = max(4 * j, x_lo), x1 = min(4 * j + 3, x_hi);
+      z_span(zc.h, (float)(b.o1 + y0) - r.c1, (float)(b.o1 + y1) - r.c1, hmn[j], hmx[j]);
+      z_span(zc.w, (float)(b.o2 + x0) - r.c2, (float)(b.o2 + x1) - r.c2, wmn[j], wmx[j]);
+    }
+  }
+  unsigned int total = 0;
+#pragma unroll
+  for (int j1 = 0; j1 < kMaxBlk; ++j1) {
+#pragma unroll
+    for (int j2 = 0; j2 < kMaxBlk; ++j2) {
+      if (j1 < nb && j2 < nb && l1 <= j1 && j1 <= h1 && l2 <= j2 && j2 <= h2) {
+        float zl, zh;
+        column_rows(r.c0, hmn[j1], hmx[j1], wmn[j2], wmx[j2], lo0, hi0, zl, zh);
+        if (zl <= zh) {
+          const int rows = (int)zh - (int)zl + 1;
+          const int nh = min(kSub, b.e1 - kSub * j1), nw = min(kSub, b.e2 - kSub * j2);
+          total += (unsigned int)(rows * nh * nw);
+        }
+      }
+    }
+  }
+  return total;
+}
+
 // Exact evaluation counts for the roofline accounting: the forward kernel's
 // loop structure with the accumulation replaced by counting.
 __global__ __launch_bounds__(kTileThreads) void k_count(const GRec* __restrict__ rec,
+                                                        const ZRec* __restrict__ zt,
                                                         const int32_t* __restrict__ ranges,
                                                         const int32_t* __restrict__ pair_ids, VoxGeom geo,
                                                         unsigned long long* __restrict__ counts) {
@@ -2093,16 +2309,24 @@ __global__ __launch_bounds__(kTileThreads) void k_count(const GRec* __restrict__
     atomicAdd(&counts[0], pairs * (unsigned long long)(b.e0 * b.e1 * b.e2));
   }
   unsigned int active = 0, culled = 0;
+  unsigned long long bwd = 0;
   const bool use_mask = geo.nsb <= 2;
+  const bool first_chunk = logical % geo.chunks == 0;  // the backward's evaluations are counted once per tile
   if (threadIdx.x == 0) sh.n_touch = 0;
   for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
     const int cnt = min(kBatch, b.p_end - bs);
     __syncthreads();
-    const bool touch = (int)threadIdx.x < cnt && stage_pair(sh, threadIdx.x, rec[pair_ids[bs + threadIdx.x]], b);
+    bool touch = false;
+    if ((int)threadIdx.x < cnt) {
+      const int gid = pair_ids[bs + threadIdx.x];
+      const GRec r = rec[gid];
+      touch = stage_pair(sh, threadIdx.x, r, gid, b);
+      if (first_chunk && touch) bwd += bwd_pair_evals(r, z_load(zt, gid), b, geo.nsb);
+    }
     if (use_mask) {
       append_touch(sh, touch, threadIdx.x, lane);
       __syncthreads();
-      batch_masks(sh, b, bs, nullptr);
+      batch_masks(sh, b, bs, zt, nullptr);
     }
     __syncthreads();
     if (threadIdx.x == 0) sh.n_touch = 0;
@@ -2117,12 +2341,13 @@ __global__ __launch_bounds__(kTileThreads) void k_count(const GRec* __restrict__
       for (int z = 0; z < kSub; ++z) {
         if (z < b.n0) {
           const float u0 = (float)(b.o0 + b.s0 + z) - g.c0;
-          const float A = fmaf(fmaf(g.a00, u0, g.k1), u0, g.k0);
-          const float B = fmaf(g.b01, u0, g.k2);
+          float t1, r;
+          row_terms(g, u0, t1, r);
 #pragma unroll
           for (int y = 0; y < kSub; ++y) {
             if (y < b.n1) {
-              const float q = fmaf(fmaf(g.a11, g.u1[y], B), g.u1[y], A);
+              const float y1 = fmaf(g.y1h, g.u1[y], t1);
+              const float q = fmaf(-y1, y1, r);
               active += (g_ok && (q >= kThr)) ? 1u : 0u;
               culled += g_ok ? 1u : 0u;
             }
@@ -2136,10 +2361,12 @@ __global__ __launch_bounds__(kTileThreads) void k_count(const GRec* __restrict__
   for (int off = 32; off > 0; off >>= 1) {
     a += __shfl_xor(a, off, 64);
     c += __shfl_xor(c, off, 64);
+    bwd += __shfl_xor(bwd, off, 64);
   }
   if (lane == 0) {
     atomicAdd(&counts[1], a);
     atomicAdd(&counts[2], c);
+    atomicAdd(&counts[3], bwd);
   }
 }
 
@@ -2201,6 +2428,16 @@ size_t fill_ws_layout(int64_t total, int num_tiles, Carve& cv, int32_t** ka, int
   return align_up(cv.off, 256);
 }
 
+// Forward workspace: the records and the column-bound records.
+size_t fwd_layout(void* base, int32_t n, GRec** rec, ZRec** zt) {
+  Carve cv(base);
+  GRec* r = cv.take<GRec>((size_t)n);
+  ZRec* z = cv.take<ZRec>((size_t)n);
+  if (rec != nullptr) *rec = r;
+  if (zt != nullptr) *zt = z;
+  return align_up(cv.off, 256);
+}
+
 // Opt-in kernel timing (gstv_profile_begin / gstv_profile_end).
 constexpr int kProfMax = 1024;
 struct Profile {
@@ -2229,6 +2466,7 @@ void prof_stop(int kind, int slot, hipStream_t s) {
   if (hipEventRecord(g_prof.stop[kind][slot], s) == hipSuccess) g_prof.used[kind] = slot + 1;
 }
 
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -2246,7 +2484,7 @@ const char* gstv_status_string(int status) {
   }
 }
 
-int gstv_version(void) { return 100; }
+int gstv_version(void) { return 101; }
 
 size_t gstv_bin_count_workspace_bytes(int32_t n) {
   if (n < 0) return 0;
@@ -2417,7 +2655,7 @@ int gstv_bin_fill(const float* center_vox, const float* radius_vox, int32_t n, i
 
 size_t gstv_forward_workspace_bytes(int32_t n) {
   if (n < 0) return 0;
-  return align_up((size_t)n * sizeof(GRec), 256);
+  return fwd_layout(nullptr, n, nullptr, nullptr);
 }
 
 namespace {
@@ -2430,12 +2668,12 @@ int check_tile_args(const int32_t* tile_ranges, int32_t num_tiles, int32_t n, in
 }
 
 int launch_prep(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
-                int rho_stride, const float* half_shape, int32_t n, GRec* rec, hipStream_t s,
+                int rho_stride, const float* half_shape, int32_t n, GRec* rec, ZRec* zt, hipStream_t s,
                 const float* radius = nullptr, const BinGeom* geo = nullptr, int32_t* counts = nullptr) {
   if (n == 0 && counts == nullptr) return GSTV_OK;
   BinGeom g0{};
   hipLaunchKernelGGL(k_prep, dim3(ceil_div_i(counts != nullptr ? n + 1 : n, 256)), dim3(256), 0, s, center_vox,
-                     cov_inv, rho_r, rho_i, rho_stride, half_shape, n, rec, radius, geo != nullptr ? *geo : g0,
+                     cov_inv, rho_r, rho_i, rho_stride, half_shape, n, rec, zt, radius, geo != nullptr ? *geo : g0,
                      counts);
   GSTV_LAUNCH_CHECK("prep");
   return GSTV_OK;
@@ -2447,12 +2685,12 @@ int tile_grid(const VoxGeom& g) { return ((g.blocks + 7) / 8) * 8; }
 namespace {
 // Forward tile kernel on built records.  pair_columns: written when non-null
 // and the tile is at most 8 voxels per axis.
-void launch_forward(const GRec* rec, const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
-                    int32_t tile, int32_t d, int32_t h, int32_t w, const int nt[3], float* vol_r, float* vol_i,
-                    uint32_t* pair_columns, hipStream_t s) {
+void launch_forward(const GRec* rec, const ZRec* zt, const int32_t* tile_ranges, int32_t num_tiles,
+                    const int32_t* pair_ids, int32_t tile, int32_t d, int32_t h, int32_t w, const int nt[3],
+                    float* vol_r, float* vol_i, uint32_t* pair_columns, hipStream_t s) {
   const VoxGeom geo = make_vox_geom(d, h, w, tile, nt, num_tiles);
   const int slot = prof_start(0, s);
-  hipLaunchKernelGGL(k_forward, dim3(tile_grid(geo)), dim3(kTileThreads), 0, s, rec, tile_ranges, pair_ids, geo,
+  hipLaunchKernelGGL(k_forward, dim3(tile_grid(geo)), dim3(kTileThreads), 0, s, rec, zt, tile_ranges, pair_ids, geo,
                      vol_r, vol_i, geo.nsb <= 2 ? pair_columns : nullptr);
   prof_stop(0, slot, s);
 }
@@ -2466,7 +2704,7 @@ void launch_clear_acc(float* gacc, int32_t n, hipStream_t s) {
 // grad_center is given) the moments -> gradients pass.  pair_columns: the
 // forward's column words (tiles of at most 8 voxels per axis), or null to
 // compute the column bounds here.
-void launch_backward(const GRec* rec, float* gacc, const float* cov_inv, const float* half_shape,
+void launch_backward(const GRec* rec, const ZRec* zt, float* gacc, const float* cov_inv, const float* half_shape,
                      const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids, int32_t n,
                      const float* grad_r, const float* grad_i, int32_t tile, int32_t d, int32_t h, int32_t w,
                      const int nt[3], float* grad_center, float* grad_cov_inv, float* grad_rho_r, float* grad_rho_i,
@@ -2480,31 +2718,35 @@ void launch_backward(const GRec* rec, float* gacc, const float* cov_inv, const f
   const size_t lds_grad = (size_t)tp * grad_zs(tp) * sizeof(float2);
   const uint32_t* cols = geo.nsb <= 2 ? pair_columns : nullptr;
   if (tile == 8)
-    hipLaunchKernelGGL(k_backward<8>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, rec, tile_ranges,
-                       pair_ids, geo_b, grad_r, grad_i, gacc, cols);
+    hipLaunchKernelGGL(k_backward<8>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, rec, zt,
+                       tile_ranges, pair_ids, geo_b, grad_r, grad_i, gacc, cols);
   else
-    hipLaunchKernelGGL(k_backward<0>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, rec, tile_ranges,
-                       pair_ids, geo_b, grad_r, grad_i, gacc, cols);
+    hipLaunchKernelGGL(k_backward<0>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, rec, zt,
+                       tile_ranges, pair_ids, geo_b, grad_r, grad_i, gacc, cols);
   prof_stop(1, slot, s);
   if (grad_center != nullptr)
     hipLaunchKernelGGL(k_finalize, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, (const float*)gacc, cov_inv,
                        half_shape, n, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, rho_stride);
 }
 
-// Keep workspace: the records, then the 48-byte moment rows.
+// Keep workspace: the records, the column-bound records, then the 48-byte
+// moment rows.
 struct KeepLayout {
   GRec* rec;
+  ZRec* zt;
   float* gacc;
   int64_t* sums;  // block sums of the pair-count scan
 };
 size_t keep_layout(void* base, int32_t n, KeepLayout* out) {
   Carve cv(base);
   GRec* rec = cv.take<GRec>((size_t)n);
+  ZRec* zt = cv.take<ZRec>((size_t)n);
   float* gacc = cv.take<float>((size_t)n * kAccStride);
   int64_t* sums = cv.take<int64_t>((size_t)ceil_div_l((int64_t)n + 1, kScanTile) + 1);
-  if (out != nullptr) *out = KeepLayout{rec, gacc, sums};
+  if (out != nullptr) *out = KeepLayout{rec, zt, gacc, sums};
   return align_up(cv.off, 256);
 }
+
 }  // namespace
 
 int gstv_forward(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
@@ -2520,10 +2762,12 @@ int gstv_forward(const float* center_vox, const float* cov_inv, const float* rho
   if (n > 0 && (workspace == nullptr || workspace_bytes < gstv_forward_workspace_bytes(n)))
     return GSTV_ERR_WORKSPACE_TOO_SMALL;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  GRec* rec = static_cast<GRec*>(workspace);
-  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, 1, half_shape, n, rec, s);
+  GRec* rec;
+  ZRec* zt;
+  fwd_layout(workspace, n, &rec, &zt);
+  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, 1, half_shape, n, rec, zt, s);
   if (st != GSTV_OK) return st;
-  launch_forward(rec, tile_ranges, num_tiles, pair_ids, tile, d, h, w, nt, vol_r, vol_i, nullptr, s);
+  launch_forward(rec, zt, tile_ranges, num_tiles, pair_ids, tile, d, h, w, nt, vol_r, vol_i, nullptr, s);
   GSTV_LAUNCH_CHECK("forward");
   return GSTV_OK;
 }
@@ -2551,10 +2795,10 @@ int gstv_backward(const float* center_vox, const float* cov_inv, const float* rh
   hipStream_t s = static_cast<hipStream_t>(stream);
   KeepLayout ws;
   keep_layout(workspace, n, &ws);
-  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, 1, half_shape, n, ws.rec, s);
+  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, 1, half_shape, n, ws.rec, ws.zt, s);
   if (st != GSTV_OK) return st;
   launch_clear_acc(ws.gacc, n, s);
-  launch_backward(ws.rec, ws.gacc, cov_inv, half_shape, tile_ranges, num_tiles, pair_ids, n, grad_r, grad_i, tile, d,
+  launch_backward(ws.rec, ws.zt, ws.gacc, cov_inv, half_shape, tile_ranges, num_tiles, pair_ids, n, grad_r, grad_i, tile, d,
                   h, w, nt, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, 1, nullptr, s);
   GSTV_LAUNCH_CHECK("backward");
   return GSTV_OK;
@@ -2591,8 +2835,8 @@ int gstv_keep_prepare(const float* center_vox, const float* radius_vox, const fl
   hipStream_t s = static_cast<hipStream_t>(stream);
   KeepLayout ws;
   keep_layout(keep_workspace, n, &ws);
-  int st = launch_prep(center_vox, cov_inv, rho_r, rho_i, rho_stride, half_shape, n, ws.rec, s, radius_vox, &geo,
-                       count ? gauss_offsets : nullptr);
+  int st = launch_prep(center_vox, cov_inv, rho_r, rho_i, rho_stride, half_shape, n, ws.rec, ws.zt, s, radius_vox,
+                       &geo, count ? gauss_offsets : nullptr);
   if (st != GSTV_OK || !count) return st;
   return launch_exclusive_scan(gauss_offsets, (int64_t)n + 1, gauss_offsets, ws.sums, total_pairs, s);
 }
@@ -2623,7 +2867,7 @@ int gstv_keep_prologue(const float* positions, const float* scales, const float*
   keep_layout(keep_workspace, n, &ws);
   hipLaunchKernelGGL(k_prologue_prep, dim3(ceil_div_i(n + 1, 256)), dim3(256), 0, s, positions, scales, rotations, n,
                      (float)d, (float)h, (float)w, center_vox, radius_vox, cov_inv, half_shape, rho_r, rho_i,
-                     (int)rho_stride, ws.rec, geo, gauss_offsets);
+                     (int)rho_stride, ws.rec, ws.zt, geo, gauss_offsets);
   GSTV_LAUNCH_CHECK("keep_prologue");
   return launch_exclusive_scan(gauss_offsets, (int64_t)n + 1, gauss_offsets, ws.sums, total_pairs, s);
 }
@@ -2647,7 +2891,7 @@ int gstv_keep_bin_fill(const void* keep_workspace, size_t keep_workspace_bytes, 
   if (n < 0) return GSTV_ERR_INVALID_ARGUMENT;
   if (n > 0 && (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n)))
     return GSTV_ERR_WORKSPACE_TOO_SMALL;
-  KeepLayout ws{nullptr, nullptr};
+  KeepLayout ws{nullptr, nullptr, nullptr, nullptr};
   if (n > 0) keep_layout(const_cast<void*>(keep_workspace), n, &ws);
   return bin_fill_impl(ws.rec, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, total_pairs, tile_ranges,
                        pair_ids, workspace, workspace_bytes, stream);
@@ -2666,7 +2910,7 @@ int gstv_forward_keep(const void* keep_workspace, size_t keep_workspace_bytes, c
   hipStream_t s = static_cast<hipStream_t>(stream);
   KeepLayout ws;
   keep_layout(const_cast<void*>(keep_workspace), n, &ws);
-  launch_forward(ws.rec, tile_ranges, num_tiles, pair_ids, tile, d, h, w, nt, vol_r, vol_i, pair_columns, s);
+  launch_forward(ws.rec, ws.zt, tile_ranges, num_tiles, pair_ids, tile, d, h, w, nt, vol_r, vol_i, pair_columns, s);
   GSTV_LAUNCH_CHECK("forward_keep");
   return GSTV_OK;
 }
@@ -2691,8 +2935,8 @@ int gstv_backward_keep(void* keep_workspace, size_t keep_workspace_bytes, const 
   hipStream_t s = static_cast<hipStream_t>(stream);
   KeepLayout ws;
   keep_layout(keep_workspace, n, &ws);
-  launch_backward(ws.rec, ws.gacc, cov_inv, half_shape, tile_ranges, num_tiles, pair_ids, n, grad_r, grad_i, tile, d,
-                  h, w, nt, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, rho_stride, pair_columns, s);
+  launch_backward(ws.rec, ws.zt, ws.gacc, cov_inv, half_shape, tile_ranges, num_tiles, pair_ids, n, grad_r, grad_i,
+                  tile, d, h, w, nt, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, rho_stride, pair_columns, s);
   GSTV_LAUNCH_CHECK("backward_keep");
   return GSTV_OK;
 }
@@ -2721,8 +2965,8 @@ int gstv_backward_keep_params(void* keep_workspace, size_t keep_workspace_bytes,
   hipStream_t s = static_cast<hipStream_t>(stream);
   KeepLayout ws;
   keep_layout(keep_workspace, n, &ws);
-  launch_backward(ws.rec, ws.gacc, cov_inv, half_shape, tile_ranges, num_tiles, pair_ids, n, grad_r, grad_i, tile, d,
-                  h, w, nt, nullptr, nullptr, nullptr, nullptr, rho_stride, pair_columns, s);
+  launch_backward(ws.rec, ws.zt, ws.gacc, cov_inv, half_shape, tile_ranges, num_tiles, pair_ids, n, grad_r, grad_i,
+                  tile, d, h, w, nt, nullptr, nullptr, nullptr, nullptr, rho_stride, pair_columns, s);
   hipLaunchKernelGGL(k_finalize_epilogue, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, (const float*)ws.gacc,
                      cov_inv, half_shape, scales, rotations, n, (float)d, (float)h, (float)w, grad_positions,
                      grad_scales, grad_rotations, grad_rho_r, grad_rho_i, (int)rho_stride);
@@ -2743,14 +2987,16 @@ int gstv_count_evaluations(const float* center_vox, const float* cov_inv, const 
   if (n > 0 && (workspace == nullptr || workspace_bytes < gstv_forward_workspace_bytes(n)))
     return GSTV_ERR_WORKSPACE_TOO_SMALL;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, reinterpret_cast<int32_t*>(counts), (int64_t)6, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, reinterpret_cast<int32_t*>(counts), (int64_t)8, 0);
   GSTV_LAUNCH_CHECK("count_clear");
-  GRec* rec = static_cast<GRec*>(workspace);
-  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, 1, half_shape, n, rec, s);
+  GRec* rec;
+  ZRec* zt;
+  fwd_layout(workspace, n, &rec, &zt);
+  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, 1, half_shape, n, rec, zt, s);
   if (st != GSTV_OK) return st;
   const VoxGeom geo = make_vox_geom(d, h, w, tile, nt, num_tiles);
-  hipLaunchKernelGGL(k_count, dim3(tile_grid(geo)), dim3(kTileThreads), 0, s, (const GRec*)rec, tile_ranges, pair_ids,
-                     geo, reinterpret_cast<unsigned long long*>(counts));
+  hipLaunchKernelGGL(k_count, dim3(tile_grid(geo)), dim3(kTileThreads), 0, s, (const GRec*)rec, (const ZRec*)zt,
+                     tile_ranges, pair_ids, geo, reinterpret_cast<unsigned long long*>(counts));
   GSTV_LAUNCH_CHECK("count");
   return GSTV_OK;
 }

@@ -277,7 +277,10 @@ int gstv_epilogue(const float* scales, const float* rotations,
  *   counts[2] = E_culled  = (Gaussian, voxel) evaluations the forward kernel
  *               performs after culling 4x4x4 sub-bricks by the Gaussian's
  *               extent box and its column bound (padding slots excluded)
- * counts is an int64 [3] device array (overwritten).  Arguments otherwise as
+ *   counts[3] = E_adjoint = (Gaussian, voxel) evaluations the backward kernel
+ *               performs: over each pair's live 4x4 (H, W) columns, the D rows
+ *               their column bound leaves (padding rows and slots excluded)
+ * counts is an int64 [4] device array (overwritten).  Arguments otherwise as
  * for gstv_forward (same workspace size). */
 int gstv_count_evaluations(const float* center_vox, const float* cov_inv,
                            const float* rho_r, const float* rho_i,

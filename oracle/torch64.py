@@ -7,8 +7,10 @@ FP64 gradients with respect to positions, log-scales, quaternions and complex
 amplitudes.  Tile lists come from the caller (the oracle's fp32 binning), so
 the set of evaluated pairs is the reference's.
 
-Pinned by tests/test_oracle_backward.py against the NumPy oracle's forward and
-analytic adjoint (which are in turn pinned by central finite differences).
+Pinned by tests/test_oracle.py against the NumPy oracle's forward and
+analytic adjoint (which are in turn pinned by central finite differences),
+and by FP64 central finite differences of its own gradients with respect to
+positions, log-scales, quaternions and amplitudes.
 """
 
 from __future__ import annotations
@@ -28,8 +30,12 @@ def quat_to_rot64(q: torch.Tensor) -> torch.Tensor:
     return torch.stack([r0, r1, r2], dim=-2)
 
 
-def voxelize64(positions, log_scales, rotations, rho_r, rho_i, shape, tile, tile_ranges, pair_ids):
-    """Returns (vol_r, vol_i) float64 flat [D*H*W]; all inputs float64 tensors."""
+def voxelize64(positions, log_scales, rotations, rho_r, rho_i, shape, tile, tile_ranges, pair_ids,
+               return_active=False):
+    """Returns (vol_r, vol_i) float64 flat [D*H*W]; all inputs float64 tensors.
+    With ``return_active``, also the list of per-tile [voxels, pairs] masks of
+    the evaluations with q <= 9 (for finite-difference checks that must not
+    cross the cutoff)."""
     d, h, w = shape
     shape_t = torch.tensor([d, h, w], dtype=torch.float64)
     hs = shape_t * 0.5
@@ -39,7 +45,7 @@ def voxelize64(positions, log_scales, rotations, rho_r, rho_i, shape, tile, tile
     t = rot * (1.0 / (scales + 1e-8)).unsqueeze(1)
     cov = t @ t.transpose(1, 2)
     nth, ntw = math.ceil(h / tile), math.ceil(w / tile)
-    flats, vals_r, vals_i = [], [], []
+    flats, vals_r, vals_i, acts = [], [], [], []
     ranges = np.asarray(tile_ranges)
     pids = np.asarray(pair_ids)
     for tid in range(ranges.shape[0]):
@@ -57,6 +63,7 @@ def voxelize64(positions, log_scales, rotations, rho_r, rho_i, shape, tile, tile
         dvec = (vox.unsqueeze(1) - center[gids].unsqueeze(0)) / hs
         q = torch.einsum("vgi,gij,vgj->vg", dvec, cov[gids], dvec)
         wgt = torch.exp(-0.5 * q) * (q <= 9.0).to(torch.float64)
+        acts.append((q <= 9.0).detach())
         vals_r.append(wgt @ rho_r[gids])
         vals_i.append(wgt @ rho_i[gids])
         flats.append((vox[:, 0] * h + vox[:, 1]) * w + vox[:, 2])
@@ -66,4 +73,6 @@ def voxelize64(positions, log_scales, rotations, rho_r, rho_i, shape, tile, tile
         idx = torch.cat(flats).long()
         vol_r = vol_r.index_add(0, idx, torch.cat(vals_r))
         vol_i = vol_i.index_add(0, idx, torch.cat(vals_i))
+    if return_active:
+        return vol_r, vol_i, acts
     return vol_r, vol_i

@@ -17,6 +17,9 @@ A NumPy restatement of the reference's tile voxelizer (3DGSMR,
 * ``forward``       gaussian/cuda_ext/tile_voxelize_cuda_kernel.cu:118-141 in FP64.
 * ``backward``      gaussian/cuda_ext/tile_voxelize_cuda_kernel.cu:220-252 in FP64,
                     i.e. the reference's own adjoint formulas.
+  Both add only the evaluations with q <= 9 (the kernel's branch, :130 and
+  :233), so a non-finite density or upstream gradient reaches exactly the
+  quantities the reference's kernel lets it reach.
 
 Pinning (each function is checked by tests/test_oracle_*.py against something
 other than itself):
@@ -220,10 +223,14 @@ def forward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape
         if terms is None:
             continue
         gids, flat, _, _, q = terms
+        act = q <= 9.0
         with np.errstate(over="ignore", invalid="ignore"):
-            w = np.where(q <= 9.0, np.exp(-0.5 * q), 0.0)
-        vol_r[flat] = w @ rr[gids]
-        vol_i[flat] = w @ ri[gids]
+            w = np.where(act, np.exp(-0.5 * q), 0.0)
+            # Only evaluations with q <= 9 add anything (the reference's
+            # `if (mahal <= 9.0f)`, .cu:130): a non-finite density reaches only
+            # those voxels, where a weighted sum would give 0 * inf = NaN.
+            vol_r[flat] = np.where(act, w * rr[gids][None, :], 0.0).sum(axis=1)
+            vol_i[flat] = np.where(act, w * ri[gids][None, :], 0.0).sum(axis=1)
         if band > 0:
             amb[flat] |= (np.abs(q - 9.0) < 9.0 * band).any(axis=1)
     return vol_r, vol_i, amb
@@ -254,16 +261,19 @@ def backward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shap
         if terms is None:
             continue
         gids, flat, dvec, md, q = terms
+        act = q <= 9.0  # [V, G]; only these evaluations add anything (.cu:233)
         with np.errstate(over="ignore", invalid="ignore"):
-            w = np.where(q <= 9.0, np.exp(-0.5 * q), 0.0)  # [V, G]
-        vgr = gr[flat][:, None]
-        vgi = gi[flat][:, None]
-        s = (rr[gids][None, :] * vgr + ri[gids][None, :] * vgi) * (-0.5) * w
-        # gids are unique inside one tile, so fancy-index accumulation is exact.
-        d_rr[gids] += (w * vgr).sum(axis=0)
-        d_ri[gids] += (w * vgi).sum(axis=0)
-        d_c[gids] += np.einsum("vg,vgi->gi", s, -2.0 * md / hs)
-        d_m[gids] += np.einsum("vg,vgi,vgj->gij", s, dvec, dvec)
+            w = np.where(act, np.exp(-0.5 * q), 0.0)
+            vgr = gr[flat][:, None]
+            vgi = gi[flat][:, None]
+            s = np.where(act, (rr[gids][None, :] * vgr + ri[gids][None, :] * vgi) * (-0.5) * w, 0.0)
+            a3 = act[:, :, None]
+            # gids are unique inside one tile, so fancy-index accumulation is exact.
+            d_rr[gids] += np.where(act, w * vgr, 0.0).sum(axis=0)
+            d_ri[gids] += np.where(act, w * vgi, 0.0).sum(axis=0)
+            d_c[gids] += np.where(a3, s[:, :, None] * (-2.0 * md / hs), 0.0).sum(axis=0)
+            d_m[gids] += np.where(a3[..., None], s[:, :, None, None] * dvec[:, :, :, None] * dvec[:, :, None, :],
+                                  0.0).sum(axis=0)
         if band > 0:
             amb[gids] |= (np.abs(q - 9.0) < 9.0 * band).any(axis=0)
     return d_c, d_m, d_rr, d_ri, amb

@@ -123,3 +123,33 @@ def tile_lists(ranges, pair_ids):
 
 def num_tiles(shape, tile):
     return math.prod(math.ceil(s / tile) for s in shape)
+
+
+def case_by_name(name):
+    """The named parity case: adjoint_voxel / adjoint_literal (seeds 0 / 1),
+    an edge kind, or random<i>."""
+    if name == "adjoint_voxel":
+        return adjoint_check_case(0, "voxel")
+    if name == "adjoint_literal":
+        return adjoint_check_case(1, "literal")
+    if name.startswith("random"):
+        return random_case(int(name[6:]))
+    return edge_case(name)
+
+
+# Cases of the Function-level tests, and of the full-call (fused) tests.
+FUNCTION_CASES = ["adjoint_voxel", "adjoint_literal"] + EDGE_KINDS + [f"random{i}" for i in range(12)]
+CALL_CASES = ["adjoint_voxel"] + EDGE_KINDS + [f"random{i}" for i in range(4)]
+
+
+def function_upstream(shape):
+    """Upstream gradient of the Function-level tests (flat planes)."""
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(7)
+    return rng.normal(size=n).astype(np.float32), rng.normal(size=n).astype(np.float32)
+
+
+def call_upstream(shape, seed=4):
+    """Upstream gradient of the full-call tests ([D, H, W] planes)."""
+    rng = np.random.default_rng(seed)
+    return rng.normal(size=shape).astype(np.float32), rng.normal(size=shape).astype(np.float32)

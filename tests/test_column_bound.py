@@ -5,14 +5,16 @@ keep only the D rows inside the intersection of the D extents of the
 ellipsoid's projections on (D, H) and (D, W), each cut by the column's strip:
 
     u_0 in al t -/+ ga sqrt(rho_i^2 - t^2),   t = clamp(+/-ta, lo, hi),
-    al = cof_0i / cof_ii,  ga = sqrt(A'_jj det A') / cof_ii,  ta = al rho_i^2 / rho_0,
+    al = cof_0i / cof_ii,  ga = sqrt(A_jj det A) / cof_ii,  ta = al rho_i^2 / rho_0,
 
-with A' the fp32 scaled quadratic form of the Gaussian record, cofactors and
-determinant in fp64, rho_a = e_a - 0.01 the widened extents.  This restates
-that arithmetic in NumPy (fp32 where the kernels use fp32) and checks, on
-Gaussians from near-spherical to needles and pancakes with axis ratios of
-several hundred, that no voxel whose fp32 quadratic form passes the cutoff
-lies outside the kept rows -- and that the bound does cull.
+with A the voxel-unit quadratic form sym(M) / (hs hs^T) of the Gaussian,
+cofactors and determinant in fp64 (built once per Gaussian), rho_a = e_a - 0.01
+the widened extents.  This restates that arithmetic in NumPy, and the kernels'
+fp32 evaluation of q' = -log2(e)/2 q as squares of linear forms (factors
+rounded to fp32 from an fp64 factorisation), and checks, on Gaussians from
+near-spherical to needles and pancakes with axis ratios of several hundred,
+that no voxel whose fp32 q passes the cutoff lies outside the kept rows -- and
+that the bound does cull.
 """
 
 from __future__ import annotations
@@ -22,42 +24,58 @@ import numpy as np
 from oracle import voxelizer_oracle as vo
 
 F = np.float32
-KQ = F(-0.72134752044448170368)  # -log2(e) / 2
-THR = F(9.0 * -0.72134752044448170368)
+KQ = -0.72134752044448170368  # -log2(e) / 2
+THR = F(9.0 * KQ)
+
+
+def _fma(a, b, c):
+    """fp32 fused multiply-add (exact product and sum in fp64, one rounding)."""
+    return (np.asarray(a, np.float64) * np.asarray(b, np.float64) + np.asarray(c, np.float64)).astype(F)
+
+
+def _form(cov, hs):
+    m = cov.astype(np.float32).astype(np.float64)
+    s = 0.5 * (m + np.transpose(m, (0, 2, 1)))
+    h = hs.astype(np.float64)
+    return s / (h[None, :, None] * h[None, None, :])
 
 
 def _records(center, cov, hs):
-    m = cov.astype(np.float32)
-    h = hs.astype(np.float32)
-    a00 = KQ * (m[:, 0, 0] / (h[0] * h[0]))
-    a11 = KQ * (m[:, 1, 1] / (h[1] * h[1]))
-    a22 = KQ * (m[:, 2, 2] / (h[2] * h[2]))
-    b01 = KQ * ((m[:, 0, 1] + m[:, 1, 0]) / (h[0] * h[1]))
-    b02 = KQ * ((m[:, 0, 2] + m[:, 2, 0]) / (h[0] * h[2]))
-    b12 = KQ * ((m[:, 1, 2] + m[:, 2, 1]) / (h[1] * h[2]))
+    """The kernels' factors of q' (fp32) and the extents e (fp32)."""
+    a = _form(cov, hs) * -KQ
+    with np.errstate(all="ignore"):
+        y1h = np.sqrt(np.maximum(a[:, 1, 1], 0))
+        y1d = np.where(y1h > 0, a[:, 1, 0] / y1h, 0)
+        y1w = np.where(y1h > 0, a[:, 1, 2] / y1h, 0)
+        q00 = a[:, 0, 0] - y1d * y1d
+        q02 = a[:, 0, 2] - y1d * y1w
+        q22 = a[:, 2, 2] - y1w * y1w
+        y2d = np.sqrt(np.maximum(q00, 0))
+        y2w = np.where(y2d > 0, q02 / y2d, 0)
+        y3w = q22 - y2w * y2w
     s = 0.5 * (cov.astype(np.float64) + np.transpose(cov.astype(np.float64), (0, 2, 1)))
     si = np.linalg.inv(s)
     e = (3.0 * 1.02 * hs.astype(np.float64)[None, :] * np.sqrt(np.einsum("nii->ni", si)) + 0.01).astype(np.float32)
-    return a00, a11, a22, b01, b02, b12, e
+    return tuple(x.astype(F) for x in (y1h, y1d, y1w, y2d, y2w, y3w)), e
 
 
-def _coefs(a00, a11, a22, b01, b02, b12, e, d=np.float64):
-    s00, s11, s22 = a00.astype(d), a11.astype(d), a22.astype(d)
-    s01, s02, s12 = 0.5 * b01.astype(d), 0.5 * b02.astype(d), 0.5 * b12.astype(d)
-    cof00 = s11 * s22 - s12 * s12
-    cof11 = s00 * s22 - s02 * s02
-    cof22 = s00 * s11 - s01 * s01
-    cof01 = s02 * s12 - s01 * s22
-    cof02 = s01 * s12 - s02 * s11
-    det = s00 * cof00 + s01 * cof01 + s02 * cof02
-    c11, c22, c01, c02, dt = (x.astype(np.float32) for x in (cof11, cof22, cof01, cof02, det))
-    rho = e - F(0.01)
+def _coefs(cov, hs, e, d=np.float64):
+    """Column-bound coefficients, cofactors formed in precision d."""
+    a = _form(cov, hs).astype(d)
+    a00, a11, a22, a01, a02, a12 = a[:, 0, 0], a[:, 1, 1], a[:, 2, 2], a[:, 0, 1], a[:, 0, 2], a[:, 1, 2]
+    cof00 = a11 * a22 - a12 * a12
+    cof11 = a00 * a22 - a02 * a02
+    cof22 = a00 * a11 - a01 * a01
+    cof01 = a02 * a12 - a01 * a22
+    cof02 = a01 * a12 - a02 * a11
+    det = a00 * cof00 + a01 * cof01 + a02 * cof02
+    rho = (e - F(0.01)).astype(np.float64)
     with np.errstate(all="ignore"):
-        h = (c01 / c11, np.sqrt(a22 * dt) / c11)
-        w = (c02 / c22, np.sqrt(a11 * dt) / c22)
-        h = (h[0], h[1], h[0] * (rho[:, 1] * rho[:, 1] / rho[:, 0]), rho[:, 1])
-        w = (w[0], w[1], w[0] * (rho[:, 2] * rho[:, 2] / rho[:, 0]), rho[:, 2])
-    ok = (cof11 > 0) & (cof22 > 0) & (det < 0) & np.isfinite(rho).all(1)
+        h = (cof01 / cof11, np.sqrt(a22 * det) / cof11)
+        w = (cof02 / cof22, np.sqrt(a11 * det) / cof22)
+        h = tuple(x.astype(F) for x in (h[0], h[1], h[0] * (rho[:, 1] * rho[:, 1] / rho[:, 0]), rho[:, 1]))
+        w = tuple(x.astype(F) for x in (w[0], w[1], w[0] * (rho[:, 2] * rho[:, 2] / rho[:, 0]), rho[:, 2]))
+    ok = (cof11 > 0) & (cof22 > 0) & (det > 0) & np.isfinite(rho).all(1)
     for arr in h + w:
         ok &= np.isfinite(arr)
     return h, w, ok
@@ -75,8 +93,8 @@ def _check(pos, scales, rot, shape, cof_dtype=np.float64):
     """(misses, kept rows, box rows) of the column bound over every column of
     every Gaussian's box; a miss is a live voxel outside the kept rows."""
     center, _, cov, hs64 = vo.prologue(pos, scales, rot, shape)
-    a00, a11, a22, b01, b02, b12, e = _records(center, cov, hs64)
-    hcoef, wcoef, ok = _coefs(a00, a11, a22, b01, b02, b12, e, cof_dtype)
+    (y1h, y1d, y1w, y2d, y2w, y3w), e = _records(center, cov, hs64)
+    hcoef, wcoef, ok = _coefs(cov, hs64, e, cof_dtype)
     misses = kept = total = 0
     zi, yi, xi = (np.arange(v) for v in shape)
     z, y, x = np.meshgrid(zi, yi, xi, indexing="ij")
@@ -87,10 +105,12 @@ def _check(pos, scales, rot, shape, cof_dtype=np.float64):
         if np.any(lo > hi):
             continue
         u0, u1, u2 = z.astype(F) - c[0], y.astype(F) - c[1], x.astype(F) - c[2]
-        # The forward kernel's Horner order: (a00 u0 + b02 u2) u0 + a22 u2^2, then H.
-        a_part = (a00[g] * u0 + b02[g] * u2) * u0 + a22[g] * u2 * u2
-        b_part = b01[g] * u0 + b12[g] * u2
-        live = ((a11[g] * u1 + b_part) * u1 + a_part) >= THR
+        # The kernels' order: per D row t1 = y1d u0 + y1w u2, y2, r; per voxel y1, q'.
+        t1 = _fma(y1d[g], u0, (y1w[g] * u2).astype(F))
+        y2 = _fma(y2d[g], u0, (y2w[g] * u2).astype(F))
+        r = _fma(-y2, y2, -((y3w[g] * u2).astype(F) * u2).astype(F))
+        yy1 = _fma(y1h[g], u1, t1)
+        live = _fma(-yy1, yy1, r) >= THR
         for y0 in range(int(lo[1]), int(hi[1]) + 1, 4):
             y1 = min(y0 + 3, int(hi[1]))
             for x0 in range(int(lo[2]), int(hi[2]) + 1, 4):
@@ -128,12 +148,13 @@ def test_column_bound_keeps_every_active_voxel():
 
 
 def test_needle_needs_fp64_cofactors():
-    """A needle (voxel sigma 0.067 x 16 x 4.4, rotated) whose form's determinant
-    cancels by ~1e4: with fp32 cofactors the bound drops live voxels, with fp64
-    cofactors it keeps them all."""
+    """A needle (voxel sigma 12.5 x 6.8 x 0.056, rotated) whose form's
+    determinant cancels by ~1e4: with fp32 cofactors the bound drops live
+    voxels, with fp64 cofactors (as the kernels' per-Gaussian table forms them)
+    it keeps them all."""
     shape = (40, 40, 40)
-    pos = np.array([[0.45545065, -0.53229886, -0.22736628]], np.float32)
-    scales = np.array([[0.003345329547300935, 0.7984070181846619, 0.22116735577583313]], np.float32)
-    rot = np.array([[-0.14371681, 0.44219768, 0.7676122, 0.35766584]], np.float32)
+    pos = np.array([[-0.4479787051677704, -0.09544815868139267, -0.3014869689941406]], np.float32)
+    scales = np.array([[0.6251696944236755, 0.3388972878456116, 0.00277737551368773]], np.float32)
+    rot = np.array([[1.7393678426742554, -0.49591073393821716, 0.3289696276187897, -0.2585725486278534]], np.float32)
     assert _check(pos, scales, rot, shape)[0] == 0
     assert _check(pos, scales, rot, shape, np.float32)[0] > 0

@@ -6,9 +6,9 @@ import numpy as np
 import pytest
 import torch
 
-from oracle import torch64
 from oracle import voxelizer_oracle as vo
 from tests import cases
+from tests import reference_fp32 as rf
 
 pytestmark = pytest.mark.gpu
 
@@ -22,12 +22,15 @@ def _announce(msg):
 @pytest.mark.parametrize("reading", ["voxel", "literal"])
 def test_tile_voxelizer_param_gradients_vs_fp64(reading):
     """Gradients w.r.t. positions, log-scales, quaternions and complex amplitude
-    through the full call, against FP64 torch autograd of the same call."""
+    through the full call, against FP64 torch autograd of the same call, within
+    1e-5 relative L2 (or 1.5x the reference's own fp32 error on the case, where
+    that exceeds 1e-5; tests/golden/reference_fp32_errors.json)."""
     from gs_tmi_amd import TileVoxelizer
 
     c = cases.adjoint_check_case(0 if reading == "voxel" else 1, reading)
     shape, tile = c["shape"], c["tile"]
     _announce(f"end-to-end autograd, adjoint-check case ({reading})")
+    table = rf.load_table()
     vox = TileVoxelizer(shape, tile_size=tile, device=DEV)
     pos = torch.from_numpy(c["pos"]).to(DEV).requires_grad_()
     log_s = torch.from_numpy(np.log(c["scales"])).to(DEV).requires_grad_()
@@ -35,45 +38,24 @@ def test_tile_voxelizer_param_gradients_vs_fp64(reading):
     dr = torch.from_numpy(c["rr"]).to(DEV).requires_grad_()
     di = torch.from_numpy(c["ri"]).to(DEV).requires_grad_()
     vol_r, vol_i = vox(pos, torch.exp(log_s), rot, torch.complex(dr, di))
-    rng = np.random.default_rng(3)
-    g_r = rng.normal(size=shape).astype(np.float32)
-    g_i = rng.normal(size=shape).astype(np.float32)
+    g_r, g_i = cases.call_upstream(shape, seed=3)
     loss = (vol_r * torch.from_numpy(g_r).to(DEV)).sum() + (vol_i * torch.from_numpy(g_i).to(DEV)).sum()
     loss.backward()
     torch.cuda.synchronize()
 
     got = dict(pos=pos.grad, log_s=log_s.grad, rot=rot.grad, dr=dr.grad, di=di.grad)
-    ref, ok, vol64 = _fp64_reference(c, torch.exp(log_s).detach().cpu().numpy(), g_r, g_i)
+    ref, ok, vol64 = rf.fp64_call_reference(c, torch.exp(log_s).detach().cpu().numpy(), g_r, g_i)
     for k in got:
         e = cases.rel_l2(got[k].cpu().numpy()[ok], ref[k][ok])
-        print(f"  d/d{k}: rel L2 {e:.3e}")
-        assert e <= 1e-4, (k, e)
+        tol = rf.tolerance(table, "param", reading, k)
+        print(f"  d/d{k}: rel L2 {e:.3e} (reference fp32 {table['param'][reading][k]:.2e}, bound {tol:.2e})")
+        assert e <= tol, (k, e, tol)
     # Volume parity through the full call.
     e_vol = cases.rel_l2(np.concatenate([vol_r.detach().cpu().numpy().ravel(), vol_i.detach().cpu().numpy().ravel()]),
                          vol64)
-    print(f"  volume rel L2 {e_vol:.3e}")
-    assert e_vol <= 1e-5
-
-
-def _fp64_reference(c, s_gpu, g_r, g_i):
-    """FP64 torch autograd of the whole call on case ``c`` with upstream
-    gradient (g_r, g_i), binned as the reference bins the fp32 prologue of the
-    exact fp32 scales the GPU saw (``s_gpu``, exp on the GPU).  Returns the
-    parameter gradients (pos, log_s, rot, dr, di), the mask of Gaussians with no
-    voxel within 1e-4 of the q = 9 cutoff (elsewhere fp32 and fp64 may disagree
-    on a voxel's inclusion), and the volume (real then imaginary, flattened)."""
-    shape, tile = c["shape"], c["tile"]
-    c32, r32, cov64, hs = vo.prologue(c["pos"], s_gpu, c["rot"], shape)
-    ranges, pids = vo.bin_tiles(c32, r32, shape, tile)
-    *_, amb = vo.backward(c32, cov64, c["rr"], c["ri"], ranges, pids, hs, g_r, g_i, shape, tile, band=1e-4)
-    t = {k: torch.from_numpy(np.asarray(v, dtype=np.float64)).requires_grad_()
-         for k, v in dict(pos=c["pos"], log_s=np.log(c["scales"]), rot=c["rot"], dr=c["rr"], di=c["ri"]).items()}
-    er, ei = torch64.voxelize64(t["pos"], t["log_s"], t["rot"], t["dr"], t["di"], shape, tile, ranges, pids)
-    loss64 = (er * torch.from_numpy(g_r.ravel().astype(np.float64))).sum() + \
-             (ei * torch.from_numpy(g_i.ravel().astype(np.float64))).sum()
-    loss64.backward()
-    ref = {k: (v.grad.numpy() if v.grad is not None else np.zeros(v.shape)) for k, v in t.items()}
-    return ref, ~amb, np.concatenate([er.detach().numpy(), ei.detach().numpy()])
+    tol = rf.tolerance(table, "param", reading, "vol")
+    print(f"  volume rel L2 {e_vol:.3e} (bound {tol:.2e})")
+    assert e_vol <= tol
 
 
 def _microbench_inputs(n, shape, seed=0):
@@ -157,32 +139,54 @@ def test_full_size_properties():
         assert e < 1e-6, (name, e)
 
 
-# The two prologues give cov_inv within fp32 rounding of each other (not bit for
-# bit), which the quadratic form's conditioning amplifies for needle-shaped and
-# strongly anisotropic Gaussians (axis ratios up to ~600).  Tolerances
-# (volume, parameter gradients) between the two calls, then against FP64.
-FUSED_TOL = {"needles": (5e-6, 2e-5), "extreme_aniso": (5e-5, 5e-4)}
-FP64_TOL = {"extreme_aniso": (5e-5, 1e-3)}
+# The two prologues (HIP, and torch on the GPU) give cov_inv within fp32
+# rounding of each other (not bit for bit), which the quadratic form's
+# conditioning amplifies for needle-shaped and strongly anisotropic Gaussians
+# (axis ratios up to ~600): the exact function itself moves by up to ~1e-4 in
+# the volume and ~1e-3 in the gradients between the two fp32 inputs.  The two
+# calls must agree within 2e-6 (volume) and 2e-5 (gradients); where the exact
+# volumes on the two inputs differ by more than that, the volumes within that
+# exact difference (plus 1e-6) and each within 1e-5 of the exact function of
+# its own inputs (the gradients' mutual difference is then not a defined
+# target: the Function-level tests hold the composition's gradients to FP64 on
+# its own inputs, and the fused call is held to FP64 of the parameters below).
 
 
-@pytest.mark.parametrize("name", ["adjoint_voxel"] + cases.EDGE_KINDS + [f"random{i}" for i in range(4)])
+def _intrinsic_volume_difference(c, s_gpu):
+    """Relative L2 difference (real, imaginary) of the FP64 volumes evaluated
+    exactly on the two prologues' fp32 cov_inv (HIP and torch on the GPU), same
+    centres and lists; and those two volumes (real then imaginary, flat)."""
+    from gs_tmi_amd import _native, voxel_space_params
+
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)  # noqa: E731
+    shape, tile = c["shape"], c["tile"]
+    cen, rad, cov_t, hs = voxel_space_params(t(c["pos"]), t(s_gpu), t(c["rot"]), shape)
+    _, _, cov_h, _ = _native.prologue(t(c["pos"]), t(s_gpu), t(c["rot"]), shape)
+    c32, h32 = cen.cpu().numpy(), hs.cpu().numpy()
+    ranges, pids = vo.bin_tiles(c32, rad.cpu().numpy(), shape, tile)
+    vols = []
+    for cv in (cov_h, cov_t):
+        er, ei, _ = vo.forward(c32, cv.cpu().numpy(), c["rr"], c["ri"], ranges, pids, h32, shape, tile)
+        vols.append(np.concatenate([er, ei]))
+    n = vols[0].size // 2
+    return [cases.rel_l2(vols[0][k * n:(k + 1) * n], vols[1][k * n:(k + 1) * n]) for k in (0, 1)], vols
+
+
+@pytest.mark.parametrize("name", cases.CALL_CASES)
 def test_fused_call_matches_unfused_composition(name):
     """TileVoxelizer.forward (fused HIP prologue and epilogue) against the
-    reference's composition: torch prologue + binning + 11-argument Function."""
+    reference's composition (torch prologue + binning + 11-argument Function),
+    and against FP64 autograd of the call: volume and every parameter gradient
+    within 1e-5 relative L2, or 1.5x the reference's own fp32 error on the case
+    where that exceeds 1e-5 (tests/golden/reference_fp32_errors.json)."""
     from gs_tmi_amd import TileVoxelizer
 
-    if name == "adjoint_voxel":
-        c = cases.adjoint_check_case(0, "voxel")
-    elif name.startswith("random"):
-        c = cases.random_case(int(name[6:]))
-    else:
-        c = cases.edge_case(name)
+    c = cases.case_by_name(name)
     shape, tile = c["shape"], c["tile"]
     _announce(f"fused vs unfused {name} shape={shape} tile={tile}")
+    table = rf.load_table()
     vox = TileVoxelizer(shape, tile_size=tile, device=DEV)
-    rng = np.random.default_rng(4)
-    g_r_np = rng.normal(size=shape).astype(np.float32)
-    g_i_np = rng.normal(size=shape).astype(np.float32)
+    g_r_np, g_i_np = cases.call_upstream(shape)
     g_r, g_i = torch.from_numpy(g_r_np).to(DEV), torch.from_numpy(g_i_np).to(DEV)
     out = {}
     for mode in ("fused", "unfused"):
@@ -196,23 +200,43 @@ def test_fused_call_matches_unfused_composition(name):
         if mode == "fused":
             s_gpu = torch.exp(log_s).detach().cpu().numpy()
     names = ["vol_r", "vol_i", "pos", "log_s", "rot", "dr", "di"]
-    vtol, gtol = FUSED_TOL.get(name, (2e-6, 2e-5))
-    for nm, a, b in zip(names, out["fused"], out["unfused"]):
+    intrinsic = None
+    for k, (nm, a, b) in enumerate(zip(names[:2], out["fused"], out["unfused"])):
         e = cases.rel_l2(a.cpu().numpy(), b.cpu().numpy())
-        print(f"  {nm}: fused vs unfused rel L2 {e:.2e}")
-        assert e < (vtol if nm.startswith("vol") else gtol), (nm, e)
-    # Both against FP64 autograd of the reference's call (the fused call's own
+        tol = 2e-6
+        if e > tol:
+            intrinsic = intrinsic or _intrinsic_volume_difference(c, s_gpu)
+            tol = intrinsic[0][k] + 1e-6
+        print(f"  {nm}: fused vs unfused rel L2 {e:.2e} (bound {tol:.2e})")
+        assert e <= tol, (nm, e, tol)
+    if intrinsic is not None:
+        # Each call's volume is the exact function of its own fp32 inputs.
+        for mode, exact in zip(("fused", "unfused"), intrinsic[1]):
+            got = np.concatenate([out[mode][0].cpu().numpy().ravel(), out[mode][1].cpu().numpy().ravel()])
+            e = cases.rel_l2(got, exact)
+            print(f"  volume: {mode} vs FP64 on its own prologue's cov_inv rel L2 {e:.2e}")
+            assert e <= 1e-5, (mode, e)
+    else:
+        for nm, a, b in zip(names[2:], out["fused"][2:], out["unfused"][2:]):
+            e = cases.rel_l2(a.cpu().numpy(), b.cpu().numpy())
+            print(f"  {nm}: fused vs unfused rel L2 {e:.2e}")
+            assert e < 2e-5, (nm, e)
+    # The fused call against FP64 autograd of the reference's call (its own
     # parity, not only its agreement with the composition).
-    ref, ok, vol64 = _fp64_reference(c, s_gpu, g_r_np, g_i_np)
-    vtol, gtol = FP64_TOL.get(name, (1e-5, 1e-4))
-    e = cases.rel_l2(np.concatenate([out["fused"][0].cpu().numpy().ravel(), out["fused"][1].cpu().numpy().ravel()]),
-                     vol64)
-    print(f"  volume: fused vs FP64 rel L2 {e:.2e}")
-    assert e <= vtol, ("volume", e)
-    for nm, a in zip(names[2:], out["fused"][2:]):
-        e = cases.rel_l2(a.cpu().numpy()[ok], ref[nm][ok])
-        print(f"  d/d{nm}: fused vs FP64 rel L2 {e:.2e}")
-        assert e <= gtol, (nm, e)
+    ref, ok, vol64 = rf.fp64_call_reference(c, s_gpu, g_r_np, g_i_np)
+    for mode in ("fused",):
+        got = out[mode]
+        e = cases.rel_l2(np.concatenate([got[0].cpu().numpy().ravel(), got[1].cpu().numpy().ravel()]), vol64)
+        tol = rf.tolerance(table, "call", name, "vol")
+        print(f"  volume: {mode} vs FP64 rel L2 {e:.2e} (reference fp32 {table['call'][name]['vol']:.2e}, "
+              f"bound {tol:.2e})")
+        assert e <= tol, (mode, "volume", e, tol)
+        for nm, a in zip(names[2:], got[2:]):
+            e = cases.rel_l2(a.cpu().numpy()[ok], ref[nm][ok])
+            tol = rf.tolerance(table, "call", name, nm)
+            print(f"  d/d{nm}: {mode} vs FP64 rel L2 {e:.2e} (reference fp32 {table['call'][name][nm]:.2e}, "
+                  f"bound {tol:.2e})")
+            assert e <= tol, (mode, nm, e, tol)
 
 
 def test_fused_call_density_layouts():

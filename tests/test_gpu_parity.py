@@ -16,19 +16,18 @@ import torch
 
 from oracle import voxelizer_oracle as vo
 from tests import cases
+from tests import reference_fp32 as rf
 
 pytestmark = pytest.mark.gpu
 
 DEV = "cuda:0"
 BAND = 2e-5
 TOL = 1e-5
-# Axis ratios up to ~600: the fp32 evaluation of the quadratic form itself is
-# off from FP64 by ~1e-5 in the volume (an fp32 NumPy restatement of the kernel
-# arithmetic on this input gives 7.4e-6) and ~1e-4 in the centre gradient, as
-# in the reference's fp32 kernel; a kernel without the column bound gives the
-# same errors on this input.  That culling keeps every active evaluation is
-# checked exactly by test_culling_keeps_every_active_evaluation.
-CASE_TOL = {"extreme_aniso": 2e-4}
+# Where the reference's own fp32 tile path misses 1e-5 against FP64 on a case
+# (axis ratios up to ~600, where the quadratic form's terms cancel), the bound
+# is 1.5x that measured error: tests/golden/reference_fp32_errors.json, made by
+# tests/golden/make_reference_fp32_errors.py.  That culling keeps every active
+# evaluation is checked exactly by test_culling_keeps_every_active_evaluation.
 
 
 def _announce(msg):
@@ -100,9 +99,8 @@ def test_forward_and_backward_vs_fp64(name, make):
     _announce(f"fwd/bwd {name} shape={c['shape']} tile={c['tile']} n={len(c['rr'])}")
     center, cov, rr, ri, ranges, pids, hs = _function_inputs(c)
     vol_r, vol_i = _native.forward(center, cov, rr, ri, ranges, pids, hs, c["tile"], d, h, w)
-    rng = np.random.default_rng(7)
-    g_r = rng.normal(size=d * h * w).astype(np.float32)
-    g_i = rng.normal(size=d * h * w).astype(np.float32)
+    g_r, g_i = cases.function_upstream(c["shape"])
+    table = rf.load_table()
     gc, gm, grr, gri = _native.backward(center, cov, rr, ri, ranges, pids, hs, torch.from_numpy(g_r).to(DEV),
                                         torch.from_numpy(g_i).to(DEV), c["tile"], d, h, w)
     torch.cuda.synchronize()
@@ -114,8 +112,8 @@ def test_forward_and_backward_vs_fp64(name, make):
     assert ok.mean() > 0.9, f"too many ambiguous voxels: {amb.sum()}"
     e_fwd = cases.rel_l2(np.concatenate([vol_r.cpu().numpy()[ok], vol_i.cpu().numpy()[ok]]),
                          np.concatenate([er[ok], ei[ok]]))
-    print(f"  forward rel L2 = {e_fwd:.3e} (ambiguous voxels {int(amb.sum())})")
-    tol = CASE_TOL.get(name, TOL)
+    tol = rf.tolerance(table, "function", name, "vol")
+    print(f"  forward rel L2 = {e_fwd:.3e} (ambiguous voxels {int(amb.sum())}; bound {tol:.2e})")
     assert e_fwd <= tol
 
     dc, dm, drr, dri, gamb = vo.backward(*args, g_r, g_i, c["shape"], c["tile"], band=BAND)
@@ -129,7 +127,8 @@ def test_forward_and_backward_vs_fp64(name, make):
     }
     print(f"  backward rel L2 = {errs} (ambiguous Gaussians {int(gamb.sum())})")
     for k, e in errs.items():
-        assert e <= tol, (k, e)
+        tol = rf.tolerance(table, "function", name, k)
+        assert e <= tol, (k, e, tol)
 
 
 @pytest.mark.parametrize("name,make", ALL_CASES, ids=[n for n, _ in ALL_CASES])
@@ -137,14 +136,17 @@ def test_culling_keeps_every_active_evaluation(name, make):
     """The forward culls (pair, sub-brick) work by the exact extents and the
     column bound; the evaluations it keeps must hold every (pair, voxel) with
     q <= 9.  Its active count equals the FP64 count up to evaluations whose q
-    lies in the cutoff ambiguity band, and it never exceeds E_total."""
+    lies in the cutoff ambiguity band, and it never exceeds E_total.  The
+    backward's rows (exact D rows of each live 4x4 column) lie inside the
+    forward's kept sub-bricks and hold every active evaluation too."""
     from gs_tmi_amd import _native
 
     c = make()
     d, h, w = c["shape"]
     _announce(f"culling {name} shape={c['shape']} tile={c['tile']} n={len(c['rr'])}")
     center, cov, rr, ri, ranges, pids, hs = _function_inputs(c)
-    e_total, e_active, e_culled = _native.count_evaluations(center, cov, rr, ri, ranges, pids, hs, c["tile"], d, h, w)
+    e_total, e_active, e_culled, e_adj = _native.count_evaluations(center, cov, rr, ri, ranges, pids, hs, c["tile"],
+                                                                    d, h, w)
     cen = center.cpu().numpy().astype(np.float64)
     cv = cov.cpu().numpy().astype(np.float64).reshape(-1, 3, 3)
     rg, pi, h64 = ranges.cpu().numpy(), pids.cpu().numpy(), hs.cpu().numpy()
@@ -157,10 +159,12 @@ def test_culling_keeps_every_active_evaluation(name, make):
         n_total += q.size
         n_active += int(np.count_nonzero(q <= 9.0))
         n_band += int(np.count_nonzero(np.abs(q - 9.0) < 9.0 * BAND))
-    print(f"  E_total {e_total} E_active {e_active} (FP64 {n_active}, band {n_band}) E_culled {e_culled}")
+    print(f"  E_total {e_total} E_active {e_active} (FP64 {n_active}, band {n_band}) E_culled {e_culled} "
+          f"E_adjoint {e_adj}")
     assert e_total == n_total
     assert abs(e_active - n_active) <= n_band
     assert e_active <= e_culled <= e_total
+    assert e_active <= e_adj <= e_culled
 
 
 def _touching_lists(ranges, pids, rec, shape, tile):
@@ -228,7 +232,7 @@ def test_keep_binning_is_the_touching_pairs(name, make):
     plain = _native.backward(center, cov, rr, ri, r_t, p_t, hs, g_r, g_i, tile, d, h, w)
     keep = _native.backward(center, cov, rr, ri, t_ranges, t_pids, hs, g_r, g_i, tile, d, h, w, kept=kept)
     torch.cuda.synchronize()
-    tol = 2e-5 if name in CASE_TOL else 5e-6
+    tol = 2e-5 if name == "extreme_aniso" else 5e-6
     for a, b in zip(plain, keep):
         assert cases.rel_l2(b.cpu().numpy(), a.cpu().numpy()) < tol
     # The adjoint straight to the parameters equals backward + epilogue.
@@ -246,7 +250,7 @@ def test_keep_binning_is_the_touching_pairs(name, make):
     # Two adjoint runs differ by the order of fp32 atomics (measured up to
     # 2.1e-5 here after the chain rule, on random5); the chain rule to linear
     # scales (1/s^2) amplifies it further for the needles of the extreme case.
-    ptol = 1e-3 if name in CASE_TOL else 1e-4
+    ptol = 1e-3 if name == "extreme_aniso" else 1e-4
     for a, b in zip(want + (plain[2], plain[3]), got + (g_dv[:, 0], g_dv[:, 1])):
         assert cases.rel_l2(b.cpu().numpy(), a.cpu().numpy()) < ptol
 
@@ -297,7 +301,7 @@ def test_keep_path_matches_plain_calls(name, make):
     # Two calls differ by the order of fp32 atomic additions (~1e-6 on small
     # cases); the needles of the extreme-anisotropy case amplify it (centre
     # gradient ~3e-6).
-    tol = 2e-5 if name in CASE_TOL else 5e-6
+    tol = 2e-5 if name == "extreme_aniso" else 5e-6
     for a, b in zip(plain, keep):
         assert cases.rel_l2(b.cpu().numpy(), a.cpu().numpy()) < tol
     # A second adjoint of the same forward (its accumulator is prepared again).
@@ -619,3 +623,89 @@ def test_gradients_vs_reference_cpu_tile_path(name, make):
     print(f"  HIP vs reference CPU path rel L2 = {errs}")
     for k, e in errs.items():
         assert e <= TOL, (k, e)
+
+
+def _pattern(x):
+    """0 finite, 1 NaN, 2 +inf, 3 -inf."""
+    x = np.asarray(x)
+    return np.where(np.isnan(x), 1, np.where(np.isposinf(x), 2, np.where(np.isneginf(x), 3, 0)))
+
+
+@pytest.mark.parametrize("path", ["plain", "keep", "fused"])
+def test_non_finite_density_and_upstream_gradient_vs_oracle(path):
+    """A NaN / inf density, or a NaN / inf upstream gradient at a voxel, reaches
+    only the evaluations with q <= 9, as in the reference kernel's branch
+    (cuda_ext/tile_voxelize_cuda_kernel.cu:130, :233): the volume's NaN / inf
+    pattern, the density gradients and the centre / cov_inv gradients'
+    finiteness equal the oracle's voxel for voxel and Gaussian for Gaussian,
+    and the finite values match it (FP64) within 1e-5 (voxels and Gaussians
+    with a pair in the cutoff ambiguity band excluded)."""
+    from gs_tmi_amd import TileVoxelizer, _native, build_tile_pairs
+
+    c = cases.edge_case("partial_tiles")
+    shape, tile = c["shape"], c["tile"]
+    d, h, w = shape
+    _announce(f"non-finite density and upstream gradient, {path}, shape={shape} tile={tile}")
+    rr_np, ri_np = c["rr"].copy(), c["ri"].copy()
+    rr_np[3], rr_np[11], ri_np[17], ri_np[23] = np.nan, np.inf, -np.inf, np.nan
+    rng = np.random.default_rng(13)
+    g_r_np = rng.normal(size=d * h * w).astype(np.float32)
+    g_i_np = rng.normal(size=d * h * w).astype(np.float32)
+    g_r_np[1000], g_i_np[20000], g_r_np[30000] = np.inf, np.nan, -np.inf
+    center, radius, cov, hs = _prologue_gpu(c)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)  # noqa: E731
+    rr, ri, g_r, g_i = t(rr_np), t(ri_np), t(g_r_np), t(g_i_np)
+    if path == "fused":
+        vox = TileVoxelizer(shape, tile_size=tile, device=DEV)
+        dens = torch.complex(rr, ri).requires_grad_()
+        vr, vi = vox(t(c["pos"]), t(c["scales"]), t(c["rot"]), dens)
+        torch.autograd.backward((vr, vi), (g_r.view(shape), g_i.view(shape)))
+        vol = (vr.detach().reshape(-1), vi.detach().reshape(-1))
+        grr, gri = dens.grad.real, dens.grad.imag
+        gc = gm = None
+    else:
+        ranges, pids = build_tile_pairs(center, radius, shape, tile)
+        if path == "plain":
+            vol = _native.forward(center, cov, rr, ri, ranges, pids, hs, tile, d, h, w)
+            gc, gm, grr, gri = _native.backward(center, cov, rr, ri, ranges, pids, hs, g_r, g_i, tile, d, h, w)
+        else:
+            kept = _native.keep_prepare(center, cov, rr, ri, hs, radius, shape, tile)
+            kr, kp = _native.bin_fill(center, radius, shape, tile, kept.offsets, int(kept.total.item()), kept=kept)
+            vr, vi, kept = _native.forward(center, cov, rr, ri, kr, kp, hs, tile, d, h, w, keep=kept)
+            vol = (vr, vi)
+            gc, gm, grr, gri = _native.backward(center, cov, rr, ri, kr, kp, hs, g_r, g_i, tile, d, h, w, kept=kept)
+    torch.cuda.synchronize()
+    c32 = center.cpu().numpy()
+    cv = cov.cpu().numpy()
+    h64 = hs.cpu().numpy()
+    r_exp, p_exp = vo.bin_tiles(c32, radius.cpu().numpy(), shape, tile)
+    er, ei, amb = vo.forward(c32, cv, rr_np, ri_np, r_exp, p_exp, h64, shape, tile, band=BAND)
+    ok = ~amb
+    for got, exp in ((vol[0], er), (vol[1], ei)):
+        got = got.cpu().numpy()
+        np.testing.assert_array_equal(_pattern(got)[ok], _pattern(exp)[ok])
+        fin = ok & np.isfinite(exp)
+        assert (~np.isfinite(exp)).sum() > 20
+        e = cases.rel_l2(got[fin], exp[fin])
+        print(f"  volume: {int((~np.isfinite(exp)).sum())} non-finite voxels match; finite rel L2 {e:.2e}")
+        assert e <= TOL
+    dc, dm, drr, dri, gamb = vo.backward(c32, cv, rr_np, ri_np, r_exp, p_exp, h64, g_r_np, g_i_np, shape, tile,
+                                         band=BAND)
+    okg = ~gamb
+    for nm, got, exp in (("rho_r", grr, drr), ("rho_i", gri, dri)):
+        got = got.detach().cpu().numpy()
+        np.testing.assert_array_equal(_pattern(got)[okg], _pattern(exp)[okg])
+        fin = okg & np.isfinite(exp)
+        e = cases.rel_l2(got[fin], exp[fin])
+        print(f"  d/d{nm}: {int((~np.isfinite(exp)).sum())} non-finite; finite rel L2 {e:.2e}")
+        assert e <= TOL
+    if gc is not None:
+        for nm, got, exp in (("center", gc, dc), ("cov_inv", gm.reshape(-1, 3, 3), dm)):
+            got = got.cpu().numpy()
+            fin_exp = np.isfinite(exp).reshape(len(exp), -1).all(1)
+            fin_got = np.isfinite(got).reshape(len(got), -1).all(1)
+            np.testing.assert_array_equal(fin_got[okg], fin_exp[okg])
+            sel = okg & fin_exp
+            e = cases.rel_l2(got[sel], exp[sel])
+            print(f"  d/d{nm}: {int((~fin_exp).sum())} non-finite Gaussians; finite rel L2 {e:.2e}")
+            assert e <= TOL

@@ -342,3 +342,151 @@ def test_cpu_tile_path_matches_fp64_oracle():
     assert cases.rel_l2(gc.numpy()[okg], dc[okg]) < 1e-5
     assert cases.rel_l2(gm.numpy()[okg], dm[okg]) < 1e-5
     assert cases.rel_l2(np.stack([grr.numpy(), gri.numpy()], 1)[okg], np.stack([drr, dri], 1)[okg]) < 1e-5
+
+
+def test_non_finite_density_and_gradient_reach_only_the_cutoff_set():
+    """The reference kernel adds an evaluation only inside `if (mahal <= 9)`
+    (cuda_ext/tile_voxelize_cuda_kernel.cu:130, :233), so a NaN / inf density
+    or upstream gradient reaches exactly the voxels (Gaussians) within the
+    cutoff.  Brute force: the NaN / inf pattern of the oracle's volume is the
+    set of voxels with q <= 9 for the poisoned Gaussians (q enumerated per
+    pair), and everything else equals the call without them."""
+    c = cases.edge_case("partial_tiles")
+    shape, tile = c["shape"], c["tile"]
+    c32, r32, cov, hs = vo.prologue(c["pos"], c["scales"], c["rot"], shape)
+    ranges, pids = vo.bin_tiles(c32, r32, shape, tile)
+    rr = c["rr"].astype(np.float64).copy()
+    ri = c["ri"].astype(np.float64).copy()
+    bad = {3: (np.nan, 0.0), 11: (np.inf, 0.0), 17: (0.5, -np.inf)}
+    for g, (a, b) in bad.items():
+        rr[g], ri[g] = a, b
+    vr, vi, _ = vo.forward(c32, cov, rr, ri, ranges, pids, hs, shape, tile)
+    # Brute-force cutoff sets of the poisoned Gaussians over their tiles.
+    hit = {g: np.zeros(int(np.prod(shape)), bool) for g in bad}
+    for t in range(ranges.shape[0]):
+        vox, flat = vo._tile_voxels(t, shape, tile)
+        for g in pids[ranges[t, 0]:ranges[t, 1]]:
+            if int(g) in bad:
+                d = (vox - c32[g].astype(np.float64)) / hs.astype(np.float64)
+                q = np.einsum("vi,ij,vj->v", d, cov[g], d)
+                hit[int(g)][flat[q <= 9.0]] = True
+    assert all(h.any() for h in hit.values())
+    np.testing.assert_array_equal(np.isnan(vr), hit[3])
+    np.testing.assert_array_equal(np.isposinf(vr), hit[11] & ~hit[3])
+    np.testing.assert_array_equal(np.isneginf(vi), hit[17])
+    good = np.ones(len(rr), bool)
+    good[list(bad)] = False
+    keep = ~(hit[3] | hit[11] | hit[17])
+    rg, pg = vo.bin_tiles(c32[good], r32[good], shape, tile)
+    wr, wi, _ = vo.forward(c32[good], cov[good], rr[good], ri[good], rg, pg, hs, shape, tile)
+    np.testing.assert_allclose(vr[keep], wr[keep], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(vi[~hit[17]], wi[~hit[17]], rtol=1e-12, atol=1e-14)
+    # An infinite upstream gradient at one voxel: only Gaussians whose
+    # cutoff set holds that voxel get non-finite gradients.
+    rng = np.random.default_rng(2)
+    g_r = rng.normal(size=int(np.prod(shape)))
+    g_i = rng.normal(size=int(np.prod(shape)))
+    x = int(np.argmax(np.abs(wr)))
+    g_r[x] = np.inf
+    dc, dm, drr, dri, _ = vo.backward(c32[good], cov[good], rr[good], ri[good], rg, pg, hs, g_r, g_i, shape, tile)
+    reach = np.zeros(int(good.sum()), bool)
+    for t in range(rg.shape[0]):
+        vox, flat = vo._tile_voxels(t, shape, tile)
+        if x not in flat:
+            continue
+        xv = vox[list(flat).index(x)]
+        for g in pg[rg[t, 0]:rg[t, 1]]:
+            d = (xv - c32[good][g].astype(np.float64)) / hs.astype(np.float64)
+            reach[g] = d @ cov[good][g] @ d <= 9.0
+    assert reach.any() and not reach.all()
+    np.testing.assert_array_equal(~np.isfinite(drr), reach)
+    assert np.isfinite(dri).all()
+    assert np.isfinite(dc[~reach]).all() and np.isfinite(dm[~reach]).all()
+
+
+def test_torch64_gradients_vs_central_finite_differences_all_parameters():
+    """FP64 central finite differences (h = 1e-6) of the FP64 call
+    (oracle/torch64.py, the reference for the GPU's parameter gradients) with
+    respect to every parameter -- position, log-scale, quaternion and complex
+    amplitude -- of sampled Gaussians of an adjoint-check draw (32^3, voxel
+    sigma exp(N(0, 0.3)), random rotations), with the tile lists fixed.
+    Parameters whose +-h perturbation moves any evaluation across the q = 9
+    cutoff are skipped (the function is discontinuous there) and counted."""
+    c = cases.adjoint_check_case(0, "voxel")
+    shape, tile = c["shape"], c["tile"]
+    rng = np.random.default_rng(17)
+    sub = np.sort(rng.choice(len(c["rr"]), size=64, replace=False))
+    pos, sc, rot = c["pos"][sub], c["scales"][sub], c["rot"][sub]
+    rr, ri = c["rr"][sub], c["ri"][sub]
+    c32, r32, _, _ = vo.prologue(pos, sc, rot, shape)
+    ranges, pids = vo.bin_tiles(c32, r32, shape, tile)
+    nvox = int(np.prod(shape))
+    g_r = torch.from_numpy(rng.normal(size=nvox))
+    g_i = torch.from_numpy(rng.normal(size=nvox))
+    base = {"pos": pos, "log_s": np.log(sc), "rot": rot, "rr": rr, "ri": ri}
+    base = {k: np.asarray(v, np.float64) for k, v in base.items()}
+
+    def run(p, grad=False, rg=ranges):
+        t = {k: torch.from_numpy(v.copy()).requires_grad_(grad) for k, v in p.items()}
+        vr, vi, act = torch64.voxelize64(t["pos"], t["log_s"], t["rot"], t["rr"], t["ri"], shape, tile, rg,
+                                         pids, return_active=True)
+        loss = (vr * g_r).sum() + (vi * g_i).sum()
+        if grad:
+            loss.backward()
+            return {k: v.grad.numpy() for k, v in t.items()}, act
+        return float(loss), act
+
+    grads, _ = run(base, grad=True)
+    h = 1e-6
+    checked = skipped = 0
+    for g in rng.choice(64, size=8, replace=False):
+        # Only the tiles that hold Gaussian g change with its parameters: the
+        # differences are taken over those tiles' evaluations alone.
+        mine = np.array([g in pids[a:b] for a, b in ranges])
+        rg = np.where(mine[:, None], ranges, 0).astype(ranges.dtype)
+        _, act0 = run(base, rg=rg)
+        for k, width in (("pos", 3), ("log_s", 3), ("rot", 4), ("rr", 1), ("ri", 1)):
+            for j in range(width):
+                vals = []
+                crossed = False
+                for sign in (1.0, -1.0):
+                    p = {kk: vv.copy() for kk, vv in base.items()}
+                    if width == 1:
+                        p[k][g] += sign * h
+                    else:
+                        p[k][g, j] += sign * h
+                    val, act = run(p, rg=rg)
+                    crossed |= any(bool((a != b).any()) for a, b in zip(act, act0))
+                    vals.append(val)
+                if crossed:
+                    skipped += 1
+                    continue
+                fd = (vals[0] - vals[1]) / (2 * h)
+                an = grads[k][g] if width == 1 else grads[k][g, j]
+                assert fd == pytest.approx(an, rel=1e-5, abs=1e-6), (int(g), k, j, fd, an)
+                checked += 1
+    print(f"checked {checked} parameters, skipped {skipped} (cutoff crossings)")
+    assert checked >= 96 and skipped <= checked // 10
+
+
+@pytest.mark.parametrize("level,name", [("function", "single"), ("function", "extreme_aniso"), ("call", "random0"),
+                                        ("call", "extreme_aniso")])
+def test_reference_fp32_error_table(level, name):
+    """The committed table of the reference's own fp32 error (from which the
+    GPU parity tests take their bounds) is what its script measures: the
+    reference's CPU composition in fp32 against FP64, recomputed here."""
+    from gs_tmi_amd.tile_voxelizer import voxel_space_params
+    from tests import reference_fp32 as rf
+
+    table = rf.load_table()[level][name]
+    c = cases.case_by_name(name)
+    if level == "function":
+        t = torch.from_numpy
+        center, _, cov, _ = voxel_space_params(t(c["pos"]), t(c["scales"]), t(c["rot"]), c["shape"])
+        g_r, g_i = cases.function_upstream(c["shape"])
+        got = rf.function_errors(c, center.numpy(), cov.numpy(), g_r, g_i, 2e-5)
+    else:
+        g_r, g_i = cases.call_upstream(c["shape"])
+        got = rf.call_errors(c, g_r, g_i)
+    for k, v in table.items():
+        assert got[k] == pytest.approx(v, rel=0.25, abs=2e-7), (k, got[k], v)
