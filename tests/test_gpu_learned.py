@@ -106,3 +106,49 @@ def test_inference_matches_unbatched_and_is_deterministic(small):
         o1 = li.model(to_bf16(x[:4096]))
         o2 = li.model(to_bf16(x[:4096]))
     assert torch.equal(o1, o2)
+
+
+@pytest.mark.timeout(900)
+def test_learned_time_to_nmse_ratio_at_equal_final_psnr():
+    """Held-out config-4 acquisition (seed 7), tuned schedule, both starts warm:
+    the learned start reaches NMSE 1e-2 in at most half the wall time of the
+    plain start (initialisation + training, evaluations excluded), and after
+    the same number of steps (2000) its PSNR is not below the plain one's
+    (within 0.1 dB)."""
+    import time
+
+    from gs_tmi_amd import TileVoxelizer
+    from gs_tmi_amd.learned.init import LearnedInitialiser, initialise
+    from gs_tmi_amd.learned.pipeline import schedules
+    from gs_tmi_amd.recon.data import make_problem, poisson_disc_mask
+    from gs_tmi_amd.recon.trainer import Reconstructor
+
+    print("[learned] time to NMSE 1e-2, held-out seed 7, tuned schedule, 2000 steps each", flush=True)
+    shape = (256, 256, 128)
+    p = make_problem(shape, 60, 16, poisson_disc_mask((256, 128), 8, 24, 7), 7, noise_sigma=1e-3, device=DEV,
+                     phantom_on_device=True)
+    vox = TileVoxelizer(shape, tile_size=8, device=DEV)
+    n_plain, cfg = schedules()["tuned"]
+    learned = LearnedInitialiser(device=DEV)
+    for init in (learned, None):  # warm-up: FFT plans, first launches, allocator growth
+        g, _ = initialise(p, vox, init, plain_points=n_plain)
+        Reconstructor(p, g, vox, cfg).run(5, eval_every=5)
+        del g
+    res = {}
+    for kind, init in (("learned", learned), ("plain", None)):
+        torch.cuda.synchronize()
+        g, info = initialise(p, vox, init, plain_points=n_plain)
+        assert info["used"] == kind
+        hist = Reconstructor(p, g, vox, cfg).run(2000, eval_every=25)
+        t = hist.time_to("nmse", 1e-2)
+        res[kind] = {"time_to_target_s": None if t is None else info["init_s"] + t, "init_s": info["init_s"],
+                     "final_psnr": hist.rows[-1]["psnr"], "final_nmse": hist.rows[-1]["nmse"]}
+        print(f"[learned] {kind}: {res[kind]}", flush=True)
+        del g, hist
+        torch.cuda.empty_cache()
+    tl, tp = res["learned"]["time_to_target_s"], res["plain"]["time_to_target_s"]
+    assert tl is not None and tp is not None
+    print(f"[learned] ratio {tl / tp:.3f}; final PSNR learned - plain "
+          f"{res['learned']['final_psnr'] - res['plain']['final_psnr']:+.3f} dB", flush=True)
+    assert tl <= 0.5 * tp
+    assert res["learned"]["final_psnr"] >= res["plain"]["final_psnr"] - 0.1

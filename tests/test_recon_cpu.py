@@ -89,3 +89,28 @@ def test_adam_steps_through_cpu_voxelizer_reduce_loss():
     assert losses[-1] < 0.7 * losses[0]
     m = r.evaluate()
     assert m["psnr"] > 15 and m["nmse"] < 1.0
+
+
+def test_config1_cpu_path_fixture_is_reproducible():
+    """The committed configuration-1 run of the reference's CPU path
+    (tests/golden/config1_cpu_path.json, against which the GPU test checks the
+    HIP path's PSNR) is what its script computes: its first two steps,
+    evaluated after each, are recomputed here."""
+    import json
+    from pathlib import Path
+
+    from gs_tmi_amd.recon import Reconstructor
+    from gs_tmi_amd.recon.configs import config1_gaussians, config1_problem, config1_recon_config
+    from oracle.cpu_tile_torch import CpuTileVoxelizer
+
+    golden = json.loads((Path(__file__).resolve().parent / "golden" / "config1_cpu_path.json").read_text())
+    p = config1_problem("cpu")
+    g = config1_gaussians(p)
+    rows = Reconstructor(p, g, CpuTileVoxelizer(p.shape), config1_recon_config()).run(2, eval_every=1).rows
+    assert golden["history"][0]["psnr"] == pytest.approx(rows[0]["psnr"], abs=1e-6)
+    for a, b in zip(rows, golden["pin_two_steps"]):
+        assert a["step"] == b["step"]
+        assert a["psnr"] == pytest.approx(b["psnr"], abs=1e-3)
+        assert a["nmse"] == pytest.approx(b["nmse"], rel=1e-3)
+        if b["loss"] is not None:
+            assert a["loss"] == pytest.approx(b["loss"], rel=1e-4)
