@@ -43,6 +43,9 @@ PROTOTYPES = {
     "gstv_backward": (ctypes.c_int,
                       [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32,
                        _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _sz, _c_p]),
+    "gstv_backward_noalloc": (ctypes.c_int,
+                              [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _c_p, _c_p, _i32, _i32, _i32,
+                               _i32, _c_p, _c_p, _c_p, _c_p, _c_p]),
     "gstv_keep_workspace_bytes": (ctypes.c_size_t, [_i32]),
     "gstv_keep_prepare": (ctypes.c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _i32, _i32, _i32, _i32, _i32,
                                          _c_p, _c_p, _c_p, _sz, _c_p]),
@@ -383,6 +386,32 @@ def backward(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shap
                                _ptr(grad_i), tile_size, d, h, w, _ptr(g_c), _ptr(g_m), _ptr(g_rr), _ptr(g_ri),
                                _ptr(ws), ws_bytes, _stream(dev))
     _check(st, "gstv_backward")
+    return g_c, g_m, g_rr, g_ri
+
+
+NOALLOC_MAX_GAUSSIANS = 262144  # GSTV_NOALLOC_MAX_GAUSSIANS
+
+
+def backward_noalloc(center_vox, cov_inv, rho_r, rho_i, tile_ranges, pair_ids, half_shape, grad_r, grad_i,
+                     tile_size, d, h, w, out=None):
+    """gstv_backward_noalloc: the adjoint without a workspace (the library's
+    static device array; at most NOALLOC_MAX_GAUSSIANS Gaussians).  ``out``:
+    optional preallocated (grad_center [n,3], grad_cov_inv [n,9], grad_rho_r [n],
+    grad_rho_i [n]); returns them."""
+    lib = get_lib()
+    _require_gpu(center_vox, "center_vox")
+    dev = center_vox.device
+    n = center_vox.shape[0]
+    if out is None:
+        out = (torch.empty(n, 3, dtype=torch.float32, device=dev), torch.empty(n, 9, dtype=torch.float32, device=dev),
+               torch.empty(n, dtype=torch.float32, device=dev), torch.empty(n, dtype=torch.float32, device=dev))
+    g_c, g_m, g_rr, g_ri = out
+    with torch.cuda.device(dev):
+        st = lib.gstv_backward_noalloc(_ptr(center_vox), _ptr(cov_inv), _ptr(rho_r), _ptr(rho_i), _ptr(tile_ranges),
+                                       tile_ranges.shape[0], _ptr(pair_ids), _ptr(half_shape), n, _ptr(grad_r),
+                                       _ptr(grad_i), tile_size, d, h, w, _ptr(g_c), _ptr(g_m), _ptr(g_rr),
+                                       _ptr(g_ri), _stream(dev))
+    _check(st, "gstv_backward_noalloc")
     return g_c, g_m, g_rr, g_ri
 
 

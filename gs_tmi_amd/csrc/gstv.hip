@@ -47,6 +47,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "gstv.h"
@@ -2467,6 +2468,39 @@ void prof_stop(int kind, int slot, hipStream_t s) {
 }
 
 
+// ---------------------------------------------- no-allocation workspace --
+//
+// gstv_backward_noalloc keeps its workspace (the keep layout: records,
+// column-bound records, moment accumulator, scan sums) in this file-scope
+// device array instead of a caller's buffer.  Every GPU the library runs on
+// holds its own copy, at its own address, until the process ends; in host
+// code `scratch` names a host-side stand-in, so kernels get the device address
+// from hipGetSymbolAddress, resolved once per device with that device current.
+constexpr size_t kNoAllocScratchBytes =
+    (size_t)GSTV_NOALLOC_MAX_GAUSSIANS * (sizeof(GRec) + sizeof(ZRec) + kAccStride * sizeof(float)) + (1u << 16);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of the current device's scratch array.
+int scratch_of_current_device(void** out) {
+  int dev = 0;
+  int st = hip_status(hipGetDevice(&dev), "noalloc_get_device");
+  if (st != GSTV_OK) return st;
+  if (dev < 0 || dev >= kMaxDevices) return GSTV_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    st = hip_status(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "noalloc_symbol_address");
+    if (st != GSTV_OK) return st;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return GSTV_OK;
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -2480,6 +2514,8 @@ const char* gstv_status_string(int status) {
     case GSTV_ERR_WORKSPACE_TOO_SMALL: return "workspace too small";
     case GSTV_ERR_TOO_MANY_PAIRS: return "too many (tile, Gaussian) pairs: total must be < 2^31";
     case GSTV_ERR_HIP: return g_last_detail[0] ? g_last_detail : "HIP runtime error";
+    case GSTV_ERR_TOO_MANY_GAUSSIANS:
+      return "too many Gaussians for gstv_backward_noalloc (at most GSTV_NOALLOC_MAX_GAUSSIANS = 262144)";
     default: return "unknown status";
   }
 }
@@ -2801,6 +2837,39 @@ int gstv_backward(const float* center_vox, const float* cov_inv, const float* rh
   launch_backward(ws.rec, ws.zt, ws.gacc, cov_inv, half_shape, tile_ranges, num_tiles, pair_ids, n, grad_r, grad_i, tile, d,
                   h, w, nt, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, 1, nullptr, s);
   GSTV_LAUNCH_CHECK("backward");
+  return GSTV_OK;
+}
+
+int gstv_backward_noalloc(const float* center_vox, const float* cov_inv, const float* rho_r, const float* rho_i,
+                          const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
+                          const float* half_shape, int32_t n, const float* grad_r, const float* grad_i, int32_t tile,
+                          int32_t d, int32_t h, int32_t w, float* grad_center, float* grad_cov_inv,
+                          float* grad_rho_r, float* grad_rho_i, void* stream) {
+  int nt[3];
+  int st = check_tile_args(tile_ranges, num_tiles, n, tile, d, h, w, nt);
+  if (st != GSTV_OK) return st;
+  if (n > GSTV_NOALLOC_MAX_GAUSSIANS) return GSTV_ERR_TOO_MANY_GAUSSIANS;
+  if (n == 0) return GSTV_OK;
+  if (center_vox == nullptr || cov_inv == nullptr || rho_r == nullptr || rho_i == nullptr ||
+      half_shape == nullptr || grad_r == nullptr || grad_i == nullptr || grad_center == nullptr ||
+      grad_cov_inv == nullptr || grad_rho_r == nullptr || grad_rho_i == nullptr)
+    return GSTV_ERR_INVALID_ARGUMENT;
+  if (keep_layout(nullptr, n, nullptr) > kNoAllocScratchBytes) return GSTV_ERR_TOO_MANY_GAUSSIANS;
+  void* ws_ptr = nullptr;
+  st = scratch_of_current_device(&ws_ptr);
+  if (st != GSTV_OK) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  KeepLayout ws;
+  keep_layout(ws_ptr, n, &ws);
+  // Every scratch region is rewritten before it is read: records by the
+  // prep kernel, the accumulator by a fill kernel (no memset node, so the
+  // calls can be captured into a graph).
+  st = launch_prep(center_vox, cov_inv, rho_r, rho_i, 1, half_shape, n, ws.rec, ws.zt, s);
+  if (st != GSTV_OK) return st;
+  launch_clear_acc(ws.gacc, n, s);
+  launch_backward(ws.rec, ws.zt, ws.gacc, cov_inv, half_shape, tile_ranges, num_tiles, pair_ids, n, grad_r, grad_i,
+                  tile, d, h, w, nt, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, 1, nullptr, s);
+  GSTV_LAUNCH_CHECK("backward_noalloc");
   return GSTV_OK;
 }
 

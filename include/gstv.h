@@ -50,6 +50,7 @@ extern "C" {
 #define GSTV_ERR_WORKSPACE_TOO_SMALL 2
 #define GSTV_ERR_TOO_MANY_PAIRS 3
 #define GSTV_ERR_HIP 4
+#define GSTV_ERR_TOO_MANY_GAUSSIANS 5
 
 /* Human-readable message for a status code (static storage, never NULL). */
 const char* gstv_status_string(int status);
@@ -146,6 +147,32 @@ int gstv_backward(const float* center_vox, const float* cov_inv,
                   float* grad_center, float* grad_cov_inv,
                   float* grad_rho_r, float* grad_rho_i,
                   void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------- adjoint, no allocation --
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory or hand in a workspace: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  gstv_backward with the same
+ * arguments and results (up to the order of fp32 atomic additions), without
+ * the workspace: it keeps its workspace in a static device array of the
+ * library, 37.8 MB (GSTV_NOALLOC_MAX_GAUSSIANS x 144 B + 64 KiB) on every
+ * GPU it runs on, held until the process ends.
+ * It allocates nothing and makes no memset (its kernels rewrite every part of
+ * the array they read).
+ *   n  at most GSTV_NOALLOC_MAX_GAUSSIANS; larger n returns
+ *      GSTV_ERR_TOO_MANY_GAUSSIANS (use gstv_backward).
+ * Every call on one GPU shares that GPU's array: calls that overlap on a GPU
+ * (concurrent streams or threads) race and give wrong results; calls ordered
+ * on one stream are safe. */
+#define GSTV_NOALLOC_MAX_GAUSSIANS 262144
+
+int gstv_backward_noalloc(const float* center_vox, const float* cov_inv,
+                          const float* rho_r, const float* rho_i,
+                          const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,
+                          const float* half_shape, int32_t n,
+                          const float* grad_r, const float* grad_i,
+                          int32_t tile, int32_t d, int32_t h, int32_t w,
+                          float* grad_center, float* grad_cov_inv,
+                          float* grad_rho_r, float* grad_rho_i, void* stream);
 
 /* ------------------------------------------- forward and adjoint of one call --
  * No counterpart in the reference: a fast path for callers that run the
