@@ -247,7 +247,9 @@ struct __align__(16) GRec {
   float c0, c1, c2, rr;      // centre (voxel units, axes D,H,W), density real
   float ri, y1h, y1d, y1w;   // density imag; first linear form
   float y2d, y2w, y3w, e0;   // second linear form, u2^2 coefficient; extent D
-  float e1, e2, pad0, pad1;  // extents H, W
+  float e1, e2;              // extents H, W
+  int32_t gid;               // the Gaussian's own index
+  float pad;
 };
 static_assert(sizeof(GRec) == 64, "record must be one 64-byte line");
 
@@ -962,8 +964,8 @@ __device__ __forceinline__ void prep_one(int g, const float c[3], const float mm
   r.e0 = e0;
   r.e1 = e1;
   r.e2 = e2;
-  r.pad0 = 0.0f;
-  r.pad1 = 0.0f;
+  r.gid = g;
+  r.pad = 0.0f;
   rec[g] = r;
   // Column-bound projections in voxel units (C = A^-1, cofactors of A):
   //   al = C_0i / C_ii = cof_0i / cof_ii,  ga = sqrt(C_00 C_ii - C_0i^2) / C_ii
@@ -1075,14 +1077,11 @@ struct Stage {
   uint16_t list[kWaves][kBatch];
   uint16_t touch[kBatch];  // pairs whose box touches the tile (any order)
   int n_touch;
-  // Staged pairs (touching the tile) whose density is not finite: their
-  // staged density is zeroed and their own density kept here.  The count
-  // alternates between two slots by batch parity, so one slot is cleared
-  // while the other is read.
-  uint16_t bad_k[kBatch];
-  float bad_rr[kBatch], bad_ri[kBatch];
-  int n_bad[2];
-  float bad_acc_r[kTileThreads], bad_acc_i[kTileThreads];  // their sums, one slot per thread's voxel
+  // A staged pair touching the tile had a non-finite density (its staged
+  // density was zeroed; k_forward adds it after the batches).  Set during
+  // staging, never cleared within a tile.
+  int any_bad;
+  int n_bad;  // pairs listed by the non-finite pass of the current batch
 };
 
 struct BlockCtx {
@@ -1396,29 +1395,57 @@ __device__ __forceinline__ void fwd_groups(const Stage& sh, int nhit, const Bloc
   }
 }
 
-// Contributions of the batch's staged Gaussians whose density is not finite
-// (their staged density is zero, so the groups above add nothing for them).
-// Thread (wave, gl, xi) evaluates the voxel (z, y, x) = (gl / 4, gl % 4, xi)
-// of the wave's sub-brick -- the voxel it writes -- and adds w rho to its own
-// LDS slot only where the cutoff test passes, as the reference does (0 * inf
-// would be NaN).  The quadratic form is evaluated as in fwd_groups.
-__device__ __forceinline__ void fwd_non_finite(Stage& sh, int nbad, const BlockCtx& b, int gl, int xi) {
+// Contributions of the tile's Gaussians whose density is not finite, after
+// the batches (their staged density was zero, so the groups added nothing
+// for them; a zero weight times inf would be NaN).  The tile's list is read
+// again and such pairs staged in batches; thread (wave, gl, xi) evaluates the
+// voxel (z, y, x) = (gl / 4, gl % 4, xi) of its wave's sub-brick -- the voxel
+// whose total it holds in out_r / out_i -- and adds w rho only where the
+// cutoff test passes, as the reference does.  The quadratic form is evaluated
+// as in fwd_groups.  All threads of the block must call.
+__device__ __forceinline__ void fwd_non_finite(Stage& sh, const GRec* __restrict__ rec,
+                                               const int32_t* __restrict__ pair_ids, const BlockCtx& b, int gl,
+                                               int xi, float& out_r, float& out_i) {
   const int z = gl >> 2, y = gl & 3;
-  if (!(z < b.n0 && y < b.n1 && xi < b.n2)) return;
-  for (int j = 0; j < nbad; ++j) {
-    const int k = sh.bad_k[j];
-    const float u2 = (float)(b.o2 + b.s2 + xi) - sh.c2[k];
-    const float u0 = (float)(b.o0 + b.s0 + z) - sh.c0[k];
-    const float u1 = (float)(b.o1 + b.s1 + y) - sh.c1[k];
-    const float t1 = fmaf(sh.y1d[k], u0, sh.y1w[k] * u2);
-    const float y2 = fmaf(sh.y2d[k], u0, sh.y2w[k] * u2);
-    const float r = fmaf(-y2, y2, -(sh.y3w[k] * u2 * u2));
-    const float y1 = fmaf(sh.y1h[k], u1, t1);
-    const float q = fmaf(-y1, y1, r);
-    if (q >= kThr) {
-      const float w = __builtin_amdgcn_exp2f(q);
-      sh.bad_acc_r[threadIdx.x] = fmaf(w, sh.bad_rr[j], sh.bad_acc_r[threadIdx.x]);
-      sh.bad_acc_i[threadIdx.x] = fmaf(w, sh.bad_ri[j], sh.bad_acc_i[threadIdx.x]);
+  const bool vox_ok = b.active && z < b.n0 && y < b.n1 && xi < b.n2;
+  const float x0 = (float)(b.o0 + b.s0 + z), x1 = (float)(b.o1 + b.s1 + y), x2 = (float)(b.o2 + b.s2 + xi);
+  for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
+    const int cnt = min(kBatch, b.p_end - bs);
+    __syncthreads();
+    if (threadIdx.x == 0) sh.n_bad = 0;
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) {
+      const GRec r = rec[pair_ids[bs + threadIdx.x]];
+      // e0 < 0 marks a non-finite centre (no voxel passes its test).
+      if ((non_finite(r.rr) || non_finite(r.ri)) && r.e0 >= 0.0f) {
+        const int j = atomicAdd(&sh.n_bad, 1);
+        sh.c0[j] = r.c0;
+        sh.c1[j] = r.c1;
+        sh.c2[j] = r.c2;
+        sh.rr[j] = r.rr;
+        sh.ri[j] = r.ri;
+        sh.y1h[j] = r.y1h;
+        sh.y1d[j] = r.y1d;
+        sh.y1w[j] = r.y1w;
+        sh.y2d[j] = r.y2d;
+        sh.y2w[j] = r.y2w;
+        sh.y3w[j] = r.y3w;
+      }
+    }
+    __syncthreads();
+    const int nb = sh.n_bad;
+    for (int j = 0; j < nb && vox_ok; ++j) {
+      const float u0 = x0 - sh.c0[j], u1 = x1 - sh.c1[j], u2 = x2 - sh.c2[j];
+      const float t1 = fmaf(sh.y1d[j], u0, sh.y1w[j] * u2);
+      const float y2 = fmaf(sh.y2d[j], u0, sh.y2w[j] * u2);
+      const float rz = fmaf(-y2, y2, -(sh.y3w[j] * u2 * u2));
+      const float y1 = fmaf(sh.y1h[j], u1, t1);
+      const float q = fmaf(-y1, y1, rz);
+      if (q >= kThr) {
+        const float w = __builtin_amdgcn_exp2f(q);
+        out_r = fmaf(w, sh.rr[j], out_r);
+        out_i = fmaf(w, sh.ri[j], out_i);
+      }
     }
   }
 }
@@ -1452,62 +1479,39 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_forward(const GRec* __restr
   // Records of the next batch are loaded into registers while the current
   // batch is evaluated (loads stay in flight across the LDS barriers).
   GRec nxt = {};
-  int nxt_id = 0;
-  if ((int)threadIdx.x < min(kBatch, b.p_end - b.p_begin)) {
-    nxt_id = pair_ids[b.p_begin + threadIdx.x];
-    nxt = rec[nxt_id];
-  }
+  if ((int)threadIdx.x < min(kBatch, b.p_end - b.p_begin)) nxt = rec[pair_ids[b.p_begin + threadIdx.x]];
   const bool use_mask = geo.nsb <= 2;
   if (threadIdx.x == 0) {
     sh.n_touch = 0;
-    sh.n_bad[0] = sh.n_bad[1] = 0;
+    sh.any_bad = 0;
   }
-  sh.bad_acc_r[threadIdx.x] = 0.0f;
-  sh.bad_acc_i[threadIdx.x] = 0.0f;
-  bool any_bad = false;  // block-uniform
-  int par = 0;
-  for (int bs = b.p_begin; bs < b.p_end; bs += kBatch, par ^= 1) {
+  for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
     const int cnt = min(kBatch, b.p_end - bs);
     __syncthreads();
-    const bool touch = (int)threadIdx.x < cnt && stage_pair(sh, threadIdx.x, nxt, nxt_id, b);
+    const bool touch = (int)threadIdx.x < cnt && stage_pair(sh, threadIdx.x, nxt, nxt.gid, b);
     if (touch && (non_finite(nxt.rr) || non_finite(nxt.ri))) {
       // Zero weights times a non-finite density would be NaN: such a
-      // Gaussian is evaluated on its own below.
+      // Gaussian adds nothing here and is evaluated on its own after the
+      // batches (fwd_non_finite).
       sh.rr[threadIdx.x] = 0.0f;
       sh.ri[threadIdx.x] = 0.0f;
-      const int j = atomicAdd(&sh.n_bad[par], 1);
-      sh.bad_k[j] = (uint16_t)threadIdx.x;
-      sh.bad_rr[j] = nxt.rr;
-      sh.bad_ri[j] = nxt.ri;
+      sh.any_bad = 1;
     }
     const int bn = bs + kBatch;
-    if (bn < b.p_end && (int)threadIdx.x < min(kBatch, b.p_end - bn)) {
-      nxt_id = pair_ids[bn + threadIdx.x];
-      nxt = rec[nxt_id];
-    }
-    int nbad;
+    if (bn < b.p_end && (int)threadIdx.x < min(kBatch, b.p_end - bn)) nxt = rec[pair_ids[bn + threadIdx.x]];
     if (use_mask) {
       append_touch(sh, touch, threadIdx.x, lane);
       __syncthreads();
-      nbad = sh.n_bad[par];
       batch_masks(sh, b, bs, zt, pair_cols);
     }
     __syncthreads();
-    if (!use_mask) nbad = sh.n_bad[par];
-    if (threadIdx.x == 0) {
-      sh.n_touch = 0;
-      sh.n_bad[par ^ 1] = 0;  // the next batch's count (every wave has read the last batch's)
-    }
+    if (threadIdx.x == 0) sh.n_touch = 0;
     if (!b.active) continue;
     const int nhit = build_list(sh, cnt, b, wave, lane, use_mask);
     if (full)
       fwd_groups<false>(sh, nhit, b, wave, gl, xi, x_ok, acc_r, acc_i);
     else
       fwd_groups<true>(sh, nhit, b, wave, gl, xi, x_ok, acc_r, acc_i);
-    if (nbad > 0) {
-      any_bad = true;
-      fwd_non_finite(sh, nbad, b, gl, xi);
-    }
   }
 
   // Sum the 16 Gaussian slots of each W column (a row of 16 lanes).
@@ -1523,10 +1527,8 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_forward(const GRec* __restr
     out_r = (gl == i) ? acc_r[i] : out_r;
     out_i = (gl == i) ? acc_i[i] : out_i;
   }
-  if (any_bad) {
-    out_r += sh.bad_acc_r[threadIdx.x];
-    out_i += sh.bad_acc_i[threadIdx.x];
-  }
+  // any_bad is final: every batch's staging is behind a barrier.
+  if (sh.any_bad != 0) fwd_non_finite(sh, rec, pair_ids, b, gl, xi, out_r, out_i);
   const int z = gl >> 2, y = gl & 3;
   if (b.active && x_ok && z < b.n0 && y < b.n1) {
     const int64_t flat = ((int64_t)(b.o0 + b.s0 + z) * geo.H + (b.o1 + b.s1 + y)) * geo.W + (b.o2 + b.s2 + xi);
