@@ -10,6 +10,7 @@ missing, every entry point raises RuntimeError.
 from __future__ import annotations
 
 import ctypes
+import math
 import threading
 from pathlib import Path
 
@@ -55,6 +56,9 @@ PROTOTYPES = {
     "gstv_keep_bin_fill": (ctypes.c_int,
                            [_c_p, _sz, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32, _c_p, ctypes.c_int64, _c_p, _c_p,
                             _c_p, _sz, _c_p]),
+    "gstv_keep_bin_fill_capped": (ctypes.c_int,
+                                  [_c_p, _sz, _c_p, _c_p, _i32, _i32, _i32, _i32, _i32, _c_p, _c_p, ctypes.c_int64,
+                                   _c_p, _c_p, _c_p, _sz, _c_p]),
     "gstv_forward_keep": (ctypes.c_int,
                           [_c_p, _sz, _c_p, _i32, _c_p, _i32, _i32, _i32, _i32, _i32, _c_p, _c_p, _c_p, _c_p]),
     "gstv_backward_keep": (ctypes.c_int,
@@ -166,11 +170,17 @@ def bin_count(center_vox: torch.Tensor, radius_vox: torch.Tensor, shape, tile_si
     return offsets, total
 
 
-def bin_fill(center_vox, radius_vox, shape, tile_size, offsets, total_pairs: int, kept=None, tile_ranges=None):
+def bin_fill(center_vox, radius_vox, shape, tile_size, offsets, total_pairs: int, kept=None, tile_ranges=None,
+             capped: bool = False, pair_ids=None):
     """Phase two of binning.  Returns (tile_ranges int32 [T,2], pair_ids int32 [P]).
     ``kept``: a ``Kept`` whose ``keep_prepare`` counted the pairs (offsets
     from it): the pairs whose tile meets a Gaussian's extent box.
-    ``tile_ranges``: an int32 [T, 2] array to fill (allocated here if None)."""
+    ``tile_ranges``: an int32 [T, 2] array to fill (allocated here if None);
+    ``pair_ids`` likewise (int32, ``total_pairs`` entries).
+    ``capped`` (with ``kept``): ``total_pairs`` is a capacity and the kernels
+    read the total from ``kept.total`` on the device (gstv_keep_bin_fill_capped):
+    nothing waits for the host; ``pair_ids`` has ``total_pairs`` entries, and a
+    total above the capacity leaves every tile list empty."""
     lib = get_lib()
     dev = center_vox.device
     c = center_vox.detach().contiguous().float()
@@ -179,9 +189,14 @@ def bin_fill(center_vox, radius_vox, shape, tile_size, offsets, total_pairs: int
     num_tiles = num_tiles_for(shape, tile_size)
     if total_pairs >= 2**31 - 1:
         _check(3, "gstv_bin_fill")
+    if capped and kept is None:
+        raise ValueError("capped binning reads the total that keep_prepare / keep_prologue counted: pass kept")
     if tile_ranges is None:
         tile_ranges = torch.empty(num_tiles, 2, dtype=torch.int32, device=dev)
-    pair_ids = torch.empty(total_pairs, dtype=torch.int32, device=dev)
+    if pair_ids is None:
+        pair_ids = torch.empty(total_pairs, dtype=torch.int32, device=dev)
+    elif pair_ids.dtype != torch.int32 or pair_ids.numel() != total_pairs or not pair_ids.is_contiguous():
+        raise ValueError("pair_ids must be a contiguous int32 array of total_pairs entries")
     ws_bytes = lib.gstv_bin_fill_workspace_bytes(n, total_pairs, num_tiles)
     ws = _workspace(ws_bytes, dev)
     d, h, w = shape
@@ -189,11 +204,16 @@ def bin_fill(center_vox, radius_vox, shape, tile_size, offsets, total_pairs: int
         if kept is None:
             st = lib.gstv_bin_fill(_ptr(c), _ptr(r), n, d, h, w, tile_size, _ptr(offsets), total_pairs,
                                    _ptr(tile_ranges), _ptr(pair_ids), _ptr(ws), ws_bytes, _stream(dev))
+        elif capped:
+            st = lib.gstv_keep_bin_fill_capped(_ptr(kept.ws), kept.ws.numel(), _ptr(c), _ptr(r), n, d, h, w,
+                                               tile_size, _ptr(offsets), _ptr(kept.total), total_pairs,
+                                               _ptr(tile_ranges), _ptr(pair_ids), _ptr(ws), ws_bytes, _stream(dev))
         else:
             st = lib.gstv_keep_bin_fill(_ptr(kept.ws), kept.ws.numel(), _ptr(c), _ptr(r), n, d, h, w, tile_size,
                                         _ptr(offsets), total_pairs, _ptr(tile_ranges), _ptr(pair_ids), _ptr(ws),
                                         ws_bytes, _stream(dev))
-    _check(st, "gstv_bin_fill" if kept is None else "gstv_keep_bin_fill")
+    name = "gstv_bin_fill" if kept is None else ("gstv_keep_bin_fill_capped" if capped else "gstv_keep_bin_fill")
+    _check(st, name)
     return tile_ranges, pair_ids
 
 
@@ -216,6 +236,40 @@ def _rho_stride(rho_r, rho_i, keep):
     if st < 1:
         raise ValueError("density stride must be positive")
     return st
+
+
+class PairCapacity:
+    """Size of the pair arrays a voxelizer bins into without reading the pair
+    total back first (``bin_fill(..., capped=True)``).
+
+    Unknown until a first exact call reports a total; then ``total * headroom
+    + slack`` pairs, scaled with the Gaussian count when that grows.  It grows
+    when a total exceeds it (that call is redone with the exact size) and
+    shrinks when totals fall below half of it."""
+
+    def __init__(self, headroom: float = 1.25, slack: int = 1 << 16):
+        self.headroom = headroom
+        self.slack = slack
+        self.cap = None
+        self.n_ref = 0
+        self.overflows = 0
+
+    def capacity(self, n: int):
+        """Capacity for a call on n Gaussians, or None (no total seen yet)."""
+        if self.cap is None or n <= 0:
+            return None
+        scale = max(1.0, n / max(self.n_ref, 1))
+        return min(int(math.ceil(self.cap * scale)), 2**31 - 2)
+
+    def update(self, n: int, total: int, used=None):
+        """Records the total of a call on n Gaussians (binned with capacity
+        ``used``, or None for an exact call)."""
+        if used is not None and total > used:
+            self.overflows += 1
+        want = int(math.ceil(total * self.headroom)) + self.slack
+        if self.cap is None or (used is not None and total > used) or total > self.cap or want < self.cap // 2:
+            self.cap = want
+            self.n_ref = n
 
 
 class Kept:
@@ -266,9 +320,10 @@ def keep_prepare(center_vox, cov_inv, rho_r, rho_i, half_shape, radius_vox=None,
 
 def keep_prologue(positions, scales, rotations, rho_r, rho_i, shape, tile_size):
     """The fused call's first pass (gstv_keep_prologue): prologue, Gaussian
-    records and the extent-box pair counts.  rho_r / rho_i as in
-    ``keep_prepare``.  Returns (center_vox, radius_vox, cov_inv, half_shape,
-    kept), ``kept.offsets`` / ``kept.total`` set."""
+    records, the extent-box pair counts and a zeroed moment accumulator.
+    rho_r / rho_i as in ``keep_prepare``.  Returns (center_vox, radius_vox,
+    cov_inv, half_shape, kept), ``kept.offsets`` / ``kept.total`` set and
+    ``kept.fresh`` true."""
     lib = get_lib()
     _require_gpu(positions, "positions")
     dev = positions.device
@@ -290,6 +345,7 @@ def keep_prologue(positions, scales, rotations, rho_r, rho_i, shape, tile_size):
                                     tile_size, _ptr(center), _ptr(radius), _ptr(cov), _ptr(hs), _ptr(kept.offsets),
                                     _ptr(kept.total), _ptr(kept.ws), kept.ws.numel(), _stream(dev))
     _check(st, "gstv_keep_prologue")
+    kept.fresh = True
     return center, radius, cov, hs, kept
 
 

@@ -337,6 +337,22 @@ __device__ __forceinline__ void gauss_tile_box(const float* __restrict__ center,
   }
 }
 
+// The pair total the binning kernels work to.  With dev null it is cap (the
+// host knows the total).  Otherwise the kernels read it from device memory and
+// cap is the capacity the caller sized the grids and arrays for: a total above
+// it (or negative) reads as 0, so every kernel writes in bounds and the tile
+// lists come out empty; the caller compares the total with cap afterwards.
+struct PairTotal {
+  const int64_t* dev;
+  int cap;
+};
+
+__device__ __forceinline__ int pair_total(const PairTotal& t) {
+  if (t.dev == nullptr) return t.cap;
+  const int64_t v = *t.dev;
+  return (v >= 0 && v <= (int64_t)t.cap) ? (int)v : 0;
+}
+
 // Pairs per Gaussian; counts[n] = 0 so that the scan of n + 1 entries leaves
 // the total in its last entry.
 __global__ __launch_bounds__(256) void k_bin_count(const float* __restrict__ center,
@@ -367,13 +383,15 @@ __device__ __forceinline__ int last_le(const int32_t* a, int len, int32_t x) {
 // (tile id) and values (Gaussian id) coalesced.
 __global__ __launch_bounds__(kExpandThreads) void k_bin_expand(
     const float* __restrict__ center, const float* __restrict__ radius, int n, BinGeom geo,
-    const int32_t* __restrict__ offsets, int32_t total, int32_t* __restrict__ keys,
+    const int32_t* __restrict__ offsets, PairTotal pt, int32_t* __restrict__ keys,
     int32_t* __restrict__ vals) {
   __shared__ int32_t sh_off[kExpandTile + 1];
   __shared__ int32_t sh_t0[kExpandTile], sh_t1[kExpandTile], sh_t2[kExpandTile];
   __shared__ int32_t sh_n1[kExpandTile], sh_n2[kExpandTile];
   __shared__ int sh_g0, sh_ng;
+  const int total = pair_total(pt);
   const int j0 = blockIdx.x * kExpandTile;
+  if (j0 >= total) return;  // the grid covers the capacity
   const int j1 = min(j0 + kExpandTile, total);
   if (threadIdx.x == 0) {
     const int g_first = last_le(offsets, n + 1, j0);
@@ -470,9 +488,11 @@ struct ScatterLds {
 // Owner Gaussian of the first slot of every kRadixTile-slot block
 // (first[nblk] = owner of the last slot), written by the Gaussians themselves:
 // no per-block search.
-__global__ void k_block_first(const int32_t* __restrict__ offsets, int n, int total, int32_t* __restrict__ first) {
+__global__ void k_block_first(const int32_t* __restrict__ offsets, int n, PairTotal pt,
+                              int32_t* __restrict__ first) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n) return;
+  const int total = pair_total(pt);
+  if (g >= n || total == 0) return;  // total 0: no block reads first
   const int lo = offsets[g], hi = offsets[g + 1];
   if (lo >= hi) return;
   for (int b = (lo + kRadixTile - 1) / kRadixTile; b * (int64_t)kRadixTile < hi; ++b) first[b] = g;
@@ -621,14 +641,20 @@ struct PairSource {
 // digit-major (hist[digit * nblocks + block]) so one exclusive scan gives
 // every block's scatter base per digit.
 template <bool kGen>
-__global__ __launch_bounds__(kRadixThreads, 4) void k_radix_hist(PairSource src, int total, int shift, int bits,
+__global__ __launch_bounds__(kRadixThreads, 4) void k_radix_hist(PairSource src, PairTotal pt, int shift, int bits,
                                                              int nblocks, int32_t* __restrict__ hist) {
   struct NoGen {};
   __shared__ int32_t sh_h[1 << kRadixBits];
   __shared__ std::conditional_t<kGen, GenLds, NoGen> gen_lds;
-  sh_h[threadIdx.x] = 0;
+  const int total = pair_total(pt);
   const int mask = (1 << bits) - 1;
   const int base = blockIdx.x * kRadixTile;
+  if (base >= total) {
+    // A block past the total (the grid covers the capacity): empty histogram.
+    if (threadIdx.x < (1 << bits)) hist[threadIdx.x * nblocks + blockIdx.x] = 0;
+    return;
+  }
+  sh_h[threadIdx.x] = 0;
   bool gen_ok = false;
   if constexpr (kGen)
     gen_ok = gen_setup(gen_lds, src.center, src.radius, src.geo, src.offsets, src.first, base,
@@ -686,7 +712,8 @@ __global__ __launch_bounds__(kRadixThreads, 4) void k_radix_hist(PairSource src,
 // sub-run of that tile's global range, merged with atomicMin / atomicMax
 // into ranges pre-set to (INT_MAX, 0).
 template <bool kGen, bool kLast>
-__device__ __forceinline__ void radix_scatter(const PairSource& src, int total, int shift, int bits, int nblocks,
+__device__ __forceinline__ void radix_scatter(const PairSource& src, const PairTotal& pt, int shift, int bits,
+                                              int nblocks,
                                               const int32_t* __restrict__ hist_scanned,
                                               int32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out,
                                               int32_t* __restrict__ ranges) {
@@ -701,6 +728,8 @@ __device__ __forceinline__ void radix_scatter(const PairSource& src, int total, 
     ScatterLds sc;
   };
   __shared__ std::conditional_t<kGen, GenScatterLds, ScatterOnly> L;
+  const int total = pair_total(pt);
+  if ((int)blockIdx.x * kRadixTile >= total) return;  // past the total: nothing to move
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
   const int ndig = 1 << bits;
@@ -799,24 +828,24 @@ __device__ __forceinline__ void radix_scatter(const PairSource& src, int total, 
 
 // First pass (pairs generated in the kernel; under 40 KB of LDS, 4 blocks per CU).
 template <bool kLast>
-__global__ __launch_bounds__(kRadixThreads, 4) void k_radix_scatter_gen(PairSource src, int total, int shift, int bits,
+__global__ __launch_bounds__(kRadixThreads, 4) void k_radix_scatter_gen(PairSource src, PairTotal pt, int shift, int bits,
                                                                     int nblocks,
                                                                     const int32_t* __restrict__ hist_scanned,
                                                                     int32_t* __restrict__ keys_out,
                                                                     int32_t* __restrict__ vals_out,
                                                                     int32_t* __restrict__ ranges) {
-  radix_scatter<true, kLast>(src, total, shift, bits, nblocks, hist_scanned, keys_out, vals_out, ranges);
+  radix_scatter<true, kLast>(src, pt, shift, bits, nblocks, hist_scanned, keys_out, vals_out, ranges);
 }
 
 // Later passes (pairs read from the previous pass; 4 blocks per CU).
 template <bool kLast>
-__global__ __launch_bounds__(kRadixThreads, 4) void k_radix_scatter(PairSource src, int total, int shift, int bits,
+__global__ __launch_bounds__(kRadixThreads, 4) void k_radix_scatter(PairSource src, PairTotal pt, int shift, int bits,
                                                                     int nblocks,
                                                                     const int32_t* __restrict__ hist_scanned,
                                                                     int32_t* __restrict__ keys_out,
                                                                     int32_t* __restrict__ vals_out,
                                                                     int32_t* __restrict__ ranges) {
-  radix_scatter<false, kLast>(src, total, shift, bits, nblocks, hist_scanned, keys_out, vals_out, ranges);
+  radix_scatter<false, kLast>(src, pt, shift, bits, nblocks, hist_scanned, keys_out, vals_out, ranges);
 }
 
 // tile_ranges of the fused final pass: preset (INT_MAX, 0), then tiles with
@@ -835,8 +864,9 @@ __global__ void k_ranges_fix(int32_t* __restrict__ ranges, int num_tiles) {
 }
 
 // tile_ranges from sorted keys; gaussian/tile_voxelizer.py:151-161.
-__global__ void k_ranges(const int32_t* __restrict__ keys, int total, int32_t* __restrict__ ranges) {
+__global__ void k_ranges(const int32_t* __restrict__ keys, PairTotal pt, int32_t* __restrict__ ranges) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = pair_total(pt);
   if (j >= total) return;
   const int t = keys[j];
   if (j == 0 || keys[j - 1] != t) ranges[2 * t] = j;
@@ -2116,7 +2146,8 @@ __global__ __launch_bounds__(256) void k_prologue(const float* __restrict__ pos,
 
 // k_prologue and k_prep in one pass (the fused call): centre, radius and
 // cov_inv written as k_prologue writes them, and the record (and pair count)
-// built from the same fp32 values as k_prep builds it from memory.
+// built from the same fp32 values as k_prep builds it from memory.  With gacc,
+// each Gaussian's moment row is zeroed too (the adjoint accumulates into it).
 __global__ __launch_bounds__(256) void k_prologue_prep(const float* __restrict__ pos,
                                                        const float* __restrict__ scales,
                                                        const float* __restrict__ rot, int n, float sh0, float sh1,
@@ -2126,7 +2157,7 @@ __global__ __launch_bounds__(256) void k_prologue_prep(const float* __restrict__
                                                        const float* __restrict__ rho_r,
                                                        const float* __restrict__ rho_i, int rho_stride,
                                                        GRec* __restrict__ rec, ZRec* __restrict__ zt, BinGeom geo,
-                                                       int32_t* __restrict__ counts) {
+                                                       int32_t* __restrict__ counts, float* __restrict__ gacc) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g == 0) {
     half_shape[0] = sh0 * 0.5f;
@@ -2135,6 +2166,11 @@ __global__ __launch_bounds__(256) void k_prologue_prep(const float* __restrict__
   }
   if (counts != nullptr && g == n) counts[n] = 0;
   if (g >= n) return;
+  if (gacc != nullptr) {
+    float4* row = reinterpret_cast<float4*>(gacc + (int64_t)g * kAccStride);
+#pragma unroll
+    for (int k = 0; k < kAccStride / 4; ++k) row[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
   float c[3], M[9], rad;
   prologue_one(g, pos, scales, rot, sh0, sh1, sh2, center, radius, cov, c, M, rad);
   prep_one(g, c, M, sh0 * 0.5f, sh1 * 0.5f, sh2 * 0.5f, rho_r, rho_i, rho_stride, rec, zt, rad, geo, counts);
@@ -2576,10 +2612,12 @@ size_t gstv_bin_fill_workspace_bytes(int32_t n, int64_t total_pairs, int32_t num
 }
 
 namespace {
+// total_dev null: total_pairs is the pair total.  Otherwise total_pairs is a
+// capacity and the kernels read the total from *total_dev (see PairTotal).
 int bin_fill_impl(const GRec* rec, const float* center_vox, const float* radius_vox, int32_t n, int32_t d,
                   int32_t h, int32_t w, int32_t tile, const int32_t* gauss_offsets, int64_t total_pairs,
-                  int32_t* tile_ranges, int32_t* pair_ids, void* workspace, size_t workspace_bytes,
-                  void* stream) {
+                  const int64_t* total_dev, int32_t* tile_ranges, int32_t* pair_ids, void* workspace,
+                  size_t workspace_bytes, void* stream) {
   BinGeom geo;
   geo.rec = rec;
   int64_t num_tiles64;
@@ -2594,6 +2632,7 @@ int bin_fill_impl(const GRec* rec, const float* center_vox, const float* radius_
   geo.tile_shift = tile_shift_of(tile);
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int total = (int)total_pairs;
+  const PairTotal pt{total_dev, total};
   const int passes = radix_passes(num_tiles);
   if (total == 0 || passes == 0) {
     hipLaunchKernelGGL(k_fill_i32, dim3(ceil_div_i(2 * num_tiles, 256)), dim3(256), 0, s, tile_ranges,
@@ -2614,14 +2653,14 @@ int bin_fill_impl(const GRec* rec, const float* center_vox, const float* radius_
   if (passes == 0) {
     // A single tile: the expansion order is final.
     hipLaunchKernelGGL(k_bin_expand, dim3(ceil_div_i(total, kExpandTile)), dim3(kExpandThreads), 0, s, center_vox,
-                       radius_vox, n, geo, gauss_offsets, total, ka, pair_ids);
+                       radius_vox, n, geo, gauss_offsets, pt, ka, pair_ids);
     GSTV_LAUNCH_CHECK("bin_expand");
-    hipLaunchKernelGGL(k_ranges, dim3(ceil_div_i(total, 256)), dim3(256), 0, s, ka, total, tile_ranges);
+    hipLaunchKernelGGL(k_ranges, dim3(ceil_div_i(total, 256)), dim3(256), 0, s, ka, pt, tile_ranges);
     GSTV_LAUNCH_CHECK("ranges");
     return GSTV_OK;
   }
 
-  hipLaunchKernelGGL(k_block_first, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, gauss_offsets, n, total, first);
+  hipLaunchKernelGGL(k_block_first, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, gauss_offsets, n, pt, first);
   GSTV_LAUNCH_CHECK("block_first");
   const int bits_total = key_bits(num_tiles);
   const int width = (bits_total + passes - 1) / passes;  // balanced digit widths
@@ -2646,10 +2685,10 @@ int bin_fill_impl(const GRec* rec, const float* center_vox, const float* radius_
     int32_t* kout = buf_k[dst];
     int32_t* vout = last ? pair_ids : buf_v[dst];
     if (cur < 0)
-      hipLaunchKernelGGL(k_radix_hist<true>, dim3(nblk), dim3(kRadixThreads), 0, s, src, total, shift, bits, nblk,
+      hipLaunchKernelGGL(k_radix_hist<true>, dim3(nblk), dim3(kRadixThreads), 0, s, src, pt, shift, bits, nblk,
                          hist);
     else
-      hipLaunchKernelGGL(k_radix_hist<false>, dim3(nblk), dim3(kRadixThreads), 0, s, src, total, shift, bits, nblk,
+      hipLaunchKernelGGL(k_radix_hist<false>, dim3(nblk), dim3(kRadixThreads), 0, s, src, pt, shift, bits, nblk,
                          hist);
     GSTV_LAUNCH_CHECK("radix_hist");
     if (p == 0) {
@@ -2663,16 +2702,16 @@ int bin_fill_impl(const GRec* rec, const float* center_vox, const float* radius_
     int st = launch_exclusive_scan(hist, hist_n, hist, sums, nullptr, s);
     if (st != GSTV_OK) return st;
     if (cur < 0 && last)
-      hipLaunchKernelGGL((k_radix_scatter_gen<true>), dim3(nblk), dim3(kRadixThreads), 0, s, src, total, shift,
+      hipLaunchKernelGGL((k_radix_scatter_gen<true>), dim3(nblk), dim3(kRadixThreads), 0, s, src, pt, shift,
                          bits, nblk, hist, kout, vout, tile_ranges);
     else if (cur < 0)
-      hipLaunchKernelGGL((k_radix_scatter_gen<false>), dim3(nblk), dim3(kRadixThreads), 0, s, src, total, shift,
+      hipLaunchKernelGGL((k_radix_scatter_gen<false>), dim3(nblk), dim3(kRadixThreads), 0, s, src, pt, shift,
                          bits, nblk, hist, kout, vout, tile_ranges);
     else if (last)
-      hipLaunchKernelGGL((k_radix_scatter<true>), dim3(nblk), dim3(kRadixThreads), 0, s, src, total, shift,
+      hipLaunchKernelGGL((k_radix_scatter<true>), dim3(nblk), dim3(kRadixThreads), 0, s, src, pt, shift,
                          bits, nblk, hist, kout, vout, tile_ranges);
     else
-      hipLaunchKernelGGL((k_radix_scatter<false>), dim3(nblk), dim3(kRadixThreads), 0, s, src, total, shift,
+      hipLaunchKernelGGL((k_radix_scatter<false>), dim3(nblk), dim3(kRadixThreads), 0, s, src, pt, shift,
                          bits, nblk, hist, kout, vout, tile_ranges);
     GSTV_LAUNCH_CHECK("radix_scatter");
     cur = dst;
@@ -2687,8 +2726,8 @@ int gstv_bin_fill(const float* center_vox, const float* radius_vox, int32_t n, i
                   int32_t w, int32_t tile, const int32_t* gauss_offsets, int64_t total_pairs,
                   int32_t* tile_ranges, int32_t* pair_ids, void* workspace, size_t workspace_bytes,
                   void* stream) {
-  return bin_fill_impl(nullptr, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, total_pairs, tile_ranges,
-                       pair_ids, workspace, workspace_bytes, stream);
+  return bin_fill_impl(nullptr, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, total_pairs, nullptr,
+                       tile_ranges, pair_ids, workspace, workspace_bytes, stream);
 }
 
 size_t gstv_forward_workspace_bytes(int32_t n) {
@@ -2938,7 +2977,7 @@ int gstv_keep_prologue(const float* positions, const float* scales, const float*
   keep_layout(keep_workspace, n, &ws);
   hipLaunchKernelGGL(k_prologue_prep, dim3(ceil_div_i(n + 1, 256)), dim3(256), 0, s, positions, scales, rotations, n,
                      (float)d, (float)h, (float)w, center_vox, radius_vox, cov_inv, half_shape, rho_r, rho_i,
-                     (int)rho_stride, ws.rec, ws.zt, geo, gauss_offsets);
+                     (int)rho_stride, ws.rec, ws.zt, geo, gauss_offsets, ws.gacc);
   GSTV_LAUNCH_CHECK("keep_prologue");
   return launch_exclusive_scan(gauss_offsets, (int64_t)n + 1, gauss_offsets, ws.sums, total_pairs, s);
 }
@@ -2964,8 +3003,23 @@ int gstv_keep_bin_fill(const void* keep_workspace, size_t keep_workspace_bytes, 
     return GSTV_ERR_WORKSPACE_TOO_SMALL;
   KeepLayout ws{nullptr, nullptr, nullptr, nullptr};
   if (n > 0) keep_layout(const_cast<void*>(keep_workspace), n, &ws);
-  return bin_fill_impl(ws.rec, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, total_pairs, tile_ranges,
-                       pair_ids, workspace, workspace_bytes, stream);
+  return bin_fill_impl(ws.rec, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, total_pairs, nullptr,
+                       tile_ranges, pair_ids, workspace, workspace_bytes, stream);
+}
+
+int gstv_keep_bin_fill_capped(const void* keep_workspace, size_t keep_workspace_bytes, const float* center_vox,
+                              const float* radius_vox, int32_t n, int32_t d, int32_t h, int32_t w, int32_t tile,
+                              const int32_t* gauss_offsets, const int64_t* total_pairs, int64_t capacity,
+                              int32_t* tile_ranges, int32_t* pair_ids, void* workspace, size_t workspace_bytes,
+                              void* stream) {
+  if (n < 0 || total_pairs == nullptr || capacity < 0) return GSTV_ERR_INVALID_ARGUMENT;
+  if (n == 0) capacity = 0;  // no Gaussian, no pair: empty ranges
+  if (n > 0 && (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n)))
+    return GSTV_ERR_WORKSPACE_TOO_SMALL;
+  KeepLayout ws{nullptr, nullptr, nullptr, nullptr};
+  if (n > 0) keep_layout(const_cast<void*>(keep_workspace), n, &ws);
+  return bin_fill_impl(ws.rec, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, capacity, total_pairs,
+                       tile_ranges, pair_ids, workspace, workspace_bytes, stream);
 }
 
 int gstv_forward_keep(const void* keep_workspace, size_t keep_workspace_bytes, const int32_t* tile_ranges,

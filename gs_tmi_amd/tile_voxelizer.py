@@ -110,7 +110,7 @@ class _TileVoxelizeFusedHIP(torch.autograd.Function):
     kernels fewer per step."""
 
     @staticmethod
-    def forward(ctx, positions, scales, rotations, density, volume_shape, tile_size):
+    def forward(ctx, positions, scales, rotations, density, volume_shape, tile_size, pairs=None):
         _load_hip_ext(strict=True)
         d_size, h_size, w_size = volume_shape
         scales_c = scales.detach().contiguous().float()
@@ -121,32 +121,52 @@ class _TileVoxelizeFusedHIP(torch.autograd.Function):
         if dens.dtype != torch.complex64:
             dens = dens.to(torch.complex64)
         dv = torch.view_as_real(dens.contiguous())
-        # One pass makes the prologue's outputs and the Gaussian records, and
-        # counts the pairs whose tile meets a Gaussian's extent box: binning
-        # keeps only the pairs the tile kernels evaluate (the reference's lists
-        # hold about twice as many, which the kernels would skip).  Binning
-        # reads the pair total back to the host once; the adjoint's
-        # accumulator is cleared behind that read, while the host waits and
-        # then sizes the pair arrays.
+        # One pass makes the prologue's outputs and the Gaussian records,
+        # zeroes the adjoint's accumulator, and counts the pairs whose tile
+        # meets a Gaussian's extent box: binning keeps only the pairs the tile
+        # kernels evaluate (the reference's lists hold about twice as many,
+        # which the kernels would skip).
         center, radius, cov, half_shape, kept = _native.keep_prologue(positions, scales_c, rot_c, dv[:, 0],
                                                                       dv[:, 1], volume_shape, tile_size)
+        n = center.shape[0]
         stream = torch.cuda.current_stream(center.device)
         total_host = torch.empty(1, dtype=torch.int64, pin_memory=True)
         with torch.cuda.stream(stream):
             total_host.copy_(kept.total, non_blocking=True)
         total_ready = torch.cuda.Event()
         total_ready.record(stream)
-        _native.keep_clear(kept)
-        tile_ranges = torch.empty(_native.num_tiles_for(volume_shape, tile_size), 2, dtype=torch.int32,
-                                  device=center.device)
-        total_np = total_host.numpy()
-        total_ready.synchronize()
-        tile_ranges, pair_ids = _native.bin_fill(center, radius, volume_shape, tile_size, kept.offsets,
-                                                 int(total_np[0]), kept=kept, tile_ranges=tile_ranges)
+        cap = pairs.capacity(n) if pairs is not None else None
+
+        def exact():
+            total_ready.synchronize()
+            total = int(total_host.numpy()[0])
+            ranges, pids = _native.bin_fill(center, radius, volume_shape, tile_size, kept.offsets, total, kept=kept)
+            return total, ranges, pids
+
+        if cap is None:
+            # No capacity known: the reference's order (:118), the host reads
+            # the pair total and then sizes the pair arrays.
+            total, tile_ranges, pair_ids = exact()
+        else:
+            # Binning into the known capacity, from the total in device memory:
+            # the host does not wait before queueing the forward.
+            tile_ranges, pair_ids = _native.bin_fill(center, radius, volume_shape, tile_size, kept.offsets, cap,
+                                                     kept=kept, capped=True)
         # The forward keeps its records and per-pair column rows for the
         # adjoint of this call, which then rebuilds neither.
         vol_r, vol_i, ctx.kept = _native.forward(center, cov, dv[:, 0], dv[:, 1], tile_ranges, pair_ids,
                                                  half_shape, tile_size, d_size, h_size, w_size, keep=kept)
+        if cap is not None:
+            # The total is checked while the GPU runs the forward; above the
+            # capacity the lists were empty, and the call is redone exactly.
+            total_ready.synchronize()
+            total = int(total_host.numpy()[0])
+            if total > cap:
+                total, tile_ranges, pair_ids = exact()
+                vol_r, vol_i, ctx.kept = _native.forward(center, cov, dv[:, 0], dv[:, 1], tile_ranges, pair_ids,
+                                                         half_shape, tile_size, d_size, h_size, w_size, keep=kept)
+        if pairs is not None:
+            pairs.update(n, total, cap)
         ctx.save_for_backward(scales_c, rot_c, center, cov, dv, tile_ranges, pair_ids, half_shape)
         ctx.density_dtype = density.dtype
         ctx.density_shape = density.shape
@@ -173,7 +193,7 @@ class _TileVoxelizeFusedHIP(torch.autograd.Function):
             g_d = torch.view_as_complex(g_dv).to(ctx.density_dtype)
         else:
             g_d = g_dv[:, 0].to(ctx.density_dtype)
-        return g_p, g_s, g_q, g_d.reshape(ctx.density_shape), None, None
+        return g_p, g_s, g_q, g_d.reshape(ctx.density_shape), None, None, None
 
 
 def voxel_space_params(positions, scales, rotations, volume_shape):
@@ -219,6 +239,9 @@ class TileVoxelizer(nn.Module):
             _load_hip_ext(strict=True)
         self.use_cuda = True
         self.backend = "tile_hip"
+        # Pair-array capacity carried from call to call (binning then never
+        # waits for the host; see _native.PairCapacity).
+        self.pairs = _native.PairCapacity()
         print(f"[TileVoxelizer] volume={self.volume_shape}, tile_size={self.tile_size}, "
               f"max_radius={max_radius}, backend={self.backend}")
 
@@ -226,7 +249,7 @@ class TileVoxelizer(nn.Module):
                 density: torch.Tensor, chunk_size: int = 4096):
         del chunk_size
         return _TileVoxelizeFusedHIP.apply(positions, scales, rotations, density, self.volume_shape,
-                                           self.tile_size)
+                                           self.tile_size, self.pairs)
 
     def forward_unfused(self, positions, scales, rotations, density):
         """The same call composed as the reference composes it: torch prologue,

@@ -191,9 +191,23 @@ int gstv_backward_noalloc(const float* center_vox, const float* cov_inv,
  *                       the forward's fp32 sums differ only in order where a
  *                       tile's list spans more than one batch of 512 pairs.
  *   gstv_keep_prologue  gstv_prologue and gstv_keep_prepare (with the counts)
- *                       in one pass; the same outputs.
+ *                       in one pass; the same outputs.  It also zeroes the
+ *                       adjoint's moment accumulator (gstv_keep_clear).
  *   gstv_keep_bin_fill  gstv_bin_fill for those counts (same order within each
  *                       tile: ascending Gaussian id).
+ *   gstv_keep_bin_fill_capped
+ *                       gstv_keep_bin_fill without the host knowing the total:
+ *                       the kernels read it from total_pairs (device memory,
+ *                       as gstv_keep_prepare / gstv_keep_prologue wrote it),
+ *                       and `capacity` sizes pair_ids (capacity entries) and
+ *                       the workspace (gstv_bin_fill_workspace_bytes(n,
+ *                       capacity, num_tiles)).  The result equals
+ *                       gstv_keep_bin_fill's when total <= capacity; a larger
+ *                       total gives empty tile lists (every tile (0, 0)), and
+ *                       the caller, who reads the total back once the work
+ *                       behind it is queued, redoes the call with an exact
+ *                       size.  Replaces the blocking read of the total at
+ *                       gaussian/tile_voxelizer.py:118.
  *   gstv_keep_clear     zeroes the adjoint's moment accumulator.  Neither it
  *                       nor the records need binning, so a caller can queue
  *                       them before reading the pair total back to the host.
@@ -245,6 +259,13 @@ int gstv_keep_bin_fill(const void* keep_workspace, size_t keep_workspace_bytes,
                        const int32_t* gauss_offsets, int64_t total_pairs,
                        int32_t* tile_ranges, int32_t* pair_ids,
                        void* workspace, size_t workspace_bytes, void* stream);
+
+int gstv_keep_bin_fill_capped(const void* keep_workspace, size_t keep_workspace_bytes,
+                              const float* center_vox, const float* radius_vox, int32_t n,
+                              int32_t d, int32_t h, int32_t w, int32_t tile,
+                              const int32_t* gauss_offsets, const int64_t* total_pairs, int64_t capacity,
+                              int32_t* tile_ranges, int32_t* pair_ids,
+                              void* workspace, size_t workspace_bytes, void* stream);
 
 int gstv_forward_keep(const void* keep_workspace, size_t keep_workspace_bytes,
                       const int32_t* tile_ranges, int32_t num_tiles, const int32_t* pair_ids,

@@ -296,6 +296,47 @@ def test_fused_call_backward_twice():
         assert e < 5e-6, (nm, e)
 
 
+def test_fused_call_pair_capacity_and_overflow():
+    """The fused call bins into the pair capacity its voxelizer carries from
+    earlier calls (the total stays on the device).  The first call (no
+    capacity yet) reads the total as the reference does; later calls give the
+    same volume bit for bit; a call whose total exceeds the capacity is redone
+    with the exact size (same volume, same gradients up to the order of fp32
+    atomics) and the capacity grows."""
+    from gs_tmi_amd import TileVoxelizer
+
+    shape = (64, 64, 64)
+    _announce(f"fused call pair capacity and overflow shape={shape}")
+    c = cases.microbench_case(n=20_000, shape=shape, seed=8)
+    t = lambda a: torch.from_numpy(a).to(DEV)  # noqa: E731
+    vox = TileVoxelizer(shape, tile_size=8, device=DEV)
+    gen = torch.Generator(device="cpu").manual_seed(9)
+    g_r = torch.randn(shape, generator=gen).to(DEV)
+    g_i = torch.randn(shape, generator=gen).to(DEV)
+
+    def call():
+        leaves = [t(a).requires_grad_() for a in (c["pos"], np.log(c["scales"]), c["rot"], c["rr"], c["ri"])]
+        pos, log_s, rot, dr, di = leaves
+        vr, vi = vox(pos, torch.exp(log_s), rot, torch.complex(dr, di))
+        grads = torch.autograd.grad((vr * g_r).sum() + (vi * g_i).sum(), leaves)
+        return vr.detach(), vi.detach(), [g.detach() for g in grads]
+
+    assert vox.pairs.capacity(20_000) is None
+    r0 = call()
+    cap = vox.pairs.capacity(20_000)
+    assert cap is not None and vox.pairs.overflows == 0
+    r1 = call()  # capped
+    assert torch.equal(r0[0], r1[0]) and torch.equal(r0[1], r1[1])
+    vox.pairs.cap = 100  # far below the total: this call overflows and is redone
+    r2 = call()
+    assert vox.pairs.overflows == 1 and vox.pairs.capacity(20_000) >= cap - vox.pairs.slack
+    assert torch.equal(r0[0], r2[0]) and torch.equal(r0[1], r2[1])
+    for nm, a, b in zip(["pos", "log_s", "rot", "dr", "di"], r1[2], r2[2]):
+        e = cases.rel_l2(b.cpu().numpy(), a.cpu().numpy())
+        print(f"  overflow redo {nm}: rel L2 {e:.2e}")
+        assert e < 5e-6, (nm, e)
+
+
 def test_full_size_fused_call_matches_reference_lists():
     """Voxelizer microbenchmark size: the fused call (extent-box tile lists)
     gives the volume of the forward kernel on the reference's tile lists with

@@ -256,6 +256,39 @@ def test_keep_binning_is_the_touching_pairs(name, make):
 
 
 @pytest.mark.parametrize("name,make", ALL_CASES, ids=[n for n, _ in ALL_CASES])
+def test_capped_binning_equals_exact(name, make):
+    """gstv_keep_bin_fill_capped (total read on the device) gives
+    gstv_keep_bin_fill's lists whenever the capacity holds the total, for an
+    exact fit and with room to spare; a capacity one short gives empty lists
+    (every tile (0, 0)) and leaves pair_ids unwritten."""
+    from gs_tmi_amd import _native
+
+    c = make()
+    tile = c["tile"]
+    _announce(f"capped binning {name} shape={c['shape']} tile={tile} n={len(c['rr'])}")
+    center, radius, cov, hs = _prologue_gpu(c)
+    rr = torch.from_numpy(c["rr"]).to(DEV)
+    ri = torch.from_numpy(c["ri"]).to(DEV)
+    kept = _native.keep_prepare(center, cov, rr, ri, hs, radius, c["shape"], tile)
+    total = int(kept.total.item())
+    e_ranges, e_pids = _native.bin_fill(center, radius, c["shape"], tile, kept.offsets, total, kept=kept)
+    e_ranges, e_pids = e_ranges.cpu().numpy(), e_pids.cpu().numpy()
+    for cap in (total, total + 1000, 3 * total + 70_000):
+        ranges, pids = _native.bin_fill(center, radius, c["shape"], tile, kept.offsets, cap, kept=kept, capped=True)
+        assert pids.numel() == cap
+        np.testing.assert_array_equal(ranges.cpu().numpy(), e_ranges)
+        np.testing.assert_array_equal(pids[:total].cpu().numpy(), e_pids)
+    if total > 0:
+        ranges, pids = _native.bin_fill(center, radius, c["shape"], tile, kept.offsets, total - 1, kept=kept,
+                                        capped=True)
+        pids.fill_(-7)
+        ranges2, _ = _native.bin_fill(center, radius, c["shape"], tile, kept.offsets, total - 1, kept=kept,
+                                      capped=True, pair_ids=pids)
+        assert not ranges.cpu().numpy().any() and not ranges2.cpu().numpy().any()
+        assert bool((pids == -7).all())
+
+
+@pytest.mark.parametrize("name,make", ALL_CASES, ids=[n for n, _ in ALL_CASES])
 def test_keep_prologue_matches_separate_passes(name, make):
     """gstv_keep_prologue (one pass) gives gstv_prologue's outputs and
     gstv_keep_prepare's records and pair counts bit for bit."""
