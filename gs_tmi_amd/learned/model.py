@@ -7,7 +7,7 @@ from __future__ import annotations
 import torch
 import torch.nn as nn
 
-from .mfma import MfmaLinear
+from .mfma import MfmaLinear, relu_net
 
 PATCH = 8
 SLOTS = 32
@@ -22,8 +22,7 @@ class InitMLP(nn.Module):
         self.layers = nn.ModuleList(MfmaLinear(a, b, generator=g) for a, b in zip(dims[:-1], dims[1:]))
 
     def forward(self, x):
-        for i, layer in enumerate(self.layers):
-            x = layer(x)
-            if i < len(self.layers) - 1:
-                x = torch.relu(x)
+        # Bias and ReLU in the GEMM epilogues (relu_net); the same arithmetic
+        # as the layers composed with torch.relu.
+        x = relu_net(x, self.layers)
         return x.view(x.shape[0], SLOTS, SLOT_FIELDS)

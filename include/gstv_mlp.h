@@ -38,6 +38,23 @@ int gstv_mlp_gemm_nt_bf16(const void* a, int64_t lda, const void* b, int64_t ldb
                           const float* bias, int act, int m, int n, int k,
                           float* c, int64_t ldc, void* c_bf16, int64_t ldcb, void* stream);
 
+/* gstv_mlp_gemm_nt_bf16 with two more epilogue outputs / options:
+ *   relu_mask  bf16 m x n (row stride ldm) or NULL: every result whose mask
+ *              entry is not a positive number is zeroed (after bias and act):
+ *              the backward of a ReLU whose output is the mask;
+ *   c_bf16_t   bf16 n x m (row stride ldct >= m) or NULL: the bf16 result,
+ *              transposed (c_bf16_t[j * ldct + i] = bf16(C[i][j])), the
+ *              operand form the next weight-gradient product reads.
+ * At least one of c, c_bf16, c_bf16_t must be given. */
+int gstv_mlp_gemm_nt_bf16_ex(const void* a, int64_t lda, const void* b, int64_t ldb,
+                             const float* bias, int act, const void* relu_mask, int64_t ldm,
+                             int m, int n, int k, float* c, int64_t ldc,
+                             void* c_bf16, int64_t ldcb, void* c_bf16_t, int64_t ldct, void* stream);
+
+/* dst (rows x cols, may be NULL) = bf16(src) and dst_t (cols x rows) = its
+ * transpose, in one pass over the fp32 rows x cols matrix src (RNE). */
+int gstv_mlp_cast_t_bf16(const float* src, int rows, int cols, void* dst, void* dst_t, void* stream);
+
 /* dst (cols x rows) = transpose of src (rows x cols), bf16. */
 int gstv_mlp_transpose_bf16(const void* src, int rows, int cols, void* dst, void* stream);
 

@@ -161,6 +161,82 @@ def test_init_mlp_gradients_match_fp64_oracle():
         assert rel(l.bias.grad.cpu().double().numpy(), db) < 0.1
 
 
+def test_gemm_epilogue_mask_and_transposed_output():
+    """gstv_mlp_gemm_nt_bf16_ex: the masked result is the plain result where
+    the mask entry is a positive number and 0 elsewhere (negative, +-0, NaN
+    entries), its bf16 copy is the RNE rounding of the fp32 result, and the
+    transposed copy is exactly the transpose of the bf16 copy (ragged M for
+    the 4-row write path, both tile variants)."""
+    mf = _mfma()
+    for m, n, k in ((70, 50, 45), (2048, 512, 512), (131, 288, 512)):
+        print(f"[gemm ex] m={m} n={n} k={k}", flush=True)
+        g = torch.Generator(device="cpu").manual_seed(m + n + k)
+        a = mf.to_bf16(torch.randn(m, k, generator=g).to(DEV))
+        b = mf.to_bf16(torch.randn(n, k, generator=g).to(DEV))
+        bias = torch.randn(n, generator=g).to(DEV)
+        mask_src = torch.randn(m, n, generator=g)
+        mask_src[0, :4] = torch.tensor([0.0, -0.0, float("nan"), float("inf")])
+        mask = mask_src.to(torch.bfloat16).to(DEV)
+        plain = mf.gemm_nt(a, b, bias, act="relu")
+        c, cb, ct = mf.gemm_nt(a, b, bias, act="relu", want_bf16=True, want_bf16_t=True, relu_mask=mask)
+        torch.cuda.synchronize()
+        keep = (mask.float() > 0).cpu()
+        want = torch.where(keep, plain.cpu(), torch.zeros(()))
+        assert torch.equal(c.cpu(), want)
+        assert torch.equal(cb.cpu(), c.cpu().to(torch.bfloat16))
+        assert torch.equal(ct.cpu(), cb.cpu().t().contiguous())
+        only_t = mf.gemm_nt(a, b, bias, act="relu", want_fp32=False, want_bf16_t=True)
+        torch.cuda.synchronize()
+        assert torch.equal(only_t.cpu(), plain.cpu().to(torch.bfloat16).t().contiguous())
+
+
+def test_cast_with_transpose_exact():
+    mf = _mfma()
+    for rows, cols in ((1, 1), (33, 70), (2048, 288)):
+        x = torch.randn(rows, cols)
+        x[0, 0] = float("nan")
+        xb, xt = mf.cast_t(x.to(DEV))
+        _, xt2 = mf.cast_t(x.to(DEV), plain=False)
+        torch.cuda.synchronize()
+        ref = x.to(torch.bfloat16)
+        assert torch.equal(xb.cpu()[1:], ref[1:]) and torch.isnan(xb.cpu()[0, 0].float())
+        assert torch.equal(xt.view(torch.int16).cpu(), xb.view(torch.int16).cpu().t().contiguous())
+        assert torch.equal(xt2.view(torch.int16).cpu(), xt.view(torch.int16).cpu())
+
+
+def test_relu_net_equals_composed_layers():
+    """InitMLP's fused network (bias + ReLU in the GEMM epilogues, ReLU
+    masks in the dX epilogues, transposed operands from the producers) gives
+    the output and every gradient of MfmaLinear layers composed with
+    torch.relu, bit for bit, with and without autograd."""
+    from gs_tmi_amd.learned import InitMLP
+
+    model = InitMLP(seed=4).to(DEV)
+    g = torch.Generator(device="cpu").manual_seed(6)
+    x = torch.randn(300, 1024, generator=g).to(DEV).requires_grad_()
+    gy = torch.randn(300, 288, generator=g).to(DEV)
+    out = model(x).reshape(300, 288)
+    out.backward(gy)
+    fused = [x.grad.clone()] + [t.grad.clone() for l in model.layers for t in (l.weight, l.bias)]
+    for t in [x] + [t for l in model.layers for t in (l.weight, l.bias)]:
+        t.grad = None
+    h = x
+    for i, layer in enumerate(model.layers):
+        h = layer(h)
+        if i < len(model.layers) - 1:
+            h = torch.relu(h)
+    h.backward(gy)
+    composed = [x.grad] + [t.grad for l in model.layers for t in (l.weight, l.bias)]
+    assert torch.equal(out.detach(), h.detach())
+    for a, b in zip(fused, composed):
+        assert torch.equal(a, b)
+    with torch.no_grad():
+        assert torch.equal(model(x).reshape(300, 288), out.detach())
+        from gs_tmi_amd.learned.mfma import to_bf16
+        xb = to_bf16(x)
+        assert torch.equal(model(xb).reshape(300, 288), out.detach())
+
+
 def test_gemm_nt_k_split_invariance():
     """A K-contiguous product equals the sum of the products of its two K halves
     (sub-matrix views with the full leading dimension), to fp32 accumulation."""
@@ -185,13 +261,14 @@ def test_gemm_nt_k_split_invariance():
     assert np.all(diff <= bound), float(np.max(diff - bound))
 
 
-@pytest.mark.parametrize("m,n,k", [(64, 64, 64), (65, 63, 130), (256, 128, 200), (4096, 512, 1024),
-                                   (4100, 4097, 96)])
+@pytest.mark.parametrize("m,n,k", [(64, 64, 64), (65, 63, 130), (256, 128, 200), (2048, 288, 512),
+                                   (512, 1024, 2048), (4096, 512, 1024), (8192, 520, 72), (4100, 4097, 96)])
 def test_gemm_nt_tile_variants(m, n, k):
-    """The block-tile variants (64x64 below one 128x128 block per CU, 128x128
-    above, 256x256 from one 256x256 block per CU: the last case, with ragged
-    edges) against FP64 of the rounded operands, with bias, ReLU and the bf16
-    copy."""
+    """The block-tile variants against FP64 of the rounded operands, with bias,
+    ReLU and the bf16 copy: 32x32 tiles with 128-wide k stages (below one
+    64x64 block per CU; ragged K across a partial stage), 64x64 with 128-wide
+    stages (4096 x 512), 128x128 (8192 x 520, ragged N) and 256x256 (the last
+    case, ragged edges)."""
     mf = _mfma()
     print(f"[gemm variant] m={m} n={n} k={k}", flush=True)
     g = torch.Generator(device="cpu").manual_seed(m + n + k)

@@ -59,7 +59,8 @@ def test_mlp_library_exports_every_declared_symbol():
         pytest.skip("library not built (run `python -m gs_tmi_amd.build`)")
     lib = mfma.load_library()
     names = _declared_mlp()
-    assert {"gstv_mlp_gemm_nt_bf16", "gstv_mlp_transpose_bf16", "gstv_mlp_cast_bf16"} <= set(names)
+    assert {"gstv_mlp_gemm_nt_bf16", "gstv_mlp_gemm_nt_bf16_ex", "gstv_mlp_transpose_bf16", "gstv_mlp_cast_bf16",
+            "gstv_mlp_cast_t_bf16"} <= set(names)
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert set(names) == set(mfma.PROTOTYPES), "ctypes prototypes out of sync with the header"
