@@ -12,11 +12,20 @@ the voxel sigma per axis (3), complex amplitude over the image scale (2), and
 an existence logit (1).  Rotations are the identity, as in plain
 initialisation (gaussian/gaussian_model.py:54-56 of the reference).
 
-Targets come from slot-structured plain fits: one Gaussian is placed at every
-cell centre (``grid_gaussians``), fitted by the ordinary reconstruction loop
-with rotations held fixed and no densification, so fitted Gaussian i is the
-fit of slot i; a slot exists when its fitted amplitude is at least
-`exist_threshold` of the image scale.
+Targets come from fits of the ordinary reconstruction loop, in one of two
+forms:
+
+* plain fits (``plain_slot_targets``): the plain initialisation
+  (gaussian/gaussian_model.py:308-355 of the reference) fitted by the loop;
+  the fitted Gaussians whose centres fall in a cell are merged into one
+  axis-aligned Gaussian with the same complex mass, mass-weighted centre and
+  the diagonal of the mass-weighted second moment (moment matching);
+* slot-grid fits (``slot_targets``): one Gaussian placed at every cell centre
+  (``grid_gaussians``) and fitted with rotations held fixed and no
+  densification, so fitted Gaussian i is the fit of slot i.
+
+A slot exists when its amplitude is at least `exist_threshold` of the image
+scale.
 """
 
 from __future__ import annotations
@@ -130,6 +139,66 @@ def slot_targets(gaussians, shape, scale: float, exist_threshold: float = 0.03) 
     ai = gaussians.density_imag.detach() / scale
     exist = (torch.hypot(ar, ai) >= exist_threshold).float()
     tgt = torch.cat([off, log_sig, ar[:, None], ai[:, None], exist[:, None]], dim=1)
+    return tgt.view(-1, SLOTS, SLOT_FIELDS)
+
+
+@torch.no_grad()
+def plain_slot_targets(gaussians, shape, scale: float, exist_threshold: float = 0.03) -> torch.Tensor:
+    """[n_patches, 32, 9] targets from an unstructured fit (any number of
+    Gaussians, any rotations): every Gaussian goes to the cell holding its
+    centre, and each cell's Gaussians are merged by moment matching with
+    weights |m_i| (m_i = complex amplitude x Gaussian integral): centre
+    sum w c / W, per-axis variance sum w (Sigma_i,kk + (c_k - mu_k)^2) / W,
+    and the amplitude that gives the merged axis-aligned Gaussian the cell's
+    total complex mass sum m_i.  Empty cells are absent slots."""
+    from ..tile_voxelizer import _quaternion_to_rotation_matrix
+
+    dev = gaussians.positions.device
+    pd, ph, pw = patch_grid(shape)
+    n_slots = pd * ph * pw * SLOTS
+    c = _to_voxel(gaussians.positions.detach().float(), shape)
+    hs = torch.tensor(shape, dtype=torch.float32, device=dev) / 2
+    s = torch.exp(gaussians.log_scales.detach().float())
+    rot = _quaternion_to_rotation_matrix(gaussians.rotations.detach().float())
+    # Voxel-unit covariance diag(hs) R diag(s^2) R^T diag(hs): its diagonal and
+    # determinant (the voxelizer's q is u^T cov_inv u with u in half-shape units).
+    var = hs * hs * torch.einsum("nik,nk->ni", rot * rot, s * s)
+    integral = (2 * math.pi) ** 1.5 * (s * hs).prod(dim=1)
+    mr = gaussians.density_real.detach().float() * integral
+    mi = gaussians.density_imag.detach().float() * integral
+    w = torch.hypot(mr, mi)
+    cs = torch.tensor(CELL, dtype=torch.float32, device=dev)
+    cells = torch.tensor([pd * GRID[0], ph * GRID[1], pw * GRID[2]], device=dev)
+    cell = torch.floor((c + 0.5) / cs).long().clamp(min=0)
+    cell = torch.minimum(cell, cells - 1)
+    g = torch.tensor(GRID, device=dev)
+    patch, local = cell // g, cell % g
+    slot = (((patch[:, 0] * ph + patch[:, 1]) * pw + patch[:, 2]) * SLOTS
+            + (local[:, 0] * GRID[1] + local[:, 1]) * GRID[2] + local[:, 2])
+    ok = torch.isfinite(w) & torch.isfinite(c).all(1) & torch.isfinite(var).all(1) & (w > 0)
+    slot, c, var, w, mr, mi = slot[ok], c[ok], var[ok], w[ok], mr[ok], mi[ok]
+
+    def acc(v):
+        out = torch.zeros((n_slots,) + v.shape[1:], dtype=torch.float64, device=dev)
+        return out.index_add_(0, slot, v.double())
+
+    W = acc(w)
+    occ = W > 0
+    Wd = torch.where(occ, W, torch.ones_like(W))
+    mu = acc(w[:, None] * c) / Wd[:, None]
+    second = acc(w[:, None] * (var + c * c)) / Wd[:, None]
+    s2 = (second - mu * mu).clamp(min=1e-6)
+    sig_m = torch.sqrt(s2)
+    amp_scale = 1.0 / ((2 * math.pi) ** 1.5 * sig_m.prod(dim=1) * scale)
+    ar = acc(mr) * amp_scale
+    ai = acc(mi) * amp_scale
+    centre = slot_cell_centres(shape, dev).double()
+    half = cs.double() / 2
+    off = ((mu - centre) / half).clamp(-OFFSET_LIMIT, OFFSET_LIMIT)
+    log_sig = torch.log(sig_m).clamp(*LOG_SIGMA_RANGE)
+    exist = (occ & (torch.hypot(ar, ai) >= exist_threshold)).double()
+    tgt = torch.cat([off, log_sig, ar[:, None], ai[:, None], exist[:, None]], dim=1).float()
+    tgt[~occ, :8] = 0.0
     return tgt.view(-1, SLOTS, SLOT_FIELDS)
 
 

@@ -68,7 +68,7 @@ def run_train(args, log):
     dev = torch.device("cuda:0")
     seeds = [TRAIN_SEED0 + i for i in range(args.phantoms)]
     t0 = time.perf_counter()
-    data = build_dataset(seeds, dev, fit_steps=args.fit_steps, log=log)
+    data = build_dataset(seeds, dev, fit_steps=args.fit_steps, log=log, labels=args.labels)
     log({"dataset": len(data), "occupied": float(data.targets[..., 8].mean()), "build_s": time.perf_counter() - t0})
     model, hist = train_network(data, epochs=args.epochs, batch=args.batch, lr=args.lr,
                                 val_phantoms=max(1, args.phantoms // 20), log=log)
@@ -138,6 +138,9 @@ def main(argv=None):
     ap.add_argument("command", choices=["train", "evaluate", "all"])
     ap.add_argument("--phantoms", type=int, default=200)
     ap.add_argument("--fit-steps", type=int, default=500)
+    ap.add_argument("--labels", choices=["plain", "grid"], default="grid",
+                    help="slot targets from slot-grid fits (the shipped weights) or from the plain method's fits, "
+                         "moment-matched into the slots")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--batch", type=int, default=2048)
     ap.add_argument("--lr", type=float, default=1e-3)

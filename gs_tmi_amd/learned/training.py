@@ -3,9 +3,18 @@
 Training data: seeded synthetic acquisitions of the config-4 family at
 128 x 128 x 64 (60-ellipsoid phantom with smooth phase, 16 coils, Poisson-disc
 R=8 with a 12 x 12 calibration centre, k-space noise 1e-3).  Each is fitted by
-the plain reconstruction loop (HIP tile voxelizer, Adam) from the slot grid
-(one Gaussian per cell, rotations fixed, no densification), and the fit
-becomes the slot targets of its 8^3 patches.
+the plain reconstruction loop (HIP tile voxelizer, Adam), and the fit becomes
+the slot targets of its 8^3 patches.  Labels "plain": the plain method's own
+fit -- plain initialisation (gaussian/gaussian_model.py:308-355) with one
+Gaussian per slot, refined by the loop -- moment-matched into the slots
+(``plain_slot_targets``).  Labels "grid" (the shipped weights): a fit that
+starts from the slot grid (one Gaussian per cell, rotations fixed, no
+densification), so fitted Gaussian i is slot i.  A network trained on plain
+labels proposes about 80k broad Gaussians for a 256 x 256 x 128 volume
+(merged cells): it reaches the NMSE target faster under a fixed-size schedule,
+but densification every 100 steps splits nearly all of them three times over
+and the run does not recover; the grid labels' network proposes about 300k
+and converges under both schedules.
 
 Training: the network (``InitMLP``, BF16 MFMA GEMMs) minimises ``slot_loss``
 with Adam and a cosine learning-rate schedule; every batch is rotated by a
@@ -24,7 +33,7 @@ import torch
 from ..recon.data import gaussian_coils, make_problem, poisson_disc_mask
 from ..recon.model import GaussianSet
 from ..recon.trainer import ReconConfig, Reconstructor
-from .data import extract_patches, grid_gaussians, image_scale, slot_loss, slot_targets
+from .data import extract_patches, grid_gaussians, image_scale, plain_slot_targets, slot_loss, slot_targets
 from .model import SLOT_FIELDS, SLOTS, InitMLP
 
 TRAIN_SHAPE = (128, 128, 64)
@@ -51,6 +60,18 @@ def fit_slots(problem, voxelizer, steps: int, cfg: ReconConfig | None = None, ev
     return g, scale, hist
 
 
+def fit_plain(problem, voxelizer, steps: int, n_points: int, cfg: ReconConfig | None = None, seed: int = 0,
+              eval_every: int | None = None):
+    """The plain method's fit: plain initialisation (n_points Gaussians at
+    above-quantile voxels, identity rotations) refined by the loop with the
+    label fits' Adam settings; returns (GaussianSet, image scale, history)."""
+    scale = image_scale(problem.zero_filled)
+    g = GaussianSet.from_image(problem.zero_filled, n_points, seed=seed)
+    rec = Reconstructor(problem, g, voxelizer, cfg or label_fit_config())
+    hist = rec.run(steps, eval_every=eval_every or steps)
+    return rec.g, scale, hist
+
+
 @dataclass
 class SlotDataset:
     inputs: torch.Tensor     # bf16 [n, 1024]
@@ -61,7 +82,7 @@ class SlotDataset:
         return self.inputs.shape[0]
 
 
-def build_dataset(seeds, device, fit_steps=500, shape=TRAIN_SHAPE, log=None) -> SlotDataset:
+def build_dataset(seeds, device, fit_steps=500, shape=TRAIN_SHAPE, log=None, labels: str = "grid") -> SlotDataset:
     from ..tile_voxelizer import TileVoxelizer
     from .mfma import to_bf16
 
@@ -72,11 +93,17 @@ def build_dataset(seeds, device, fit_steps=500, shape=TRAIN_SHAPE, log=None) -> 
         t0 = time.perf_counter()
         p = training_problem(seed, device, shape, coils=coils)
         t1 = time.perf_counter()
-        g, scale, hist = fit_slots(p, vox, fit_steps)
+        if labels == "plain":
+            n_slots = (shape[0] // 8) * (shape[1] // 8) * (shape[2] // 8) * SLOTS
+            g, scale, hist = fit_plain(p, vox, fit_steps, n_slots, seed=seed)
+        elif labels == "grid":
+            g, scale, hist = fit_slots(p, vox, fit_steps)
+        else:
+            raise ValueError(f"labels must be 'plain' or 'grid', got {labels!r}")
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         xs.append(to_bf16(extract_patches(p.zero_filled, scale)))
-        ts.append(slot_targets(g, shape, scale))
+        ts.append(plain_slot_targets(g, shape, scale) if labels == "plain" else slot_targets(g, shape, scale))
         ps.append(torch.full((xs[-1].shape[0],), i, dtype=torch.int32, device=device))
         if log:
             log({"phantom": i, "seed": seed, "problem_s": round(t1 - t0, 3), "fit_s": round(t2 - t1, 3),

@@ -128,3 +128,69 @@ def test_phase_rotation_is_consistent():
     assert torch.equal(to[..., :6], t[..., :6]) and torch.equal(to[..., 8], t[..., 8])
     # Magnitudes are unchanged.
     assert np.allclose(z.abs().numpy(), torch.complex(x[:, :512], x[:, 512:]).abs().numpy(), atol=1e-5)
+
+
+def _set(pos_vox, sig_vox, amp, shape, rot=None):
+    """GaussianSet from voxel centres / sigmas (normalised parameters as the model keeps them)."""
+    shp = torch.tensor(shape, dtype=torch.float32)
+    pos = (torch.tensor(pos_vox, dtype=torch.float32) + 0.5) / shp * 2 - 1
+    log_s = torch.log(torch.tensor(sig_vox, dtype=torch.float32) / (shp / 2))
+    n = pos.shape[0]
+    if rot is None:
+        rot = torch.zeros(n, 4)
+        rot[:, 0] = 1.0
+    a = torch.tensor(amp, dtype=torch.complex64)
+    return GaussianSet(pos, log_s, torch.as_tensor(rot, dtype=torch.float32), a.real.clone(), a.imag.clone())
+
+
+def test_plain_targets_single_gaussian_is_itself():
+    """An axis-aligned Gaussian alone in its cell becomes that cell's slot unchanged
+    (offset, sigma, amplitude); every other slot is absent."""
+    shape = (8, 8, 16)
+    scale = 2.0
+    g = _set([[2.3, 5.1, 9.7]], [[0.9, 1.4, 2.2]], [1.5 - 0.5j], shape)
+    t = ld.plain_slot_targets(g, shape, scale)
+    # Cell (1, 2, 2) of patch (0, 0, 1): slot 32 + (1 * 4 + 2) * 2 + 0 (cell W index 2 -> patch 1, local 0).
+    s = SLOTS + (1 * 4 + 2) * 2 + 0
+    flat = t.reshape(-1, 9)
+    centre = ld.slot_cell_centres(shape, "cpu")[s]
+    assert torch.allclose(flat[s, :3], (torch.tensor([2.3, 5.1, 9.7]) - centre) / (torch.tensor(ld.CELL) / 2),
+                          atol=1e-5)
+    assert torch.allclose(flat[s, 3:6], torch.log(torch.tensor([0.9, 1.4, 2.2])), atol=1e-5)
+    assert torch.allclose(flat[s, 6:8], torch.tensor([0.75, -0.25]), atol=1e-5)
+    assert flat[s, 8] == 1.0 and float(flat[:, 8].sum()) == 1.0
+
+
+def test_plain_targets_merge_conserves_mass_and_moments():
+    """Two Gaussians in one cell (one rotated, amplitudes of different phase):
+    the merged slot carries their total complex mass, the |mass|-weighted
+    centre and the diagonal of the weighted second moment, each Gaussian's
+    own covariance from its rotation in the voxel frame of an anisotropic
+    volume."""
+    shape = (8, 16, 8)
+    scale = 1.0
+    q = torch.tensor([[1.0, 0.0, 0.0, 0.0], [0.9, 0.3, -0.2, 0.25]])
+    pos = [[2.1, 2.4, 4.9], [2.8, 3.3, 5.6]]
+    sig = [[0.6, 0.8, 1.0], [0.5, 0.7, 1.3]]
+    amp = [1.0 + 0.5j, -0.3 + 0.8j]
+    g = _set(pos, sig, amp, shape, rot=q)
+    flat = ld.plain_slot_targets(g, shape, scale, exist_threshold=0.0).reshape(-1, 9)
+    occ = torch.nonzero(flat[:, 8]).squeeze(-1)
+    assert occ.numel() == 1
+    s = int(occ)
+    # Reference moments in float64.
+    from gs_tmi_amd.tile_voxelizer import _quaternion_to_rotation_matrix
+    hs = torch.tensor(shape, dtype=torch.float64) / 2
+    sd = torch.tensor(sig, dtype=torch.float64) / hs  # normalised scales
+    R = _quaternion_to_rotation_matrix(q.double())
+    cov = torch.diag_embed(hs).unsqueeze(0) @ R @ torch.diag_embed(sd * sd) @ R.transpose(1, 2) @ torch.diag_embed(hs)
+    mass = torch.tensor(amp, dtype=torch.complex128) * (2 * math.pi) ** 1.5 * torch.sqrt(torch.linalg.det(cov))
+    w = mass.abs()
+    c = torch.tensor(pos, dtype=torch.float64)
+    mu = (w[:, None] * c).sum(0) / w.sum()
+    var = (w[:, None] * (torch.diagonal(cov, dim1=1, dim2=2) + (c - mu) ** 2)).sum(0) / w.sum()
+    centre = ld.slot_cell_centres(shape, "cpu")[s].double()
+    assert torch.allclose(flat[s, :3].double(), (mu - centre) / (torch.tensor(ld.CELL) / 2), atol=1e-4)
+    assert torch.allclose(flat[s, 3:6].double(), 0.5 * torch.log(var), atol=1e-4)
+    merged_mass = torch.complex(flat[s, 6].double(), flat[s, 7].double()) * (2 * math.pi) ** 1.5 * torch.sqrt(var).prod()
+    assert abs(complex(merged_mass) - complex(mass.sum())) < 1e-4 * float(w.sum())
