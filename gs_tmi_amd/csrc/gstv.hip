@@ -1327,21 +1327,44 @@ __device__ __forceinline__ int build_list(Stage& sh, int cnt, const BlockCtx& b,
   return nhit;
 }
 
-// Reductions over the 16 lanes of each row of a wave with DPP lane moves (no
-// LDS traffic): quad_perm [1,0,3,2] and [2,3,0,1] combine the lanes of each
-// quad, then row_ror:4 and row_ror:8 combine the quads; every lane of the row
-// ends with the result.  All 64 lanes must be active.
+// Operations over the 16 lanes of each row of a wave with DPP lane moves (no
+// LDS traffic).  All 64 lanes must be active.
 template <int kCtrl>
 __device__ __forceinline__ int dpp_mov(int v) {
   return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xf, 0xf, false);
 }
-__device__ __forceinline__ float row16_sum(float v) {
-  v += __int_as_float(dpp_mov<0xB1>(__float_as_int(v)));
-  v += __int_as_float(dpp_mov<0x4E>(__float_as_int(v)));
-  v += __int_as_float(dpp_mov<0x124>(__float_as_int(v)));
-  v += __int_as_float(dpp_mov<0x128>(__float_as_int(v)));
-  return v;
+
+// One halving step of a reduce-scatter over the 16 lanes of a row: lanes
+// paired by the DPP pattern kCtrl (a mirror, so partners sit on opposite sides
+// of the split), `upper` on the lane whose half is the upper one.  Each lane
+// keeps the half of v it owns plus its partner's copy of that half.
+template <int kCtrl, int kHalf>
+__device__ __forceinline__ void row16_halve(float (&v)[2 * kHalf], bool upper) {
+#pragma unroll
+  for (int j = 0; j < kHalf; ++j) {
+    const float send = upper ? v[j] : v[j + kHalf];
+    const float keep = upper ? v[j + kHalf] : v[j];
+    v[j] = keep + __int_as_float(dpp_mov<kCtrl>(__float_as_int(send)));
+  }
 }
+
+// Reduce-scatter of 16 values over the 16 lanes of each row: lane l of the row
+// ends with the sum over the row of value l in v[0].  Mirrors pair lane l
+// with 15 - l (row_mirror), with 7 - l inside each half (row_half_mirror), with
+// 3 - l and then l ^ 1 inside each quad; lane l's bits 3..0 pick the halves.
+__device__ __forceinline__ void row16_reduce_scatter(float (&v)[16], int l) {
+  row16_halve<0x140, 8>(v, (l & 8) != 0);
+  float (&v8)[8] = reinterpret_cast<float (&)[8]>(v);
+  row16_halve<0x141, 4>(v8, (l & 4) != 0);
+  float (&v4)[4] = reinterpret_cast<float (&)[4]>(v);
+  row16_halve<0x1B, 2>(v4, (l & 2) != 0);
+  float (&v2)[2] = reinterpret_cast<float (&)[2]>(v);
+  row16_halve<0xB1, 1>(v2, (l & 1) != 0);
+}
+
+// Maximum over the row, in every lane of the row: quad_perm [1,0,3,2] and
+// [2,3,0,1] combine the lanes of each quad, then row_ror:4 and row_ror:8 the
+// quads.
 __device__ __forceinline__ int row16_max(int v) {
   v = max(v, dpp_mov<0xB1>(v));
   v = max(v, dpp_mov<0x4E>(v));
@@ -1544,19 +1567,12 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_forward(const GRec* __restr
       fwd_groups<true>(sh, nhit, b, wave, gl, xi, x_ok, acc_r, acc_i);
   }
 
-  // Sum the 16 Gaussian slots of each W column (a row of 16 lanes).
-#pragma unroll
-  for (int i = 0; i < kSub * kSub; ++i) {
-    acc_r[i] = row16_sum(acc_r[i]);
-    acc_i[i] = row16_sum(acc_i[i]);
-  }
-  // Lane gl of row xi writes position (z, y) = (gl / 4, gl % 4) of column xi.
+  // Sum the 16 Gaussian slots of each W column (a row of 16 lanes), scattered
+  // so that lane gl of row xi holds position (z, y) = (gl / 4, gl % 4) of
+  // column xi.
+  row16_reduce_scatter(acc_r, gl);
+  row16_reduce_scatter(acc_i, gl);
   float out_r = acc_r[0], out_i = acc_i[0];
-#pragma unroll
-  for (int i = 1; i < kSub * kSub; ++i) {
-    out_r = (gl == i) ? acc_r[i] : out_r;
-    out_i = (gl == i) ? acc_i[i] : out_i;
-  }
   // any_bad is final: every batch's staging is behind a barrier.
   if (sh.any_bad != 0) fwd_non_finite(sh, rec, pair_ids, b, gl, xi, out_r, out_i);
   const int z = gl >> 2, y = gl & 3;
@@ -2164,13 +2180,18 @@ __global__ __launch_bounds__(256) void k_prologue_prep(const float* __restrict__
     half_shape[1] = sh1 * 0.5f;
     half_shape[2] = sh2 * 0.5f;
   }
+  if (gacc != nullptr) {
+    // The block's 256 moment rows are one contiguous range: zeroed with
+    // coalesced 16-byte stores (consecutive lanes, consecutive addresses).
+    constexpr int kVec = 256 * kAccStride / 4;
+    float4* rows = reinterpret_cast<float4*>(gacc + (int64_t)blockIdx.x * 256 * kAccStride);
+    const int valid = min(256, n - (int)blockIdx.x * 256) * (kAccStride / 4);
+#pragma unroll
+    for (int i = threadIdx.x; i < kVec; i += 256)
+      if (i < valid) rows[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
   if (counts != nullptr && g == n) counts[n] = 0;
   if (g >= n) return;
-  if (gacc != nullptr) {
-    float4* row = reinterpret_cast<float4*>(gacc + (int64_t)g * kAccStride);
-#pragma unroll
-    for (int k = 0; k < kAccStride / 4; ++k) row[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-  }
   float c[3], M[9], rad;
   prologue_one(g, pos, scales, rot, sh0, sh1, sh2, center, radius, cov, c, M, rad);
   prep_one(g, c, M, sh0 * 0.5f, sh1 * 0.5f, sh2 * 0.5f, rho_r, rho_i, rho_stride, rec, zt, rad, geo, counts);
