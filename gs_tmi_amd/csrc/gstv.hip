@@ -59,6 +59,7 @@ namespace {
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;                      // items per thread in scans
 constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr int kScanFoldBlocks = 1024;                // scans of up to this many blocks use two kernels
 
 constexpr int kExpandThreads = 256;
 constexpr int kExpandTile = 1024;                  // pairs per expansion block
@@ -168,7 +169,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int32_t* __r
   if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
 }
 
-// Phase 2: exclusive scan of the block sums in one block; total to *total.
+// Exclusive scan of the block sums in one block (large inputs only; see
+// launch_exclusive_scan); total to *total.
 __global__ __launch_bounds__(kScanThreads) void k_scan_block_sums(int64_t* __restrict__ sums, int nb,
                                                                   int64_t* __restrict__ total) {
   __shared__ int64_t sh[kScanThreads / 64];
@@ -184,12 +186,24 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_block_sums(int64_t* __res
   if (threadIdx.x == 0 && total != nullptr) *total = carry;
 }
 
-// Phase 3: exclusive scan inside each block plus the block offset.  Each
-// thread owns kScanItems consecutive items.  out may alias in.
+// Phase 2: exclusive scan inside each block plus the block's offset.  With
+// `scanned` the block sums were scanned in place by k_scan_block_sums;
+// otherwise each block sums the preceding blocks' sums itself (a few hundred
+// values, read from L2: one pass fewer) and the last block writes the grand
+// total to *total when given.  Each thread owns kScanItems consecutive items.
+// out may alias in.
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int32_t* in, int64_t n,
-                                                             const int64_t* __restrict__ block_offsets,
-                                                             int32_t* out) {
+                                                             const int64_t* __restrict__ block_sums, bool scanned,
+                                                             int64_t* __restrict__ total, int32_t* out) {
   __shared__ int64_t sh[kScanThreads / 64];
+  int64_t offset;
+  if (scanned) {
+    offset = block_sums[blockIdx.x];
+  } else {
+    int64_t part = 0;
+    for (int i = threadIdx.x; i < (int)blockIdx.x; i += kScanThreads) part += block_sums[i];
+    block_exclusive_scan(part, sh, &offset);
+  }
   const int64_t first = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
   int32_t v[kScanItems];
   int64_t s = 0;
@@ -200,13 +214,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int32_t* in, 
     s += v[r];
   }
   int64_t tot;
-  int64_t run = block_exclusive_scan(s, sh, &tot) + block_offsets[blockIdx.x];
+  int64_t run = block_exclusive_scan(s, sh, &tot) + offset;
 #pragma unroll
   for (int r = 0; r < kScanItems; ++r) {
     const int64_t i = first + r;
     if (i < n) out[i] = (int32_t)run;
     run += v[r];
   }
+  if (!scanned && total != nullptr && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) *total = offset + tot;
 }
 
 // Device-wide exclusive scan of n int32 values into int32 (values must fit).
@@ -217,10 +232,15 @@ int launch_exclusive_scan(const int32_t* in, int64_t n, int32_t* out, int64_t* b
   if (nb == 0) return GSTV_OK;
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanThreads), 0, s, in, n, block_sums);
   GSTV_LAUNCH_CHECK("scan_reduce");
-  hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(kScanThreads), 0, s, block_sums, nb, total);
-  GSTV_LAUNCH_CHECK("scan_block_sums");
-  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanThreads), 0, s, in, n,
-                     (const int64_t*)block_sums, out);
+  // Up to kScanFoldBlocks blocks, each block sums its predecessors' sums
+  // itself (two kernels); beyond, the block sums get a pass of their own.
+  const bool scanned = nb > kScanFoldBlocks;
+  if (scanned) {
+    hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(kScanThreads), 0, s, block_sums, nb, total);
+    GSTV_LAUNCH_CHECK("scan_block_sums");
+  }
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanThreads), 0, s, in, n, (const int64_t*)block_sums, scanned,
+                     total, out);
   GSTV_LAUNCH_CHECK("scan_apply");
   return GSTV_OK;
 }
@@ -487,10 +507,15 @@ struct ScatterLds {
 
 // Owner Gaussian of the first slot of every kRadixTile-slot block
 // (first[nblk] = owner of the last slot), written by the Gaussians themselves:
-// no per-block search.
+// no per-block search.  Thread g also presets tile g's range to (INT_MAX, 0)
+// for the last radix pass's merge (one launch for both).
 __global__ void k_block_first(const int32_t* __restrict__ offsets, int n, PairTotal pt,
-                              int32_t* __restrict__ first) {
+                              int32_t* __restrict__ first, int32_t* __restrict__ ranges, int num_tiles) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < num_tiles) {
+    ranges[2 * (int64_t)g] = INT_MAX;
+    ranges[2 * (int64_t)g + 1] = 0;
+  }
   const int total = pair_total(pt);
   if (g >= n || total == 0) return;  // total 0: no block reads first
   const int lo = offsets[g], hi = offsets[g + 1];
@@ -848,16 +873,8 @@ __global__ __launch_bounds__(kRadixThreads, 4) void k_radix_scatter(PairSource s
   radix_scatter<false, kLast>(src, pt, shift, bits, nblocks, hist_scanned, keys_out, vals_out, ranges);
 }
 
-// tile_ranges of the fused final pass: preset (INT_MAX, 0), then tiles with
-// no pair get (0, 0) as in the reference.
-__global__ void k_ranges_preset(int32_t* __restrict__ ranges, int num_tiles) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < num_tiles) {
-    ranges[2 * (int64_t)t] = INT_MAX;
-    ranges[2 * (int64_t)t + 1] = 0;
-  }
-}
-
+// tile_ranges of the fused final pass (preset to (INT_MAX, 0) by
+// k_block_first): tiles with no pair get (0, 0) as in the reference.
 __global__ void k_ranges_fix(int32_t* __restrict__ ranges, int num_tiles) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < num_tiles && ranges[2 * (int64_t)t] == INT_MAX) ranges[2 * (int64_t)t] = 0;
@@ -2681,7 +2698,8 @@ int bin_fill_impl(const GRec* rec, const float* center_vox, const float* radius_
     return GSTV_OK;
   }
 
-  hipLaunchKernelGGL(k_block_first, dim3(ceil_div_i(n, 256)), dim3(256), 0, s, gauss_offsets, n, pt, first);
+  hipLaunchKernelGGL(k_block_first, dim3(ceil_div_i(std::max(n, num_tiles), 256)), dim3(256), 0, s, gauss_offsets, n,
+                     pt, first, tile_ranges, num_tiles);
   GSTV_LAUNCH_CHECK("block_first");
   const int bits_total = key_bits(num_tiles);
   const int width = (bits_total + passes - 1) / passes;  // balanced digit widths
@@ -2712,13 +2730,6 @@ int bin_fill_impl(const GRec* rec, const float* center_vox, const float* radius_
       hipLaunchKernelGGL(k_radix_hist<false>, dim3(nblk), dim3(kRadixThreads), 0, s, src, pt, shift, bits, nblk,
                          hist);
     GSTV_LAUNCH_CHECK("radix_hist");
-    if (p == 0) {
-      // Needed only by the last pass; launched once the first long kernel is
-      // queued (the host launches while the GPU works).
-      hipLaunchKernelGGL(k_ranges_preset, dim3(ceil_div_i(num_tiles, 256)), dim3(256), 0, s, tile_ranges,
-                         num_tiles);
-      GSTV_LAUNCH_CHECK("ranges_preset");
-    }
     const int64_t hist_n = (int64_t)(1 << bits) * nblk;
     int st = launch_exclusive_scan(hist, hist_n, hist, sums, nullptr, s);
     if (st != GSTV_OK) return st;

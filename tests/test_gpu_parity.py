@@ -586,6 +586,32 @@ def test_binning_at_tile_count_bounds(shape):
     np.testing.assert_array_equal(pids.cpu().numpy(), p_exp)
 
 
+def test_binning_with_a_long_count_scan():
+    """2.6M Gaussians: the pair-count scan spans more than 1024 blocks, where
+    the scan scans its block sums in a pass of its own.  Offsets and total
+    equal the exclusive prefix sum of the oracle's per-Gaussian tile counts,
+    and the tile lists equal the oracle's."""
+    from gs_tmi_amd import _native
+
+    shape, tile = (64, 64, 64), 8
+    _announce(f"binning with a long count scan shape={shape} tile={tile} n=2600000")
+    rng = np.random.default_rng(29)
+    n = 2_600_000
+    center = (rng.random((n, 3)) * np.asarray(shape) - 0.5).astype(np.float32)
+    radius = (rng.random(n) * 1.5).astype(np.float32)
+    c_t, r_t = torch.from_numpy(center).to(DEV), torch.from_numpy(radius).to(DEV)
+    offsets, total = _native.bin_count(c_t, r_t, shape, tile)
+    lo, cnt = vo.tile_boxes(center, radius, shape, tile)
+    counts = np.prod(cnt.astype(np.int64), axis=1)
+    exp = np.concatenate([[0], np.cumsum(counts)])
+    np.testing.assert_array_equal(offsets.cpu().numpy().astype(np.int64), exp)
+    assert int(total.item()) == int(exp[-1])
+    ranges, pids = _native.bin_fill(c_t, r_t, shape, tile, offsets, int(total.item()))
+    r_exp, p_exp = vo.bin_tiles(center, radius, shape, tile)
+    np.testing.assert_array_equal(ranges.cpu().numpy(), r_exp)
+    np.testing.assert_array_equal(pids.cpu().numpy(), p_exp)
+
+
 def test_forward_writes_every_voxel():
     """Outputs come from torch.empty: every voxel, including empty tiles, must be written."""
     from gs_tmi_amd import _native
