@@ -73,6 +73,7 @@ constexpr int kTileThreads = 512;                  // 8 waves per tile block
 constexpr int kWaves = kTileThreads / 64;
 constexpr int kBatch = 512;                        // pairs staged per LDS batch
 constexpr int kSub = 4;                            // sub-brick edge (voxels)
+constexpr int kMaxSubTile = 16;                    // largest sub-tile edge the tile kernels render
 
 // -log2(e)/2: q' = kQScale * q, w = exp(-q/2) = exp2(q').
 constexpr double kQScaleD = -0.72134752044448170368;  // -0.5 / ln(2)
@@ -1107,12 +1108,21 @@ __device__ __forceinline__ void column_rows(float c0, float hmin, float hmax, fl
 
 // ------------------------------------------------------------ tile kernels --
 
+// A tile of more than 16 voxels per axis is rendered as parts^3 sub-tiles of
+// at most 16 (edge `sub` = ceil(tile / parts), parts = ceil(tile / 16)); each
+// sub-tile reads its tile's whole pair list and skips the pairs whose box
+// misses it, so every voxel sees the same pairs in the same order as with one
+// block per tile.  Tiles of at most 16 are their own single sub-tile.
 struct VoxGeom {
   int D, H, W, tile;
   int nTD, nTH, nTW, num_tiles;
-  int nsb;     // sub-bricks per tile axis, ceil(tile / 4)
-  int chunks;  // blocks per tile, ceil(nsb^3 / 8)
-  int blocks;  // num_tiles * chunks
+  int sub;            // sub-tile edge (voxels), <= 16
+  int parts;          // sub-tiles per tile axis
+  int nSD, nSH, nSW;  // sub-tile grid, nT * parts per axis
+  int num_sub;        // sub-tiles
+  int nsb;            // sub-bricks per sub-tile axis, ceil(sub / 4)
+  int chunks;         // blocks per sub-tile, ceil(nsb^3 / 8)
+  int blocks;         // num_sub * chunks
 };
 
 struct Stage {
@@ -1145,20 +1155,31 @@ struct BlockCtx {
 __device__ __forceinline__ BlockCtx make_ctx(int logical, const VoxGeom& geo, const int32_t* __restrict__ ranges,
                                              int wave) {
   BlockCtx b;
-  const int tile = logical / geo.chunks;
-  const int chunk = logical - tile * geo.chunks;
-  const int nthw = geo.nTH * geo.nTW;
-  const int td = tile / nthw;
-  const int th = (tile - td * nthw) / geo.nTW;
-  const int tw = tile - td * nthw - th * geo.nTW;
-  b.valid = tile < geo.num_tiles;
+  const int st = logical / geo.chunks;  // sub-tile
+  const int chunk = logical - st * geo.chunks;
+  const int nshw = geo.nSH * geo.nSW;
+  const int sd = st / nshw;
+  const int sh = (st - sd * nshw) / geo.nSW;
+  const int sw = st - sd * nshw - sh * geo.nSW;
+  int td = sd, th = sh, tw = sw, jd = 0, jh = 0, jw = 0;
+  if (geo.parts > 1) {
+    td = sd / geo.parts;
+    th = sh / geo.parts;
+    tw = sw / geo.parts;
+    jd = sd - td * geo.parts;
+    jh = sh - th * geo.parts;
+    jw = sw - tw * geo.parts;
+  }
+  const int tile = (td * geo.nTH + th) * geo.nTW + tw;
   b.tile = tile;
-  b.o0 = td * geo.tile;
-  b.o1 = th * geo.tile;
-  b.o2 = tw * geo.tile;
-  b.e0 = min(geo.tile, geo.D - b.o0);
-  b.e1 = min(geo.tile, geo.H - b.o1);
-  b.e2 = min(geo.tile, geo.W - b.o2);
+  b.o0 = td * geo.tile + jd * geo.sub;
+  b.o1 = th * geo.tile + jh * geo.sub;
+  b.o2 = tw * geo.tile + jw * geo.sub;
+  // Extent: the sub-tile, cut by its tile's end and the volume's.
+  b.e0 = min(min(geo.sub, geo.tile - jd * geo.sub), geo.D - b.o0);
+  b.e1 = min(min(geo.sub, geo.tile - jh * geo.sub), geo.H - b.o1);
+  b.e2 = min(min(geo.sub, geo.tile - jw * geo.sub), geo.W - b.o2);
+  b.valid = st < geo.num_sub && b.e0 > 0 && b.e1 > 0 && b.e2 > 0;
   const int sb = chunk * kWaves + wave;
   const int nsb = geo.nsb;
   const int sb0 = sb / (nsb * nsb);
@@ -1537,7 +1558,7 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_forward(const GRec* __restr
   const BlockCtx b = make_ctx(logical, geo, ranges, wave);
   if (!b.valid) return;
   const bool x_ok = xi < b.n2;
-  const bool full = b.e0 == geo.tile && b.e1 == geo.tile && b.e2 == geo.tile && (geo.tile & 3) == 0;
+  const bool full = b.e0 == geo.sub && b.e1 == geo.sub && b.e2 == geo.sub && (geo.sub & 3) == 0;
 
   float acc_r[kSub * kSub], acc_i[kSub * kSub];
 #pragma unroll
@@ -2449,8 +2470,10 @@ __global__ __launch_bounds__(kTileThreads) void k_count(const GRec* __restrict__
 
 // ------------------------------------------------------------- host helpers --
 
+// Any positive tile edge (the reference takes any, gaussian/tile_voxelizer.py:248-251);
+// the tile kernels split tiles above 16 into sub-tiles (VoxGeom).
 bool tile_geometry(int32_t d, int32_t h, int32_t w, int32_t tile, int nt[3], int64_t* num_tiles) {
-  if (d <= 0 || h <= 0 || w <= 0 || tile <= 0 || tile > 16) return false;
+  if (d <= 0 || h <= 0 || w <= 0 || tile <= 0) return false;
   nt[0] = ceil_div_i(d, tile);
   nt[1] = ceil_div_i(h, tile);
   nt[2] = ceil_div_i(w, tile);
@@ -2480,10 +2503,26 @@ VoxGeom make_vox_geom(int d, int h, int w, int tile, const int nt[3], int num_ti
   g.nTH = nt[1];
   g.nTW = nt[2];
   g.num_tiles = num_tiles;
-  g.nsb = ceil_div_i(tile, kSub);
+  g.parts = ceil_div_i(tile, kMaxSubTile);
+  g.sub = ceil_div_i(tile, g.parts);
+  g.nSD = nt[0] * g.parts;
+  g.nSH = nt[1] * g.parts;
+  g.nSW = nt[2] * g.parts;
+  g.num_sub = g.nSD * g.nSH * g.nSW;
+  g.nsb = ceil_div_i(g.sub, kSub);
   g.chunks = ceil_div_i(g.nsb * g.nsb * g.nsb, kWaves);
-  g.blocks = num_tiles * g.chunks;
+  g.blocks = g.num_sub * g.chunks;
   return g;
+}
+
+// Whether the tile kernels' grid of sub-tiles fits int indexing.
+bool vox_geom_fits(const int nt[3], int tile) {
+  const int64_t parts = ceil_div_i(tile, kMaxSubTile);
+  const int64_t sub = ceil_div_i(tile, (int)parts);
+  const int64_t nsb = (sub + kSub - 1) / kSub;
+  const int64_t chunks = (nsb * nsb * nsb + kWaves - 1) / kWaves;
+  const int64_t blocks = nt[0] * parts * nt[1] * parts * nt[2] * parts * chunks;
+  return nt[0] * parts < INT_MAX && nt[1] * parts < INT_MAX && nt[2] * parts < INT_MAX && blocks < INT_MAX - 8;
 }
 
 // Binning workspace: one key/value buffer pair (the first radix pass output,
@@ -2771,7 +2810,8 @@ namespace {
 int check_tile_args(const int32_t* tile_ranges, int32_t num_tiles, int32_t n, int32_t tile, int32_t d, int32_t h,
                     int32_t w, int nt[3]) {
   int64_t expect;
-  if (n < 0 || !tile_geometry(d, h, w, tile, nt, &expect)) return GSTV_ERR_INVALID_ARGUMENT;
+  if (n < 0 || !tile_geometry(d, h, w, tile, nt, &expect) || !vox_geom_fits(nt, tile))
+    return GSTV_ERR_INVALID_ARGUMENT;
   if ((int64_t)num_tiles != expect || tile_ranges == nullptr) return GSTV_ERR_INVALID_ARGUMENT;
   return GSTV_OK;
 }
@@ -2822,11 +2862,11 @@ void launch_backward(const GRec* rec, const ZRec* zt, float* gacc, const float* 
   const int slot = prof_start(1, s);
   VoxGeom geo_b = geo;  // the backward runs one block per tile
   geo_b.chunks = 1;
-  geo_b.blocks = num_tiles;
+  geo_b.blocks = geo.num_sub;
   const int tp = geo.nsb * kSub;
   const size_t lds_grad = (size_t)tp * grad_zs(tp) * sizeof(float2);
   const uint32_t* cols = geo.nsb <= 2 ? pair_columns : nullptr;
-  if (tile == 8)
+  if (geo.sub == 8)
     hipLaunchKernelGGL(k_backward<8>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, rec, zt,
                        tile_ranges, pair_ids, geo_b, grad_r, grad_i, gacc, cols);
   else

@@ -227,8 +227,8 @@ class TileVoxelizer(nn.Module):
         self.max_radius = max_radius  # kept for signature parity; unused, as in the reference
         self.device = device
         self.strict_cuda = strict_cuda
-        if not 1 <= self.tile_size <= 16:
-            raise ValueError(f"tile_size must be in [1, 16], got {tile_size}")
+        if self.tile_size < 1:
+            raise ValueError(f"tile_size must be a positive integer, got {tile_size}")
         if not use_cuda:
             raise RuntimeError("TileVoxelizer(use_cuda=False) selects the reference's PyTorch CPU backend, "
                                "which this package does not provide; the HIP backend needs a ROCm GPU")

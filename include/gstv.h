@@ -23,7 +23,9 @@
  *     gives the message.  Kernel launch errors are reported as GSTV_ERR_HIP.
  *   - Volumes are row-major [D][H][W]; flat index = d*H*W + h*W + w.
  *   - Tiles are cubes of `tile` voxels; nT* = ceil(size / tile); tile id =
- *     td*(nTH*nTW) + th*nTW + tw.  `tile` must be in [1, 16].
+ *     td*(nTH*nTW) + th*nTW + tw.  `tile` is any positive edge; the tile
+ *     kernels render a tile above 16 as sub-tiles of at most 16 per axis, each
+ *     reading the tile's whole pair list.
  *   - Gaussian parameters are in voxel space: center_vox [N][3] (axis order
  *     D, H, W), cov_inv [N][3][3] row-major (the inverse covariance in units
  *     normalised by half_shape), density planes rho_r [N], rho_i [N].
@@ -32,8 +34,8 @@
  *   - A Gaussian with a non-finite centre coordinate or cov_inv fails the
  *     reference's cutoff test (mahal <= 9, tile_voxelize_cuda_kernel.cu:130,233)
  *     at every voxel: it adds nothing and its gradients are exactly zero.  A
- *     non-finite density reaches every voxel the kernels evaluate for its
- *     Gaussian (the reference's only those within the cutoff).
+ *     non-finite density, or upstream gradient, reaches exactly the voxels
+ *     within its Gaussian's cutoff, as in the reference.
  */
 #ifndef GSTV_H
 #define GSTV_H

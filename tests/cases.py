@@ -103,7 +103,8 @@ def random_case(seed):
     return dict(shape=shape, tile=tile, pos=pos, scales=scales, rot=rot, rr=rr, ri=ri)
 
 
-EDGE_KINDS = ["partial_tiles", "outside", "tile1", "tile3", "tile5", "tile12", "tile16", "needles", "extreme_aniso",
+EDGE_KINDS = ["partial_tiles", "outside", "tile1", "tile3", "tile5", "tile12", "tile16", "tile17", "tile24", "tile32",
+              "needles", "extreme_aniso",
               "large_sigma", "tiny_sigma", "anisotropic_volume", "single"]
 
 

@@ -2,7 +2,8 @@
 error against FP64 on every parity case (see tests/reference_fp32.py), from
 which the GPU parity tests derive their tolerances.  CPU only:
 
-    python tests/golden/make_reference_fp32_errors.py
+    python tests/golden/make_reference_fp32_errors.py            # every case
+    python tests/golden/make_reference_fp32_errors.py --missing  # only cases the table lacks
 """
 
 from __future__ import annotations
@@ -34,9 +35,14 @@ def function_inputs(c):
 
 def main():
     torch.set_num_threads(8)
+    missing = "--missing" in sys.argv[1:]
     out = {"note": "relative L2 error against FP64 of the reference's fp32 CPU composition; "
                    "made by tests/golden/make_reference_fp32_errors.py", "function": {}, "call": {}, "param": {}}
+    if missing:
+        out = rf.load_table()
     for name in cases.FUNCTION_CASES:
+        if name in out["function"]:
+            continue
         t0 = time.time()
         c = cases.case_by_name(name)
         center, cov = function_inputs(c)
@@ -44,12 +50,16 @@ def main():
         out["function"][name] = rf.function_errors(c, center, cov, g_r, g_i, FUNCTION_BAND)
         print(f"function {name}: {out['function'][name]} ({time.time() - t0:.1f} s)", flush=True)
     for name in cases.CALL_CASES:
+        if name in out["call"]:
+            continue
         t0 = time.time()
         c = cases.case_by_name(name)
         g_r, g_i = cases.call_upstream(c["shape"])
         out["call"][name] = rf.call_errors(c, g_r, g_i)
         print(f"call {name}: {out['call'][name]} ({time.time() - t0:.1f} s)", flush=True)
     for reading, seed in (("voxel", 0), ("literal", 1)):
+        if reading in out["param"]:
+            continue
         c = cases.adjoint_check_case(seed, reading)
         g_r, g_i = cases.call_upstream(c["shape"], seed=3)
         out["param"][reading] = rf.call_errors(c, g_r, g_i)

@@ -105,18 +105,17 @@ def test_drop_in_differences_raise_clear_errors():
     """Where this package deliberately differs from the reference's
     TileVoxelizer (gaussian/tile_voxelizer.py:248-277), it raises instead of
     silently changing backend: use_cuda=False (the reference's PyTorch CPU
-    path), tile sizes above 16, and -- on a machine without a ROCm GPU -- both
-    strict and non-strict construction (the reference would fall back to its
-    CPU path; a fallback here would not be the HIP path)."""
+    path), a tile size below 1 (the reference would fail later, inside its
+    binning), and -- on a machine without a ROCm GPU -- both strict and
+    non-strict construction (the reference would fall back to its CPU path; a
+    fallback here would not be the HIP path)."""
     import torch
 
     from gs_tmi_amd import TileVoxelizer
 
     with pytest.raises(RuntimeError, match="use_cuda=False"):
         TileVoxelizer((16, 16, 16), use_cuda=False)
-    with pytest.raises(ValueError, match=r"tile_size must be in \[1, 16\]"):
-        TileVoxelizer((16, 16, 16), tile_size=17)
-    with pytest.raises(ValueError, match=r"tile_size must be in \[1, 16\]"):
+    with pytest.raises(ValueError, match="tile_size must be a positive integer"):
         TileVoxelizer((16, 16, 16), tile_size=0)
     if not torch.cuda.is_available():
         for strict in (False, True):
