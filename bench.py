@@ -390,6 +390,11 @@ def main():
                      "reference_basis_tflops": round(k_flop_ref / (k_ms * 1e-3) / 1e12, 3),
                      "algorithmic_speedup": round((fwd_flop_ref + bwd_flop_ref) / (fwd_flop + bwd_flop), 3),
                      "hw_tflops": None if kc is None else round(kc["fp32_flop"] / (k_ms * 1e-3) / 1e12, 3),
+                     # The kernel's shader clock under load (GRBM_GUI_ACTIVE over its duration) and the
+                     # fraction of the peak scaled to that clock (157.3 TFLOP/s is quoted at 2.4 GHz).
+                     "clock_ghz": None if kc is None or kc.get("clock_ghz") is None else round(kc["clock_ghz"], 3),
+                     "frac_at_clock": None if kc is None or not kc.get("clock_ghz") else
+                     round(achieved / (PEAK_FP32_TFLOPS * kc["clock_ghz"] / 2.4), 4),
                      "counters": None if counters is None else COUNTERS.name},
         "cpu_baseline": cpu,
         "learned_init": learned,

@@ -13,7 +13,9 @@ Each DIR is the ``-d`` output directory of one rocprofv3 run of
            SQ_INSTS_VALU_TRANS_F32 SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_BUSY_CYCLES GRBM_GUI_ACTIVE``
   --lds    ``--pmc SQ_LDS_BANK_CONFLICT SQ_INSTS_LDS SQ_WAVE_CYCLES SQ_INSTS_SALU``
 
-HBM bytes are the counters scaled by what the same counter reads for the
+The shader clock of each kernel is its GRBM_GUI_ACTIVE count in the --valu
+run over the XCD count and the dispatch's duration.  HBM bytes are the
+counters scaled by what the same counter reads for the
 bench's copy of a known size (the ``__amd_rocclr_copyBuffer`` dispatch of
 ``--calib-mib`` MiB, which reads and writes that many bytes once).  Executed
 FP32 FLOPs are 64 lanes x (2 FMA + ADD + MUL + TRANS) wave instructions.
@@ -78,6 +80,40 @@ def counters(d, reduce=None):
     return {k: {c: red(v) for c, v in cs.items()} for k, cs in vals.items()}
 
 
+def num_xcc(d):
+    """XCDs of the profiled GPU, from the run's agent info (8 on MI355X)."""
+    for f in _csvs(d, "agent_info.csv"):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Agent_Type") == "GPU" and r.get("Num_Xcc", "") not in ("", "0"):
+                    return int(r["Num_Xcc"])
+    return 8
+
+
+def dispatch_clocks(d, min_ns=200_000):
+    """{kernel name: shader clock in GHz}: the GRBM_GUI_ACTIVE count of a
+    dispatch (summed over the XCDs) over the XCD count and the dispatch's own
+    duration, over the kernel's dispatches of at least ``min_ns`` (the count
+    includes a start-up that dominates short dispatches)."""
+    nx = num_xcc(d)
+    per = collections.defaultdict(float)
+    dur, names = {}, {}
+    for f in _csvs(d, "counter_collection.csv"):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if r["Counter_Name"] != "GRBM_GUI_ACTIVE":
+                    continue
+                per[r["Dispatch_Id"]] += float(r["Counter_Value"])
+                dur[r["Dispatch_Id"]] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+                names[r["Dispatch_Id"]] = _short(r["Kernel_Name"])
+    cnt, tim = collections.defaultdict(float), collections.defaultdict(float)
+    for k, v in per.items():
+        if dur[k] >= min_ns:
+            cnt[names[k]] += v / nx
+            tim[names[k]] += dur[k]
+    return {k: cnt[k] / tim[k] for k in cnt}
+
+
 def _find(table, key):
     hits = [k for k in table if k.startswith(key) or k.split("(")[0] == key.rstrip("(")]
     return hits[0] if hits else None
@@ -92,6 +128,8 @@ def summarise(stats_d, fetch_d, write_d, valu_d, lds_d, calib_mib):
     copy_f, copy_w = _find(fe_max, COPY_KERNEL), _find(wr_max, COPY_KERNEL)
     if copy_f is None or copy_w is None:
         raise SystemExit("no calibration copy in the FETCH_SIZE / WRITE_SIZE runs (bench.py --copy-calibration-mib)")
+    clk = dispatch_clocks(valu_d)
+    copy_clk = _find(clk, COPY_KERNEL)
     nbytes = calib_mib * (1 << 20)
     f_scale = nbytes / (fe_max[copy_f]["FETCH_SIZE"] * 1024.0)  # true bytes per counted byte
     w_scale = nbytes / (wr_max[copy_w]["WRITE_SIZE"] * 1024.0)
@@ -117,6 +155,9 @@ def summarise(stats_d, fetch_d, write_d, valu_d, lds_d, calib_mib):
             row["fp32_flop"] = 64.0 * (2 * fp[0] + fp[1] + fp[2] + fp[3])
             row["fp32_share_of_valu"] = sum(fp) / c["SQ_INSTS_VALU"] if c.get("SQ_INSTS_VALU") else None
             row["valu_issue"] = c["SQ_INSTS_VALU"] / (VALU_PEAK_ISSUE * avg_ns * 1e-9) if c.get("SQ_INSTS_VALU") else None
+        k = _find(clk, key)
+        if k is not None:
+            row["clock_ghz"] = clk[k]
         k = _find(ld, key)
         if k is not None:
             row["lds_bank_conflict_cycles"] = ld[k].get("SQ_LDS_BANK_CONFLICT")
@@ -129,9 +170,11 @@ def summarise(stats_d, fetch_d, write_d, valu_d, lds_d, calib_mib):
             kernels[tag] = {"fetch_bytes": r["fetch_bytes"], "write_bytes": r["write_bytes"],
                             "fp32_flop": r["fp32_flop"], "avg_us": r["avg_us"],
                             "fp32_share_of_valu": r.get("fp32_share_of_valu"), "valu_issue": r.get("valu_issue"),
-                            "lds_bank_conflict_cycles": r.get("lds_bank_conflict_cycles")}
+                            "lds_bank_conflict_cycles": r.get("lds_bank_conflict_cycles"),
+                            "clock_ghz": r.get("clock_ghz")}
     return {"gstv_hip_sha256": hashlib.sha256(SOURCE.read_bytes()).hexdigest(),
-            "calibration": {"copy_mib": calib_mib, "fetch_scale": f_scale, "write_scale": w_scale},
+            "calibration": {"copy_mib": calib_mib, "fetch_scale": f_scale, "write_scale": w_scale,
+                            "copy_clock_ghz": clk[copy_clk] if copy_clk is not None else None},
             "kernels": kernels, "rows": list(rows.values())}
 
 
@@ -146,19 +189,23 @@ def markdown(s, title, commands):
             f"{s['calibration']['fetch_scale'] * 1024:.1f} and WRITE_SIZE KiB x "
             f"{s['calibration']['write_scale'] * 1024:.1f} (bytes per KiB counted).",
             f"Kernel source SHA-256: `{s['gstv_hip_sha256']}`.", "",
+            "Clock: GRBM_GUI_ACTIVE of each dispatch over the XCD count and the dispatch's duration; the "
+            f"calibration copy (memory-bound) reads {f(s['calibration'].get('copy_clock_ghz'), '.2f')} GHz by the "
+            "same measure, against a published peak engine clock of 2.4 GHz; dispatches shorter than 200 µs are "
+            "left out (–). VALU issue is wave instructions x 4 cycles over 1024 SIMDs x 2.4 GHz x the duration.", "",
             "| kernel | calls | avg µs | HBM read MB | HBM write MB | VALU wave-instr | FP32 share of VALU | "
-            "executed FP32 TFLOP/s | VALU issue | LDS bank-conflict cycles |",
-            "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+            "executed FP32 TFLOP/s | VALU issue | clock GHz | LDS bank-conflict cycles |",
+            "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
     for r in s["rows"]:
         tf = r["fp32_flop"] / (r["avg_us"] * 1e-6) / 1e12 if r.get("fp32_flop") is not None else None
-        out.append("| `{}` | {} | {} | {} | {} | {} | {} | {} | {} | {} |".format(
+        out.append("| `{}` | {} | {} | {} | {} | {} | {} | {} | {} | {} | {} |".format(
             r["name"], r["calls"], f(r["avg_us"], ".1f"),
             f(r.get("fetch_bytes", None) and r["fetch_bytes"] / 1e6, ".0f"),
             f(r.get("write_bytes", None) and r["write_bytes"] / 1e6, ".0f"),
             f(r.get("valu"), ".3e"),
             f(r.get("fp32_share_of_valu") and 100 * r["fp32_share_of_valu"], ".0f") + ("%" if r.get("fp32_share_of_valu") else ""),
             f(tf, ".1f"), f(r.get("valu_issue") and 100 * r["valu_issue"], ".0f") + ("%" if r.get("valu_issue") else ""),
-            f(r.get("lds_bank_conflict_cycles"), ".2e")))
+            f(r.get("clock_ghz"), ".2f"), f(r.get("lds_bank_conflict_cycles"), ".2e")))
     return "\n".join(out) + "\n"
 
 
