@@ -212,6 +212,13 @@ def learned_init_leg(dev, max_steps=1500, eval_every=25, target=1e-2, config4_st
             torch.cuda.empty_cache()
         a, b = leg["learned"]["time_to_target_s"], leg["plain"]["time_to_target_s"]
         leg["ratio"] = None if (a is None or b is None) else round(a / b, 4)
+        if a is not None and b is None:
+            # The plain start did not reach the target within its steps, so its
+            # time to target exceeds its whole wall time: the ratio is below
+            # learned time to target / plain wall time.
+            plain_wall = leg["plain"]["_init_s"] + leg["plain"]["_rows"][-1]["train_s"]
+            leg["plain"]["wall_s"] = round(plain_wall, 3)
+            leg["ratio_upper_bound"] = round(a / plain_wall, 4)
         if a is not None:
             leg["plain_nmse_at_learned_time_to_target"] = _nmse_at(leg["plain"]["_rows"], leg["plain"]["_init_s"], a)
         for kind in ("learned", "plain"):
