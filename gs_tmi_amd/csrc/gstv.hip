@@ -280,7 +280,42 @@ struct BinGeom {
   int tile;
   int tile_shift;   // log2(tile) for a power of two, else -1
   const GRec* rec;  // non-null: pairs only where the records' extent boxes meet the tile
+  // Non-null (with rec): per Gaussian, the live tiles of a tile box of at most
+  // kLiveMaxTiles tiles, bit k for the box's k-th tile in row-major local order
+  // (written by the counting pass, read by the expansion); larger boxes keep
+  // every tile.
+  uint32_t* live;
 };
+
+// Tile boxes of at most this many tiles carry a live-tile mask.
+constexpr int kLiveMaxTiles = 32;
+
+// Position of the set bit of rank l (0-based) in m; m has more than l set bits.
+__device__ __forceinline__ int nth_set_bit(uint32_t m, int l) {
+  int pos = 0;
+#pragma unroll
+  for (int w = 16; w >= 1; w >>= 1) {
+    const int c = __popc(m & ((1u << w) - 1u));
+    if (l >= c) {
+      l -= c;
+      m >>= w;
+      pos += w;
+    }
+  }
+  return pos;
+}
+
+// Local index l of a Gaussian's pair -> (ld, lh, lw) in its tile box of
+// tcnt[1] x tcnt[2] tiles per D slab; with a live mask, l counts live tiles.
+__device__ __forceinline__ void box_local(int l, const int tcnt[3], const uint32_t* live, int g, int& ld, int& lh,
+                                          int& lw) {
+  const int n12 = tcnt[1] * tcnt[2];
+  if (live != nullptr && tcnt[0] * n12 <= kLiveMaxTiles) l = nth_set_bit(live[g], l);
+  ld = l / n12;
+  const int rem = l - ld * n12;
+  lh = rem / tcnt[2];
+  lw = rem - lh * tcnt[2];
+}
 
 inline int tile_shift_of(int tile) {
   int sh = 0;
@@ -408,7 +443,7 @@ __global__ __launch_bounds__(kExpandThreads) void k_bin_expand(
     int32_t* __restrict__ vals) {
   __shared__ int32_t sh_off[kExpandTile + 1];
   __shared__ int32_t sh_t0[kExpandTile], sh_t1[kExpandTile], sh_t2[kExpandTile];
-  __shared__ int32_t sh_n1[kExpandTile], sh_n2[kExpandTile];
+  __shared__ int32_t sh_n0[kExpandTile], sh_n1[kExpandTile], sh_n2[kExpandTile];
   __shared__ int sh_g0, sh_ng;
   const int total = pair_total(pt);
   const int j0 = blockIdx.x * kExpandTile;
@@ -431,12 +466,8 @@ __global__ __launch_bounds__(kExpandThreads) void k_bin_expand(
       const int g = last_le(offsets, n + 1, j);
       int tmin[3], tcnt[3];
       gauss_tile_box(center, radius, g, geo, tmin, tcnt);
-      const int l = j - offsets[g];
-      const int n12 = tcnt[1] * tcnt[2];
-      const int ld = l / n12;
-      const int rem = l - ld * n12;
-      const int lh = rem / tcnt[2];
-      const int lw = rem - lh * tcnt[2];
+      int ld, lh, lw;
+      box_local(j - offsets[g], tcnt, geo.live, g, ld, lh, lw);
       keys[j] = (tmin[0] + ld) * nthw + (tmin[1] + lh) * ntw + (tmin[2] + lw);
       vals[j] = g;
     }
@@ -449,19 +480,16 @@ __global__ __launch_bounds__(kExpandThreads) void k_bin_expand(
     sh_t0[i] = tmin[0];
     sh_t1[i] = tmin[1];
     sh_t2[i] = tmin[2];
+    sh_n0[i] = tcnt[0];
     sh_n1[i] = tcnt[1];
     sh_n2[i] = tcnt[2];
   }
   __syncthreads();
   for (int j = j0 + threadIdx.x; j < j1; j += kExpandThreads) {
     const int i = last_le(sh_off, ng, j);
-    const int l = j - sh_off[i];
-    const int n2 = sh_n2[i];
-    const int n12 = sh_n1[i] * n2;
-    const int ld = l / n12;
-    const int rem = l - ld * n12;
-    const int lh = rem / n2;
-    const int lw = rem - lh * n2;
+    const int tcnt[3] = {sh_n0[i], sh_n1[i], sh_n2[i]};
+    int ld, lh, lw;
+    box_local(j - sh_off[i], tcnt, geo.live, g0 + i, ld, lh, lw);
     keys[j] = (sh_t0[i] + ld) * nthw + (sh_t1[i] + lh) * ntw + (sh_t2[i] + lw);
     vals[j] = g0 + i;
   }
@@ -549,8 +577,16 @@ __device__ __forceinline__ bool gen_setup(GenLds& L, const float* __restrict__ c
     gauss_tile_box(center, radius, g0 + i, geo, tmin, tcnt);
     L.off[i] = offsets[g0 + i];
     L.k0[i] = tmin[0] * nthw + tmin[1] * ntw + tmin[2];
-    L.n12[i] = tcnt[1] * tcnt[2];
-    L.n2[i] = tcnt[2];
+    const int n12 = tcnt[1] * tcnt[2];
+    if (geo.live != nullptr && tcnt[0] * n12 <= kLiveMaxTiles) {
+      // A box with a live mask: n12, n2 <= 32 packed as -1 - (n12 << 6 | n2),
+      // and the mask in place of n2.
+      L.n12[i] = -1 - ((n12 << 6) | tcnt[2]);
+      L.n2[i] = (int32_t)geo.live[g0 + i];
+    } else {
+      L.n12[i] = n12;
+      L.n2[i] = tcnt[2];
+    }
   }
   __syncthreads();
   // Slot -> Gaussian map.  The owner of a slot is the last Gaussian starting
@@ -623,8 +659,15 @@ __device__ __forceinline__ int div_small(int l, int d) {
 __device__ __forceinline__ void gen_pair(const GenLds& L, int j, int j0, const BinGeom& geo, int32_t& key,
                                          int32_t& val) {
   const int i = L.gidx[j - j0];
-  const int l = j - L.off[i];
-  const int n12 = L.n12[i], n2 = L.n2[i];
+  int l = j - L.off[i];
+  int n12 = L.n12[i], n2 = L.n2[i];
+  if (n12 < 0) {
+    // A live-masked box: the l-th live tile.
+    l = nth_set_bit((uint32_t)n2, l);
+    const int p = -1 - n12;
+    n12 = p >> 6;
+    n2 = p & 63;
+  }
   const int ld = div_small(l, n12);
   const int rem = l - ld * n12;
   const int lh = div_small(rem, n2);
@@ -640,12 +683,8 @@ __device__ __forceinline__ void gen_pair_global(const float* __restrict__ center
   const int g = last_le(offsets, n + 1, j);
   int tmin[3], tcnt[3];
   gauss_tile_box(center, radius, g, geo, tmin, tcnt);
-  const int l = j - offsets[g];
-  const int n12 = tcnt[1] * tcnt[2];
-  const int ld = l / n12;
-  const int rem = l - ld * n12;
-  const int lh = rem / tcnt[2];
-  const int lw = rem - lh * tcnt[2];
+  int ld, lh, lw;
+  box_local(j - offsets[g], tcnt, geo.live, g, ld, lh, lw);
   key = (tmin[0] + ld) * (geo.nt[1] * geo.nt[2]) + (tmin[1] + lh) * geo.nt[2] + (tmin[2] + lw);
   val = g;
 }
@@ -915,6 +954,110 @@ __device__ __forceinline__ ZCoefs z_load(const ZRec* __restrict__ zt, int g) {
 
 
 
+// ------------------------------------------------ D extent over a column --
+//
+// The D rows of a voxel column (H in [y0, y1], W in [x0, x1]) that can hold a
+// voxel of the widened ellipsoid E = {u : u^T A u <= R^2} (R = 3.06, the
+// radius of the extents e_a = R sqrt(C_aa) + 0.01, C = A^-1) lie in the
+// intersection of two intervals: the D extent of E cut by the H strip alone
+// and by the W strip alone.  Cutting by the strip u_i in [lo, hi] is the same
+// as cutting E's projection on (u_0, u_i), an ellipse with matrix C_sub^-1;
+// along u_i = t that ellipse spans
+//     u_0 in al t -/+ ga sqrt(rho^2 - t^2),
+//     al = C_0i / C_ii,  ga = sqrt(C_00 C_ii - C_0i^2) / C_ii,  rho = R sqrt(C_ii),
+// a concave upper and a convex lower boundary whose extremes lie at
+// t = +/- ta, ta = al rho^2 / rho_0.  Over [lo, hi] the maximum of u_0 is
+// therefore at clamp(ta, lo, hi) and the minimum at clamp(-ta, lo, hi).
+// With cofactors of the scaled form A' = kq A (C_ii = cof_ii / det and, by
+// Jacobi's identity, C_00 C_ii - C_0i^2 = A_jj / det A):
+//     al = cof_0i / cof_ii,  ga = sqrt(A'_jj det A') / cof_ii.
+// The determinant cancels badly for needle-shaped Gaussians, so cofactors and
+// determinant are formed in fp64 from the fp32 form the kernels evaluate.
+// D extent [zmin, zmax] (relative to the centre) of the projected ellipse over
+// the strip u_i in [lo, hi].  Clamps are single med3 instructions: with a
+// non-finite strip (non-finite centre) the result is arbitrary, which is
+// harmless -- such a Gaussian has q = NaN or -inf everywhere and contributes
+// nothing.
+__device__ __forceinline__ void z_span(const ZAxis& a, float lo, float hi, float& zmin, float& zmax) {
+  const float tp = __builtin_amdgcn_fmed3f(a.ta, lo, hi), tm = __builtin_amdgcn_fmed3f(-a.ta, lo, hi);
+  const float rr = a.rho * a.rho;
+  const float sp = __builtin_amdgcn_sqrtf(__builtin_amdgcn_fmed3f(fmaf(-tp, tp, rr), 0.0f, INFINITY));
+  const float sm = __builtin_amdgcn_sqrtf(__builtin_amdgcn_fmed3f(fmaf(-tm, tm, rr), 0.0f, INFINITY));
+  zmax = fmaf(a.ga, sp, a.al * tp);
+  zmin = fmaf(-a.ga, sm, a.al * tm);
+}
+
+// Absolute D rows [zl, zh] of a column from the strip extents, clipped to the
+// Gaussian's box rows [lo0, hi0]; a NaN bound falls back to the box.
+__device__ __forceinline__ void column_rows(float c0, float hmin, float hmax, float wmin, float wmax, float lo0,
+                                            float hi0, float& zl, float& zh) {
+  zl = fmaxf(ceilf(c0 + fmaxf(hmin, wmin) - 0.01f), lo0);
+  zh = fminf(floorf(c0 + fminf(hmax, wmax) + 0.01f), hi0);
+}
+
+// Live tiles of one Gaussian's tile box (tile_box_v) for the binning, bit k
+// for the box's k-th tile in row-major local order.  A tile is dead when its
+// voxels inside the Gaussian's extent box cannot meet the widened ellipsoid
+// that the column bound keeps (R = 3.06): the D rows that the (D, H) and
+// (D, W) projections leave over the tile's whole H and W strips are empty, or
+// the H rows that the (H, W) projection leaves over its W strip.  Over the
+// whole strips each bound contains the tile kernels' per-column bounds, and
+// the margin (0.05 voxel against the kernels' 0.01) covers fp32 rounding, so a
+// dead tile is one where the tile kernels evaluate no voxel whose q passes the
+// cutoff: dropping its pair changes no contribution.  A Gaussian without a
+// bound keeps every tile.
+constexpr float kLiveMargin = 0.05f;
+__device__ __forceinline__ uint32_t live_tiles(const GRec& r, const ZRec& z, const ZAxis& zhw, const BinGeom& geo,
+                                               const int tmin[3], const int tcnt[3]) {
+  const int cnt = tcnt[0] * tcnt[1] * tcnt[2];
+  const uint32_t all = cnt >= 32 ? 0xffffffffu : ((1u << cnt) - 1u);
+  // More than 4 tiles along an axis, or no bound: keep the box.
+  if (tcnt[0] > 4 || tcnt[1] > 4 || tcnt[2] > 4 || !(z.h.rho > 0.0f) || !(zhw.rho > 0.0f)) return all;
+  const float c[3] = {r.c0, r.c1, r.c2}, e[3] = {r.e0, r.e1, r.e2};
+  // Voxel range of the box's tiles per axis (the box cut to the extent box).
+  float vlo[3][4], vhi[3][4];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float blo = ceilf(c[a] - e[a]), bhi = floorf(c[a] + e[a]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int t = tmin[a] + k;
+      vlo[a][k] = fmaxf(blo, (float)(t * geo.tile));
+      vhi[a][k] = fminf(bhi, (float)(min(t * geo.tile + geo.tile, geo.size[a]) - 1));
+    }
+  }
+  // Spans per H tile (the (D, H) projection) and per W tile (the (D, W) and
+  // (H, W) projections): the tests of a tile combine its H and W spans.
+  float hmn[4], hmx[4], wmn[4], wmx[4], ylo[4], yhi[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    z_span(z.h, vlo[1][k] - c[1], vhi[1][k] - c[1], hmn[k], hmx[k]);
+    z_span(z.w, vlo[2][k] - c[2], vhi[2][k] - c[2], wmn[k], wmx[k]);
+    float ymn, ymx;
+    z_span(zhw, vlo[2][k] - c[2], vhi[2][k] - c[2], ymn, ymx);
+    ylo[k] = ceilf(c[1] + ymn - kLiveMargin);
+    yhi[k] = floorf(c[1] + ymx + kLiveMargin);
+  }
+  uint32_t mask = 0;
+#pragma unroll
+  for (int i1 = 0; i1 < 4; ++i1) {
+#pragma unroll
+    for (int i2 = 0; i2 < 4; ++i2) {
+      const float zl = ceilf(c[0] + fmaxf(hmn[i1], wmn[i2]) - kLiveMargin);
+      const float zh = floorf(c[0] + fminf(hmx[i1], wmx[i2]) + kLiveMargin);
+      // Written so that a NaN keeps the tile.
+      const bool hdead = fmaxf(ylo[i2], vlo[1][i1]) > fminf(yhi[i2], vhi[1][i1]);
+#pragma unroll
+      for (int i0 = 0; i0 < 4; ++i0) {
+        const bool dead = hdead || (fmaxf(zl, vlo[0][i0]) > fminf(zh, vhi[0][i0]));
+        if (i0 < tcnt[0] && i1 < tcnt[1] && i2 < tcnt[2] && !dead)
+          mask |= 1u << ((i0 * tcnt[1] + i1) * tcnt[2] + i2);
+      }
+    }
+  }
+  return mask;
+}
+
 // Record of one Gaussian.  The quadratic form of the reference,
 // q = d^T M d with d = (x - c) / hs, is q = u^T A u in voxel units u = x - c
 // with A = sym(M) / (hs hs^T).  The record keeps q' = -log2(e)/2 q (so that
@@ -1018,13 +1161,21 @@ __device__ __forceinline__ void prep_one(int g, const float c[3], const float mm
   // Column-bound projections in voxel units (C = A^-1, cofactors of A):
   //   al = C_0i / C_ii = cof_0i / cof_ii,  ga = sqrt(C_00 C_ii - C_0i^2) / C_ii
   //   = sqrt(A_jj det A) / cof_ii (Jacobi), rho_i = e_i - 0.01 = R sqrt(C_ii),
-  //   ta = al rho_i^2 / rho_0; for i = H (j = W) and i = W (j = H).
+  //   ta = al rho_i^2 / rho_0; for i = H (j = W) and i = W (j = H).  For the
+  // binning's tile test, also the projection on (H, W): the H range along
+  // u_2 = t, al = cof_12 / cof_22, ga = sqrt(A_00 det A) / cof_22,
+  // ta = al rho_2^2 / rho_1.
+  ZRec z;
+  ZAxis zhw;
   {
     const double c00 = A11 * A22 - A12 * A12, c11 = A00 * A22 - A02 * A02, c22 = A00 * A11 - A01 * A01;
-    const double c01 = A02 * A12 - A01 * A22, c02 = A01 * A12 - A02 * A11;
+    const double c01 = A02 * A12 - A01 * A22, c02 = A01 * A12 - A02 * A11, c12 = A01 * A02 - A00 * A12;
     const double dA = A00 * c00 + A01 * c01 + A02 * c02;
     const double r0 = (double)(e0 - 0.01f), r1 = (double)(e1 - 0.01f), r2 = (double)(e2 - 0.01f);
-    ZRec z;
+    zhw.al = (float)(c12 / c22);
+    zhw.ga = (float)(sqrt(A00 * dA) / c22);
+    zhw.rho = (float)r2;
+    zhw.ta = (float)(c12 / c22 * (r2 * r2 / r1));
     z.h.al = (float)(c01 / c11);
     z.h.ga = (float)(sqrt(A22 * dA) / c11);
     z.h.rho = (float)r1;
@@ -1037,13 +1188,24 @@ __device__ __forceinline__ void prep_one(int g, const float c[3], const float mm
                     isfinite(r1) && isfinite(r2) && isfinite(z.h.al) && isfinite(z.h.ga) && isfinite(z.h.ta) &&
                     isfinite(z.w.al) && isfinite(z.w.ga) && isfinite(z.w.ta);
     if (!ok) z.h.rho = z.w.rho = -1.0f;
+    // The (H, W) projection bounds nothing unless it, too, is finite.
+    if (!(ok && isfinite(zhw.al) && isfinite(zhw.ga) && isfinite(zhw.ta) && zhw.rho > 0.0f)) zhw.rho = -1.0f;
     zt[g] = z;
   }
   if (counts != nullptr) {
     const float tc[3] = {r.c0, r.c1, r.c2}, te[3] = {e0, e1, e2};
     int tmin[3], tcnt[3];
     tile_box_v(c, rad, geo, tc, te, tmin, tcnt);
-    counts[g] = tcnt[0] * tcnt[1] * tcnt[2];
+    int cnt = tcnt[0] * tcnt[1] * tcnt[2];
+    if (geo.live != nullptr) {
+      uint32_t mask = 0;
+      if (cnt > 0 && cnt <= kLiveMaxTiles) {
+        mask = live_tiles(r, z, zhw, geo, tmin, tcnt);
+        cnt = __popc(mask);
+      }
+      geo.live[g] = mask;
+    }
+    counts[g] = cnt;
   }
 }
 
@@ -1063,47 +1225,6 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ center, 
   const float c[3] = {center[3 * (int64_t)g + 0], center[3 * (int64_t)g + 1], center[3 * (int64_t)g + 2]};
   prep_one(g, c, mm, half_shape[0], half_shape[1], half_shape[2], rho_r, rho_i, rho_stride, rec, zt,
            counts != nullptr ? radius[g] : 0.0f, geo, counts);
-}
-
-// ------------------------------------------------ D extent over a column --
-//
-// The D rows of a voxel column (H in [y0, y1], W in [x0, x1]) that can hold a
-// voxel of the widened ellipsoid E = {u : u^T A u <= R^2} (R = 3.06, the
-// radius of the extents e_a = R sqrt(C_aa) + 0.01, C = A^-1) lie in the
-// intersection of two intervals: the D extent of E cut by the H strip alone
-// and by the W strip alone.  Cutting by the strip u_i in [lo, hi] is the same
-// as cutting E's projection on (u_0, u_i), an ellipse with matrix C_sub^-1;
-// along u_i = t that ellipse spans
-//     u_0 in al t -/+ ga sqrt(rho^2 - t^2),
-//     al = C_0i / C_ii,  ga = sqrt(C_00 C_ii - C_0i^2) / C_ii,  rho = R sqrt(C_ii),
-// a concave upper and a convex lower boundary whose extremes lie at
-// t = +/- ta, ta = al rho^2 / rho_0.  Over [lo, hi] the maximum of u_0 is
-// therefore at clamp(ta, lo, hi) and the minimum at clamp(-ta, lo, hi).
-// With cofactors of the scaled form A' = kq A (C_ii = cof_ii / det and, by
-// Jacobi's identity, C_00 C_ii - C_0i^2 = A_jj / det A):
-//     al = cof_0i / cof_ii,  ga = sqrt(A'_jj det A') / cof_ii.
-// The determinant cancels badly for needle-shaped Gaussians, so cofactors and
-// determinant are formed in fp64 from the fp32 form the kernels evaluate.
-// D extent [zmin, zmax] (relative to the centre) of the projected ellipse over
-// the strip u_i in [lo, hi].  Clamps are single med3 instructions: with a
-// non-finite strip (non-finite centre) the result is arbitrary, which is
-// harmless -- such a Gaussian has q = NaN or -inf everywhere and contributes
-// nothing.
-__device__ __forceinline__ void z_span(const ZAxis& a, float lo, float hi, float& zmin, float& zmax) {
-  const float tp = __builtin_amdgcn_fmed3f(a.ta, lo, hi), tm = __builtin_amdgcn_fmed3f(-a.ta, lo, hi);
-  const float rr = a.rho * a.rho;
-  const float sp = __builtin_amdgcn_sqrtf(__builtin_amdgcn_fmed3f(fmaf(-tp, tp, rr), 0.0f, INFINITY));
-  const float sm = __builtin_amdgcn_sqrtf(__builtin_amdgcn_fmed3f(fmaf(-tm, tm, rr), 0.0f, INFINITY));
-  zmax = fmaf(a.ga, sp, a.al * tp);
-  zmin = fmaf(-a.ga, sm, a.al * tm);
-}
-
-// Absolute D rows [zl, zh] of a column from the strip extents, clipped to the
-// Gaussian's box rows [lo0, hi0]; a NaN bound falls back to the box.
-__device__ __forceinline__ void column_rows(float c0, float hmin, float hmax, float wmin, float wmax, float lo0,
-                                            float hi0, float& zl, float& zh) {
-  zl = fmaxf(ceilf(c0 + fmaxf(hmin, wmin) - 0.01f), lo0);
-  zh = fminf(floorf(c0 + fminf(hmax, wmax) + 0.01f), hi0);
 }
 
 // ------------------------------------------------------------ tile kernels --
@@ -1676,15 +1797,20 @@ __host__ __device__ constexpr int grad_ys(int t) { return t + 2; }
 __host__ __device__ constexpr int grad_zs(int t) { return t * (t + 2) + 2; }
 
 // MC: live columns a pair can have (column blocks per tile).
+// Pairs staged per batch of the backward.  Measured on the microbenchmark's
+// fused-call lists (≈ 580 pairs per tile): 384 gives 1.30 ms against 1.32-1.39
+// ms for 320-512 and for two equal batches.
+constexpr int kBwdBatch = 384;
+
 template <int MC>
 struct BwdStage {
-  float c0[kBatch], c1[kBatch], c2[kBatch], rr[kBatch], ri[kBatch];
-  float y1h[kBatch], y1d[kBatch], y1w[kBatch], y2d[kBatch], y2w[kBatch], y3w[kBatch];
-  int32_t gid[kBatch];            // Gaussian id of each staged pair
-  uint16_t cols[kBatch][MC];      // live columns in (H, W) block order
-  uint16_t key[kBatch];           // (columns - 1) * 16 + largest row count - 1, or kNoKey
-  uint16_t order[kBatch];         // batch indices sorted by key
-  uint16_t slot[kBatch];          // rank inside the key
+  float c0[kBwdBatch], c1[kBwdBatch], c2[kBwdBatch], rr[kBwdBatch], ri[kBwdBatch];
+  float y1h[kBwdBatch], y1d[kBwdBatch], y1w[kBwdBatch], y2d[kBwdBatch], y2w[kBwdBatch], y3w[kBwdBatch];
+  int32_t gid[kBwdBatch];            // Gaussian id of each staged pair
+  uint16_t cols[kBwdBatch][MC];      // live columns in (H, W) block order
+  uint16_t key[kBwdBatch];           // (columns - 1) * 16 + largest row count - 1, or kNoKey
+  uint16_t order[kBwdBatch];         // batch indices sorted by key
+  uint16_t slot[kBwdBatch];          // rank inside the key
   int key_cnt[kKeys];
   int key_start[kKeys];
   int seg_begin[kMaxCols];        // per column count cc (index cc - 1): first sorted position
@@ -1850,16 +1976,17 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
   // Records of the next batch are loaded into registers while the current
   // batch is evaluated (loads stay in flight across the LDS barriers).
   const bool use_cols = pair_cols != nullptr && nb <= 2;
+  const int bsz = kBwdBatch;
   GRec nxt = {};
   int nxt_id = 0;
   uint32_t nxt_cols = 0;
-  if (tid < min(kBatch, b.p_end - b.p_begin)) {
+  if (tid < min(bsz, b.p_end - b.p_begin)) {
     nxt_id = pair_ids[b.p_begin + tid];
     nxt = rec[nxt_id];
     if (use_cols) nxt_cols = pair_cols[b.p_begin + tid];
   }
-  for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
-    const int cnt = min(kBatch, b.p_end - bs);
+  for (int bs = b.p_begin; bs < b.p_end; bs += bsz) {
+    const int cnt = min(bsz, b.p_end - bs);
     __syncthreads();
     // Stage: parameters, live columns and key of every pair of the batch.
     if (tid == 0) sh.next_group = 0;
@@ -1951,8 +2078,8 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
       sh.key[tid] = key;
     }
     {
-      const int bn = bs + kBatch;
-      if (bn < b.p_end && tid < min(kBatch, b.p_end - bn)) {
+      const int bn = bs + bsz;
+      if (bn < b.p_end && tid < min(bsz, b.p_end - bn)) {
         nxt_id = pair_ids[bn + tid];
         nxt = rec[nxt_id];
         if (use_cols) nxt_cols = pair_cols[bn + tid];
@@ -2592,7 +2719,8 @@ void prof_stop(int kind, int slot, hipStream_t s) {
 // code `scratch` names a host-side stand-in, so kernels get the device address
 // from hipGetSymbolAddress, resolved once per device with that device current.
 constexpr size_t kNoAllocScratchBytes =
-    (size_t)GSTV_NOALLOC_MAX_GAUSSIANS * (sizeof(GRec) + sizeof(ZRec) + kAccStride * sizeof(float)) + (1u << 16);
+    (size_t)GSTV_NOALLOC_MAX_GAUSSIANS * (sizeof(GRec) + sizeof(ZRec) + kAccStride * sizeof(float) + sizeof(uint32_t)) +
+    (1u << 16);
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 constexpr int kMaxDevices = 64;
@@ -2650,6 +2778,7 @@ int bin_count_impl(const GRec* rec, const float* center_vox, const float* radius
                    void* workspace, size_t workspace_bytes, void* stream) {
   BinGeom geo;
   geo.rec = rec;
+  geo.live = nullptr;
   int64_t num_tiles;
   if (n < 0 || gauss_offsets == nullptr || total_pairs == nullptr ||
       !tile_geometry(d, h, w, tile, geo.nt, &num_tiles))
@@ -2691,12 +2820,13 @@ size_t gstv_bin_fill_workspace_bytes(int32_t n, int64_t total_pairs, int32_t num
 namespace {
 // total_dev null: total_pairs is the pair total.  Otherwise total_pairs is a
 // capacity and the kernels read the total from *total_dev (see PairTotal).
-int bin_fill_impl(const GRec* rec, const float* center_vox, const float* radius_vox, int32_t n, int32_t d,
-                  int32_t h, int32_t w, int32_t tile, const int32_t* gauss_offsets, int64_t total_pairs,
+int bin_fill_impl(const GRec* rec, uint32_t* live, const float* center_vox, const float* radius_vox, int32_t n,
+                  int32_t d, int32_t h, int32_t w, int32_t tile, const int32_t* gauss_offsets, int64_t total_pairs,
                   const int64_t* total_dev, int32_t* tile_ranges, int32_t* pair_ids, void* workspace,
                   size_t workspace_bytes, void* stream) {
   BinGeom geo;
   geo.rec = rec;
+  geo.live = live;
   int64_t num_tiles64;
   if (n < 0 || total_pairs < 0 || tile_ranges == nullptr || !tile_geometry(d, h, w, tile, geo.nt, &num_tiles64))
     return GSTV_ERR_INVALID_ARGUMENT;
@@ -2797,7 +2927,7 @@ int gstv_bin_fill(const float* center_vox, const float* radius_vox, int32_t n, i
                   int32_t w, int32_t tile, const int32_t* gauss_offsets, int64_t total_pairs,
                   int32_t* tile_ranges, int32_t* pair_ids, void* workspace, size_t workspace_bytes,
                   void* stream) {
-  return bin_fill_impl(nullptr, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, total_pairs, nullptr,
+  return bin_fill_impl(nullptr, nullptr, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, total_pairs, nullptr,
                        tile_ranges, pair_ids, workspace, workspace_bytes, stream);
 }
 
@@ -2884,7 +3014,8 @@ struct KeepLayout {
   GRec* rec;
   ZRec* zt;
   float* gacc;
-  int64_t* sums;  // block sums of the pair-count scan
+  int64_t* sums;   // block sums of the pair-count scan
+  uint32_t* live;  // live-tile masks of the binning (BinGeom::live)
 };
 size_t keep_layout(void* base, int32_t n, KeepLayout* out) {
   Carve cv(base);
@@ -2892,7 +3023,8 @@ size_t keep_layout(void* base, int32_t n, KeepLayout* out) {
   ZRec* zt = cv.take<ZRec>((size_t)n);
   float* gacc = cv.take<float>((size_t)n * kAccStride);
   int64_t* sums = cv.take<int64_t>((size_t)ceil_div_l((int64_t)n + 1, kScanTile) + 1);
-  if (out != nullptr) *out = KeepLayout{rec, zt, gacc, sums};
+  uint32_t* live = cv.take<uint32_t>((size_t)n);
+  if (out != nullptr) *out = KeepLayout{rec, zt, gacc, sums, live};
   return align_up(cv.off, 256);
 }
 
@@ -3017,6 +3149,7 @@ int gstv_keep_prepare(const float* center_vox, const float* radius_vox, const fl
   hipStream_t s = static_cast<hipStream_t>(stream);
   KeepLayout ws;
   keep_layout(keep_workspace, n, &ws);
+  if (count) geo.live = ws.live;
   int st = launch_prep(center_vox, cov_inv, rho_r, rho_i, rho_stride, half_shape, n, ws.rec, ws.zt, s, radius_vox,
                        &geo, count ? gauss_offsets : nullptr);
   if (st != GSTV_OK || !count) return st;
@@ -3047,6 +3180,7 @@ int gstv_keep_prologue(const float* positions, const float* scales, const float*
   hipStream_t s = static_cast<hipStream_t>(stream);
   KeepLayout ws;
   keep_layout(keep_workspace, n, &ws);
+  geo.live = ws.live;
   hipLaunchKernelGGL(k_prologue_prep, dim3(ceil_div_i(n + 1, 256)), dim3(256), 0, s, positions, scales, rotations, n,
                      (float)d, (float)h, (float)w, center_vox, radius_vox, cov_inv, half_shape, rho_r, rho_i,
                      (int)rho_stride, ws.rec, ws.zt, geo, gauss_offsets, ws.gacc);
@@ -3073,9 +3207,9 @@ int gstv_keep_bin_fill(const void* keep_workspace, size_t keep_workspace_bytes, 
   if (n < 0) return GSTV_ERR_INVALID_ARGUMENT;
   if (n > 0 && (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n)))
     return GSTV_ERR_WORKSPACE_TOO_SMALL;
-  KeepLayout ws{nullptr, nullptr, nullptr, nullptr};
+  KeepLayout ws{nullptr, nullptr, nullptr, nullptr, nullptr};
   if (n > 0) keep_layout(const_cast<void*>(keep_workspace), n, &ws);
-  return bin_fill_impl(ws.rec, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, total_pairs, nullptr,
+  return bin_fill_impl(ws.rec, ws.live, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, total_pairs, nullptr,
                        tile_ranges, pair_ids, workspace, workspace_bytes, stream);
 }
 
@@ -3088,9 +3222,9 @@ int gstv_keep_bin_fill_capped(const void* keep_workspace, size_t keep_workspace_
   if (n == 0) capacity = 0;  // no Gaussian, no pair: empty ranges
   if (n > 0 && (keep_workspace == nullptr || keep_workspace_bytes < gstv_keep_workspace_bytes(n)))
     return GSTV_ERR_WORKSPACE_TOO_SMALL;
-  KeepLayout ws{nullptr, nullptr, nullptr, nullptr};
+  KeepLayout ws{nullptr, nullptr, nullptr, nullptr, nullptr};
   if (n > 0) keep_layout(const_cast<void*>(keep_workspace), n, &ws);
-  return bin_fill_impl(ws.rec, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, capacity, total_pairs,
+  return bin_fill_impl(ws.rec, ws.live, center_vox, radius_vox, n, d, h, w, tile, gauss_offsets, capacity, total_pairs,
                        tile_ranges, pair_ids, workspace, workspace_bytes, stream);
 }
 

@@ -197,11 +197,12 @@ def _touching_lists(ranges, pids, rec, shape, tile):
 
 @pytest.mark.parametrize("name,make", ALL_CASES, ids=[n for n, _ in ALL_CASES])
 def test_keep_binning_is_the_touching_pairs(name, make):
-    """gstv_keep_prepare's counts and gstv_keep_bin_fill give exactly the
-    reference's pairs that the tile kernels' box test keeps, in the same order,
-    and the forward on those lists equals the forward on the reference's lists
-    bit for bit (these cases' tile lists fit one 512-pair batch, so the sums
-    run in the same order)."""
+    """gstv_keep_prepare's counts and gstv_keep_bin_fill give, per tile, a
+    subsequence of the reference's pairs that the tile kernels' box test keeps
+    (the binning also drops pairs whose tile holds no voxel of the widened
+    ellipsoid); the kernels' active evaluations (q <= 9) on those lists are
+    exactly those on the reference's lists, and the forward and the adjoint
+    agree with the reference lists' to fp32 summation order."""
     from gs_tmi_amd import _native
 
     c = make()
@@ -219,13 +220,23 @@ def test_keep_binning_is_the_touching_pairs(name, make):
     n = center.shape[0]
     rec = kept.ws[: n * 64].view(torch.float32).view(n, 16).cpu().numpy()
     e_ranges, e_pids = _touching_lists(ranges, pids, rec, c["shape"], tile)
-    np.testing.assert_array_equal(t_ranges.cpu().numpy(), e_ranges)
-    np.testing.assert_array_equal(t_pids.cpu().numpy(), e_pids)
-    print(f"  pairs: reference {len(pids)}, touching {len(e_pids)}")
+    k_ranges, k_pids = t_ranges.cpu().numpy(), t_pids.cpu().numpy()
+    for t in range(len(e_ranges)):
+        full = e_pids[e_ranges[t, 0]:e_ranges[t, 1]]
+        sub = k_pids[k_ranges[t, 0]:k_ranges[t, 1]]
+        at = np.searchsorted(full, sub)  # lists ascend in Gaussian id
+        assert np.all(at < len(full)) and np.array_equal(full[np.minimum(at, len(full) - 1)], sub), t
+        assert np.all(np.diff(sub) > 0)
+    print(f"  pairs: reference {len(pids)}, touching {len(e_pids)}, kept {len(k_pids)}")
     r_t, p_t = torch.from_numpy(ranges).to(DEV), torch.from_numpy(pids).to(DEV)
+    e_ref = _native.count_evaluations(center, cov, rr, ri, r_t, p_t, hs, tile, d, h, w)
+    e_keep = _native.count_evaluations(center, cov, rr, ri, t_ranges, t_pids, hs, tile, d, h, w)
+    assert e_keep[1] == e_ref[1] and e_keep[2] <= e_ref[2] and e_keep[3] <= e_ref[3]
     vol_r, vol_i = _native.forward(center, cov, rr, ri, r_t, p_t, hs, tile, d, h, w)
     kvol_r, kvol_i, kept = _native.forward(center, cov, rr, ri, t_ranges, t_pids, hs, tile, d, h, w, keep=kept)
-    assert torch.equal(vol_r, kvol_r) and torch.equal(vol_i, kvol_i)
+    vol = torch.cat([vol_r, vol_i]).cpu().numpy()
+    kvol = torch.cat([kvol_r, kvol_i]).cpu().numpy()
+    assert cases.rel_l2(kvol, vol) < 1e-6
     rng = np.random.default_rng(10)
     g_r = torch.from_numpy(rng.normal(size=d * h * w).astype(np.float32)).to(DEV)
     g_i = torch.from_numpy(rng.normal(size=d * h * w).astype(np.float32)).to(DEV)
