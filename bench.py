@@ -282,7 +282,7 @@ def main():
         center, radius, cov, hs = voxel_space_params(params["pos"], torch.exp(params["log_s"]), params["rot"],
                                                      SHAPE)
         ranges, pids = build_tile_pairs(center, radius, SHAPE, TILE)
-        e_total, e_active, e_culled, e_adjoint = _native.count_evaluations(
+        e_total, e_active, e_culled_ref, e_adjoint_ref = _native.count_evaluations(
             center, cov.contiguous(), params["dr"].detach(), params["di"].detach(), ranges, pids, hs, TILE, *SHAPE)
         pairs = int(pids.numel())
         # Binning alone, for the breakdown.
@@ -308,12 +308,20 @@ def main():
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
             kept = _native.keep_prepare(center, cov.contiguous(), dv[:, 0], dv[:, 1], hs, radius, SHAPE, TILE)
-            _native.bin_fill(center, radius, SHAPE, TILE, kept.offsets, cap.capacity(N_GAUSS), kept=kept,
-                             capped=True)
+            k_ranges, k_pids = _native.bin_fill(center, radius, SHAPE, TILE, kept.offsets, cap.capacity(N_GAUSS),
+                                                kept=kept, capped=True)
             e1.record()
             torch.cuda.synchronize()
             kbt.append(e0.elapsed_time(e1))
-        del kept, dv
+        # The evaluations the tile kernels perform on the fused call's lists
+        # (the reference's lists hold pairs that add nothing; E_active is the
+        # same on both).
+        _, e_active_kept, e_culled, e_adjoint = _native.count_evaluations(
+            center, cov.contiguous(), params["dr"].detach(), params["di"].detach(), k_ranges, k_pids, hs, TILE,
+            *SHAPE)
+        if e_active_kept != e_active:
+            log(f"[bench] active evaluations differ between the lists: {e_active_kept} vs {e_active}")
+        del kept, dv, k_ranges, k_pids
     log(f"[bench] pairs {pairs}, E_total {e_total:.4e}, E_active {e_active:.4e}, E_culled {e_culled:.4e}, "
         f"E_adjoint {e_adjoint:.4e}")
 
@@ -417,6 +425,9 @@ def main():
             "E_active": e_active,
             "E_culled": e_culled,
             "E_adjoint": e_adjoint,
+            "E_active_fused_lists": e_active_kept,
+            "E_culled_reference_lists": e_culled_ref,
+            "E_adjoint_reference_lists": e_adjoint_ref,
             "forward_executed_tflops": fwd_flop / (fwd_ms * 1e-3) / 1e12,
             "forward_roofline_frac": fwd_flop / (fwd_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
             "reference_evals_per_s_fwd": e_total / (fwd_ms * 1e-3),
