@@ -187,11 +187,13 @@ int gstv_backward_noalloc(const float* center_vox, const float* cov_inv,
  *                       of gstv_bin_count restricted to the pairs whose tile
  *                       meets the Gaussian's extent box (the box of its q <= 9
  *                       ellipsoid in the record, widened so that no voxel with
- *                       fp32 q <= 9 lies outside).  The tile kernels skip every
- *                       other pair of the reference's lists, so the forward and
- *                       the adjoint on such lists make the same evaluations;
- *                       the forward's fp32 sums differ only in order where a
- *                       tile's list spans more than one batch of 512 pairs.
+ *                       fp32 q <= 9 lies outside) and, for tile boxes of at
+ *                       most 32 tiles (4 per axis), can hold a voxel of that
+ *                       widened ellipsoid by its (D, H), (D, W) and (H, W)
+ *                       projections over the tile.  Every dropped pair adds
+ *                       nothing to the reference's result, so the forward and
+ *                       the adjoint on such lists make the same active
+ *                       evaluations; the forward's fp32 sums differ in order.
  *   gstv_keep_prologue  gstv_prologue and gstv_keep_prepare (with the counts)
  *                       in one pass; the same outputs.  It also zeroes the
  *                       adjoint's moment accumulator (gstv_keep_clear).
