@@ -1760,7 +1760,7 @@ __device__ __forceinline__ float pair_sum_rows01(float a, float b) {
 }
 
 constexpr int kMoments = 11;  // rho_r, rho_i, 3 first moments, 6 second moments
-constexpr int kAccStride = 12;
+constexpr int kAccStride = 16;  // floats per accumulator row: one 64-byte row per Gaussian
 
 // ---- backward ----
 //
@@ -2177,8 +2177,8 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
                                          rr, ri, y1h, y1d, y1w, y2d, y2w, y3w, acc);
           }
         }
-        const float v[kAccStride] = {acc.dr,  acc.di,  acc.m0,  acc.m1,  acc.m2,  acc.m00,
-                                     acc.m11, acc.m22, acc.m01, acc.m02, acc.m12, 0.0f};
+        const float v[12] = {acc.dr,  acc.di,  acc.m0,  acc.m1,  acc.m2,  acc.m00,
+                             acc.m11, acc.m22, acc.m01, acc.m02, acc.m12, 0.0f};
         // Reduce-scatter over the 4 W lanes of each Gaussian: afterwards the
         // lane in row xi holds the totals of moments 4*jj + xi, jj = 0..2.
         float t3[3];
@@ -2188,14 +2188,27 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
           const float hi = pair_sum_rows02(v[4 * jj + 1], v[4 * jj + 3]);  // rows 0,1: m 4jj+1; rows 2,3: m 4jj+3
           t3[jj] = pair_sum_rows01(lo, hi);
         }
-        // Each lane adds its 3 totals straight into the Gaussian's 48-byte row:
-        // one atomic instruction covers 16 rows with 4 consecutive floats each,
-        // and the atomics drain while the wave evaluates its next group.
-        if (slot_ok) {
-          float* row = gacc + (int64_t)sh.gid[k] * kAccStride + xi;
+        // The totals go to the Gaussians' 64-byte rows by whole rows: atomic
+        // instruction i covers Gaussians 4i .. 4i + 3 of the group, lane L
+        // adding moment L % 16 of Gaussian 4i + L / 16, so that it touches 4
+        // cache lines instead of 16 (the L2 handles a float atomic instruction
+        // per line it touches).  Moment m of Gaussian g sits in register t3[m / 4]
+        // of lane 16 (m % 4) + g; lane g holds the Gaussian's id.  The atomics
+        // drain while the wave evaluates its next group.
+        {
+          const int m = lane & 15, jm = m >> 2;
+          const int id = slot_ok ? sh.gid[k] : -1;
 #pragma unroll
-          for (int jj = 0; jj < 3; ++jj)
-            if (4 * jj + xi < kMoments && t3[jj] != 0.0f) unsafeAtomicAdd(row + 4 * jj, t3[jj]);
+          for (int i = 0; i < 4; ++i) {
+            const int g = 4 * i + (lane >> 4);
+            const int src = ((m & 3) * 16 + g) * 4;
+            const float a0 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(t3[0])));
+            const float a1 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(t3[1])));
+            const float a2 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(t3[2])));
+            const float val = jm == 0 ? a0 : (jm == 1 ? a1 : a2);
+            const int gid = __builtin_amdgcn_ds_bpermute(g * 4, id);
+            if (m < kMoments && gid >= 0 && val != 0.0f) unsafeAtomicAdd(gacc + (int64_t)gid * kAccStride + m, val);
+          }
         }
       }
     }
@@ -3008,7 +3021,7 @@ void launch_backward(const GRec* rec, const ZRec* zt, float* gacc, const float* 
                        half_shape, n, grad_center, grad_cov_inv, grad_rho_r, grad_rho_i, rho_stride);
 }
 
-// Keep workspace: the records, the column-bound records, then the 48-byte
+// Keep workspace: the records, the column-bound records, then the 64-byte
 // moment rows.
 struct KeepLayout {
   GRec* rec;

@@ -156,7 +156,7 @@ int gstv_backward(const float* center_vox, const float* cov_inv,
  * loops, harnesses that forbid hipMalloc.  gstv_backward with the same
  * arguments and results (up to the order of fp32 atomic additions), without
  * the workspace: it keeps its workspace in a static device array of the
- * library, 38.9 MB (GSTV_NOALLOC_MAX_GAUSSIANS x 148 B + 64 KiB) on every
+ * library, 43.1 MB (GSTV_NOALLOC_MAX_GAUSSIANS x 164 B + 64 KiB) on every
  * GPU it runs on, held until the process ends.
  * It allocates nothing and makes no memset (its kernels rewrite every part of
  * the array they read).
