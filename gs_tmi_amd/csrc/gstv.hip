@@ -1857,7 +1857,12 @@ __device__ __forceinline__ void bwd_column(LdsFloat* __restrict__ sh_g, int tp, 
     u1q[y] = u1v[y] * u1v[y];
   }
   // The next row's upstream gradient is loaded from LDS while the current row
-  // is evaluated; rows past the lane's own re-read its last row.
+  // is evaluated.  With the zero slabs past the tile (kZeroSlabs), rows past
+  // the lane's own need no mask: inside the tile the column bound leaves no
+  // voxel there whose q passes the cutoff, and past it the gradient is zero
+  // (the weights are at most 1).  Otherwise, and on the select path, where a
+  // density may be infinite, they get q = -inf and re-read the lane's last row.
+  constexpr bool kZeroSlabs = TT == 8;
   const int zlast_off = (z0 + own_rows - 1) * ZS;
   int zn_off = (z0 + 1) * ZS;
   float2 gn[kSub];
@@ -1871,12 +1876,12 @@ __device__ __forceinline__ void bwd_column(LdsFloat* __restrict__ sh_g, int tp, 
     const float p1 = fmaf(y1d, u0, g1);
     const float y2 = fmaf(y2d, u0, h2);
     float rz = fmaf(-y2, y2, -c3);
-    rz = r < own_rows ? rz : -INFINITY;
+    if (!kZeroSlabs || kSafe) rz = r < own_rows ? rz : -INFINITY;
     float2 gc[kSub];
 #pragma unroll
     for (int y = 0; y < kSub; ++y) gc[y] = gn[y];
     {
-      const int nb = gbase + min(zn_off, zlast_off);
+      const int nb = gbase + ((kZeroSlabs && !kSafe) ? zn_off : min(zn_off, zlast_off));
       zn_off += ZS;
 #pragma unroll
       for (int y = 0; y < kSub; ++y) gn[y] = lds_g(sh_g, nb + y * YS);
@@ -1958,7 +1963,10 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
   const bool partial = b.e1 < T || b.e2 < T;  // D rows are always clipped to the tile
 
   int g_bad = 0;
-  for (int i = tid; i < T * T * T; i += kTileThreads) {
+  // For 8-voxel tiles the copy has 8 more slabs of zeros past the tile: a
+  // group's row walk from any first row stays inside it.
+  const int copy_slabs = TT == 8 ? 2 * T : T;
+  for (int i = tid; i < copy_slabs * T * T; i += kTileThreads) {
     const int z = i / (T * T), y = (i / T) % T, x = i % T;
     float2 g = make_float2(0.0f, 0.0f);
     if (z < b.e0 && y < b.e1 && x < b.e2) {
@@ -2158,8 +2166,10 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
         for (int j = 0; j < cc; ++j) {
           const uint16_t cw = sh.cols[k][j];
           const int own_rows = col_rows(cw);
-          // The group's largest row count for its j-th columns (over the 16 slots).
-          const int rows = row16_max(own_rows);
+          // The group's largest row count for its j-th columns (over the 16 slots;
+          // every row of the wave holds the same 16 Gaussians, so it is uniform
+          // and the row loop counts in scalar registers).
+          const int rows = __builtin_amdgcn_readfirstlane(row16_max(own_rows));
           const int j1 = cw & 3, j2 = (cw >> 2) & 3, z0 = (cw >> 4) & 15;
           LdsFloat* lg = (LdsFloat*)sh_g;
           if (safe) {
@@ -3007,7 +3017,8 @@ void launch_backward(const GRec* rec, const ZRec* zt, float* gacc, const float* 
   geo_b.chunks = 1;
   geo_b.blocks = geo.num_sub;
   const int tp = geo.nsb * kSub;
-  const size_t lds_grad = (size_t)tp * grad_zs(tp) * sizeof(float2);
+  // 8-voxel tiles: 8 zero slabs past the tile (k_backward<8>).
+  const size_t lds_grad = (size_t)(geo.sub == 8 ? 2 * tp : tp) * grad_zs(tp) * sizeof(float2);
   const uint32_t* cols = geo.nsb <= 2 ? pair_columns : nullptr;
   if (geo.sub == 8)
     hipLaunchKernelGGL(k_backward<8>, dim3(tile_grid(geo_b)), dim3(kTileThreads), lds_grad, s, rec, zt,
