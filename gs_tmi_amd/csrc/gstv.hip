@@ -1246,14 +1246,18 @@ struct VoxGeom {
   int blocks;         // num_sub * chunks
 };
 
+// k_forward stages up to kFwdBatch pairs at a time (two per thread); the
+// counting kernel and the non-finite pass stage kBatch.
+constexpr int kFwdBatch = 2 * kBatch;
+
 struct Stage {
-  float c0[kBatch], c1[kBatch], c2[kBatch], rr[kBatch], ri[kBatch];
-  float y1h[kBatch], y1d[kBatch], y1w[kBatch], y2d[kBatch], y2w[kBatch], y3w[kBatch];
-  int32_t gid[kBatch];  // Gaussian id of each staged pair
-  uint32_t box[kBatch];
-  uint8_t sbm[kBatch];  // sub-bricks touched, bit 4*j0 + 2*j1 + j2 (tiles of at most 2x2x2 sub-bricks)
-  uint16_t list[kWaves][kBatch];
-  uint16_t touch[kBatch];  // pairs whose box touches the tile (any order)
+  float c0[kFwdBatch], c1[kFwdBatch], c2[kFwdBatch], rr[kFwdBatch], ri[kFwdBatch];
+  float y1h[kFwdBatch], y1d[kFwdBatch], y1w[kFwdBatch], y2d[kFwdBatch], y2w[kFwdBatch], y3w[kFwdBatch];
+  int32_t gid[kFwdBatch];  // Gaussian id of each staged pair
+  uint32_t box[kFwdBatch];
+  uint8_t sbm[kFwdBatch];  // sub-bricks touched, bit 4*j0 + 2*j1 + j2 (tiles of at most 2x2x2 sub-bricks)
+  uint16_t list[kWaves][kFwdBatch];
+  uint16_t touch[kFwdBatch];  // pairs whose box touches the tile (any order)
   int n_touch;
   // A staged pair touching the tile had a non-finite density (its staged
   // density was zeroed; k_forward adds it after the batches).  Set during
@@ -1697,9 +1701,14 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_forward(const GRec* __restr
     sh.n_touch = 0;
     sh.any_bad = 0;
   }
-  for (int bs = b.p_begin; bs < b.p_end; bs += kBatch) {
-    const int cnt = min(kBatch, b.p_end - bs);
+  for (int bs = b.p_begin; bs < b.p_end; bs += kFwdBatch) {
+    const int cnt = min(kFwdBatch, b.p_end - bs);
     __syncthreads();
+    // The second half of the batch (thread t stages pair kBatch + t) is read
+    // here; the first was loaded during the previous batch.
+    const int t2 = threadIdx.x + kBatch;
+    GRec r2 = {};
+    if (t2 < cnt) r2 = rec[pair_ids[bs + t2]];
     const bool touch = (int)threadIdx.x < cnt && stage_pair(sh, threadIdx.x, nxt, nxt.gid, b);
     if (touch && (non_finite(nxt.rr) || non_finite(nxt.ri))) {
       // Zero weights times a non-finite density would be NaN: such a
@@ -1709,10 +1718,17 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_forward(const GRec* __restr
       sh.ri[threadIdx.x] = 0.0f;
       sh.any_bad = 1;
     }
-    const int bn = bs + kBatch;
+    const int bn = bs + kFwdBatch;
     if (bn < b.p_end && (int)threadIdx.x < min(kBatch, b.p_end - bn)) nxt = rec[pair_ids[bn + threadIdx.x]];
+    const bool touch2 = t2 < cnt && stage_pair(sh, t2, r2, r2.gid, b);
+    if (touch2 && (non_finite(r2.rr) || non_finite(r2.ri))) {
+      sh.rr[t2] = 0.0f;
+      sh.ri[t2] = 0.0f;
+      sh.any_bad = 1;
+    }
     if (use_mask) {
       append_touch(sh, touch, threadIdx.x, lane);
+      append_touch(sh, touch2, t2, lane);
       __syncthreads();
       batch_masks(sh, b, bs, zt, pair_cols);
     }
@@ -1797,20 +1813,25 @@ __host__ __device__ constexpr int grad_ys(int t) { return t + 2; }
 __host__ __device__ constexpr int grad_zs(int t) { return t * (t + 2) + 2; }
 
 // MC: live columns a pair can have (column blocks per tile).
-// Pairs staged per batch of the backward.  Measured on the microbenchmark's
-// fused-call lists (≈ 580 pairs per tile): 384 gives 1.30 ms against 1.32-1.39
-// ms for 320-512 and for two equal batches.
-constexpr int kBwdBatch = 384;
+// Pairs staged per batch of the backward.  8-voxel tiles stage up to two per
+// thread: most tiles of the microbenchmark's fused-call lists (≈ 580 pairs per
+// tile) then take one batch and their Gaussians are grouped together.  Larger
+// tiles keep 384, so that their bigger LDS gradient copy still leaves room for
+// two blocks per CU.
+template <int TT>
+constexpr int bwd_batch() {
+  return TT == 8 ? 2 * kTileThreads : 384;
+}
 
-template <int MC>
+template <int MC, int NB>
 struct BwdStage {
-  float c0[kBwdBatch], c1[kBwdBatch], c2[kBwdBatch], rr[kBwdBatch], ri[kBwdBatch];
-  float y1h[kBwdBatch], y1d[kBwdBatch], y1w[kBwdBatch], y2d[kBwdBatch], y2w[kBwdBatch], y3w[kBwdBatch];
-  int32_t gid[kBwdBatch];            // Gaussian id of each staged pair
-  uint16_t cols[kBwdBatch][MC];      // live columns in (H, W) block order
-  uint16_t key[kBwdBatch];           // (columns - 1) * 16 + largest row count - 1, or kNoKey
-  uint16_t order[kBwdBatch];         // batch indices sorted by key
-  uint16_t slot[kBwdBatch];          // rank inside the key
+  float c0[NB], c1[NB], c2[NB], rr[NB], ri[NB];
+  float y1h[NB], y1d[NB], y1w[NB], y2d[NB], y2w[NB], y3w[NB];
+  int32_t gid[NB];            // Gaussian id of each staged pair
+  uint16_t cols[NB][MC];      // live columns in (H, W) block order
+  uint16_t key[NB];           // (columns - 1) * 16 + largest row count - 1, or kNoKey
+  uint16_t order[NB];         // batch indices sorted by key
+  uint16_t slot[NB];          // rank inside the key
   int key_cnt[kKeys];
   int key_start[kKeys];
   int seg_begin[kMaxCols];        // per column count cc (index cc - 1): first sorted position
@@ -1935,7 +1956,7 @@ __device__ __forceinline__ void bwd_column(LdsFloat* __restrict__ sh_g, int tp, 
 // pair_cols (optional, tiles of at most 8 voxels per axis): the forward's
 // column words of the same pairs, used instead of recomputing the bound.
 template <int TT>
-__global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restrict__ rec,
+__global__ __launch_bounds__(kTileThreads, 4) void k_backward(const GRec* __restrict__ rec,
                                                            const ZRec* __restrict__ zt,
                                                            const int32_t* __restrict__ ranges,
                                                            const int32_t* __restrict__ pair_ids, VoxGeom geo,
@@ -1945,7 +1966,8 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
                                                            const uint32_t* __restrict__ pair_cols) {
   constexpr int NB = TT > 0 ? (TT + kSub - 1) / kSub : kMaxBlk;  // column blocks per axis (bound)
   constexpr int MC = NB * NB;
-  __shared__ BwdStage<MC> sh;
+  constexpr int kB = bwd_batch<TT>();
+  __shared__ BwdStage<MC, kB> sh;
   // Upstream gradient of the tile, TP^3 with TP the tile edge rounded up to a
   // multiple of 4 (strides grad_zs, grad_ys, 1), zero outside the tile's valid
   // voxels.
@@ -1984,7 +2006,7 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
   // Records of the next batch are loaded into registers while the current
   // batch is evaluated (loads stay in flight across the LDS barriers).
   const bool use_cols = pair_cols != nullptr && nb <= 2;
-  const int bsz = kBwdBatch;
+  const int bsz = kB;
   GRec nxt = {};
   int nxt_id = 0;
   uint32_t nxt_cols = 0;
@@ -1996,22 +2018,23 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
   for (int bs = b.p_begin; bs < b.p_end; bs += bsz) {
     const int cnt = min(bsz, b.p_end - bs);
     __syncthreads();
-    // Stage: parameters, live columns and key of every pair of the batch.
+    // Stage: parameters, live columns and key of every pair of the batch;
+    // thread t stages pair t (loaded during the previous batch) and pair
+    // kTileThreads + t (read here).
     if (tid == 0) sh.next_group = 0;
-    if (tid < cnt) {
-      const GRec r = nxt;
-      sh.gid[tid] = nxt_id;
-      sh.c0[tid] = r.c0;
-      sh.c1[tid] = r.c1;
-      sh.c2[tid] = r.c2;
-      sh.rr[tid] = r.rr;
-      sh.ri[tid] = r.ri;
-      sh.y1h[tid] = r.y1h;
-      sh.y1d[tid] = r.y1d;
-      sh.y1w[tid] = r.y1w;
-      sh.y2d[tid] = r.y2d;
-      sh.y2w[tid] = r.y2w;
-      sh.y3w[tid] = r.y3w;
+    auto stage = [&](int t, const GRec& r, int id, uint32_t colw) {
+      sh.gid[t] = id;
+      sh.c0[t] = r.c0;
+      sh.c1[t] = r.c1;
+      sh.c2[t] = r.c2;
+      sh.rr[t] = r.rr;
+      sh.ri[t] = r.ri;
+      sh.y1h[t] = r.y1h;
+      sh.y1d[t] = r.y1d;
+      sh.y1w[t] = r.y1w;
+      sh.y2d[t] = r.y2d;
+      sh.y2w[t] = r.y2w;
+      sh.y3w[t] = r.y3w;
       uint16_t key = kNoKey;
       const float lo0 = fmaxf(ceilf(r.c0 - r.e0), (float)b.o0);
       const float lo1 = fmaxf(ceilf(r.c1 - r.e1), (float)b.o1);
@@ -2023,11 +2046,11 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
       if (use_cols && touch) {
         // Live columns from the forward's column word (written for the pairs
         // whose box touches the tile), in (H, W) block order.
-        uint16_t* cl = sh.cols[tid];
+        uint16_t* cl = sh.cols[t];
         int ncol = 0, mxr = 0;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-          const uint32_t f = (nxt_cols >> (7 * c)) & 127u;
+          const uint32_t f = (colw >> (7 * c)) & 127u;
           if (f & 1u) {
             const int rows = (int)((f >> 4) & 7u) + 1;
             cl[ncol] = pack_col(c >> 1, c & 1, (int)((f >> 1) & 7u), rows);
@@ -2037,14 +2060,14 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
         }
         if (ncol > 0) {
           key = (uint16_t)((ncol - 1) * kMaxRows + mxr - 1);
-          sh.slot[tid] = (uint16_t)atomicAdd(&sh.key_cnt[key], 1);
+          sh.slot[t] = (uint16_t)atomicAdd(&sh.key_cnt[key], 1);
         }
       }
       if (!use_cols && touch) {
         const int y_lo = (int)lo1 - b.o1, y_hi = (int)hi1 - b.o1;  // tile-local box
         const int x_lo = (int)lo2 - b.o2, x_hi = (int)hi2 - b.o2;
         const int l1 = y_lo >> 2, h1 = y_hi >> 2, l2 = x_lo >> 2, h2 = x_hi >> 2;
-        const ZCoefs zc = z_load(zt, nxt_id);
+        const ZCoefs zc = z_load(zt, id);
         float hmn[NB], hmx[NB], wmn[NB], wmx[NB];
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
@@ -2058,7 +2081,7 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
           }
         }
         // Live columns in (H, W) block order, and their largest row count.
-        uint16_t* cl = sh.cols[tid];
+        uint16_t* cl = sh.cols[t];
         int ncol = 0;
         int mxr = 0;
 #pragma unroll
@@ -2080,10 +2103,16 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
         }
         if (ncol > 0) {
           key = (uint16_t)((ncol - 1) * kMaxRows + mxr - 1);
-          sh.slot[tid] = (uint16_t)atomicAdd(&sh.key_cnt[key], 1);
+          sh.slot[t] = (uint16_t)atomicAdd(&sh.key_cnt[key], 1);
         }
       }
-      sh.key[tid] = key;
+      sh.key[t] = key;
+    };
+    if (tid < cnt) stage(tid, nxt, nxt_id, nxt_cols);
+    if (tid + kTileThreads < cnt) {
+      const int t2 = tid + kTileThreads;
+      const int id2 = pair_ids[bs + t2];
+      stage(t2, rec[id2], id2, use_cols ? pair_cols[bs + t2] : 0u);
     }
     {
       const int bn = bs + bsz;
@@ -2132,7 +2161,8 @@ __global__ __launch_bounds__(kTileThreads) void k_backward(const GRec* __restric
       if (lane == 0) sh.n_groups = gin;
     }
     __syncthreads();
-    if (tid < cnt && sh.key[tid] != kNoKey) sh.order[sh.key_start[sh.key[tid]] + sh.slot[tid]] = (uint16_t)tid;
+    for (int t = tid; t < cnt; t += kTileThreads)
+      if (sh.key[t] != kNoKey) sh.order[sh.key_start[sh.key[t]] + sh.slot[t]] = (uint16_t)t;
     __syncthreads();
 
     // Groups are taken in descending cost order (column count, then rows) from
