@@ -24,7 +24,7 @@
 //     the box-level work has no voxel inside the ellipsoid.
 //   * Forward: one 512-thread block per 8^3 tile (one block per 8 sub-bricks
 //     for larger tiles); each wave owns a 4x4x4 sub-brick.  A batch of up to
-//     512 pairs is staged in LDS with the mask of sub-bricks the column bound
+//     1024 pairs is staged in LDS with the mask of sub-bricks the column bound
 //     leaves; every wave compacts its Gaussians (ballot) and evaluates them 16
 //     at a time: lane = (Gaussian slot 0..15, W offset 0..3), each lane walks
 //     the 4x4 (D, H) positions of its W column, so the quadratic form costs
@@ -35,8 +35,9 @@
 //     columns (4x4 (H, W) blocks of its box, each with the D rows the column
 //     bound leaves) with lane = (Gaussian slot, W offset), in groups of 16
 //     Gaussians with equal column counts, keeping its 11 gradient moments in
-//     registers; a lane-swap reduce-scatter over the 4 W lanes leaves 3
-//     totals per lane, added with fp32 atomics into the Gaussian's moment row.
+//     registers; a lane-swap reduce-scatter over the 4 W lanes and a lane
+//     permute leave each atomic instruction 4 whole 64-byte moment rows, added
+//     with fp32 atomics.
 //     A final per-Gaussian kernel maps moments to the reference's outputs
 //     (centre, cov_inv, density gradients).
 
@@ -343,7 +344,7 @@ __device__ __forceinline__ int tile_of(int x, const BinGeom& geo) {
 // that the extent box [ceil(c - e), floor(c + e)] meets: the tile kernels skip
 // every other pair of the tile lists (the box test of stage_pair, with the same
 // float bounds), so dropping them changes no evaluation (only how a tile's
-// pairs fall into the forward's 512-pair batches); a NaN bound restricts
+// pairs fall into the forward's batches); a NaN bound restricts
 // nothing, as there.
 __device__ __forceinline__ void tile_box_v(const float cref[3], float r, const BinGeom& geo, const float* tc,
                                            const float* te, int tmin[3], int tcnt[3]) {
