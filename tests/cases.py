@@ -80,6 +80,11 @@ def edge_case(kind, seed=0):
         hs = np.asarray(shape, np.float64) / 2
         s_vox = np.minimum(np.exp(rng.uniform(np.log(0.05), np.log(30.0), size=(n, 3))), 30.0)
         scales = (s_vox / hs).astype(np.float32)
+    elif kind == "crowded":
+        # About 1,400 Gaussians per tile: the tile kernels stage more than one
+        # batch per tile and two pairs per thread.
+        shape, n, tile = (16, 16, 16), 1500, 8
+        pos, scales, rot, rr, ri = make_gaussians(n, shape, seed, sigma_median=3.0, sigma_log_sd=0.3)
     elif kind == "single":
         shape, n, tile = (16, 16, 16), 1, 8
         pos, scales, rot, rr, ri = make_gaussians(n, shape, seed, sigma_median=2.0, sigma_log_sd=0.2)
@@ -104,7 +109,7 @@ def random_case(seed):
 
 
 EDGE_KINDS = ["partial_tiles", "outside", "tile1", "tile3", "tile5", "tile12", "tile16", "tile17", "tile24", "tile32",
-              "needles", "extreme_aniso",
+              "needles", "extreme_aniso", "crowded",
               "large_sigma", "tiny_sigma", "anisotropic_volume", "single"]
 
 
