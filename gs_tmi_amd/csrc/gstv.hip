@@ -1370,17 +1370,14 @@ __device__ __forceinline__ uint32_t col_field(int z_lo, int z_hi) {
 }
 
 // 2x2x2 mask of the sub-bricks that can hold a voxel of the ellipsoid, for a
-// staged pair whose box touches a tile of at most 2 sub-brick per axis: the
-// box's (H, W) column blocks, each with the D rows its column bound leaves.
-// The same rows are returned as a column word in `cols`.
-__device__ __forceinline__ uint32_t pair_mask(const Stage& sh, int k, const BlockCtx& b, const ZRec* __restrict__ zt,
-                                              uint32_t& cols) {
-  const float c0 = sh.c0[k], c1 = sh.c1[k], c2 = sh.c2[k];
-  const uint32_t bx = sh.box[k];
+// pair (centre c, packed box bx) whose box touches a tile of at most 2
+// sub-brick per axis: the box's (H, W) column blocks, each with the D rows its
+// column bound leaves.  The same rows are returned as a column word in `cols`.
+__device__ __forceinline__ uint32_t pair_mask(float c0, float c1, float c2, uint32_t bx, const ZCoefs& zc,
+                                              const BlockCtx& b, uint32_t& cols) {
   const int l0 = bx & 31, l1 = (bx >> 5) & 31, l2 = (bx >> 10) & 31;
   const int h0 = (bx >> 15) & 31, h1 = (bx >> 20) & 31, h2 = (bx >> 25) & 31;
   const float lo0 = (float)(b.o0 + l0), hi0 = (float)(b.o0 + h0);
-  const ZCoefs zc = z_load(zt, sh.gid[k]);
   float hmn[2], hmx[2], wmn[2], wmx[2];
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
@@ -1426,18 +1423,26 @@ __device__ __forceinline__ void append_touch(Stage& sh, bool touch, int k, int l
 }
 
 // Sub-brick masks of the batch's touching pairs, spread over the block's
-// threads (pairs that do not touch the tile cost nothing here), and, when
-// pair_cols is given, their column words at the pairs' positions bs + k.
-// Called between two barriers.
-__device__ __forceinline__ void batch_masks(Stage& sh, const BlockCtx& b, int bs, const ZRec* __restrict__ zt,
-                                            uint32_t* __restrict__ pair_cols) {
+// threads (pairs that do not touch the tile cost nothing here).  Called
+// between two barriers.  (k_forward, whose lists hold few pairs that miss the
+// tile, computes each mask in the thread that staged the pair instead.)
+__device__ __forceinline__ void batch_masks(Stage& sh, const BlockCtx& b, const ZRec* __restrict__ zt) {
   const int nt = sh.n_touch;
   for (int i = threadIdx.x; i < nt; i += kTileThreads) {
     const int k = sh.touch[i];
     uint32_t cols;
-    sh.sbm[k] = (uint8_t)pair_mask(sh, k, b, zt, cols);
-    if (pair_cols != nullptr) pair_cols[bs + k] = cols;
+    sh.sbm[k] = (uint8_t)pair_mask(sh.c0[k], sh.c1[k], sh.c2[k], sh.box[k], z_load(zt, sh.gid[k]), b, cols);
   }
+}
+
+// Sub-brick mask of a pair staged at k from registers (its record r and
+// column-bound record z), and its column word at bs + k in pair_cols when
+// given.
+__device__ __forceinline__ void stage_mask(Stage& sh, int k, const GRec& r, const ZRec& z, const BlockCtx& b,
+                                           int bs, uint32_t* __restrict__ pair_cols) {
+  uint32_t cols;
+  sh.sbm[k] = (uint8_t)pair_mask(r.c0, r.c1, r.c2, sh.box[k], ZCoefs{z.h, z.w, z.h.rho > 0.0f}, b, cols);
+  if (pair_cols != nullptr) pair_cols[bs + k] = cols;
 }
 
 // Compacts, in batch order, the staged Gaussians whose box touches this
@@ -1696,21 +1701,31 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_forward(const GRec* __restr
   // Records of the next batch are loaded into registers while the current
   // batch is evaluated (loads stay in flight across the LDS barriers).
   GRec nxt = {};
-  if ((int)threadIdx.x < min(kBatch, b.p_end - b.p_begin)) nxt = rec[pair_ids[b.p_begin + threadIdx.x]];
+  int nxt_id = 0;
   const bool use_mask = geo.nsb <= 2;
-  if (threadIdx.x == 0) {
-    sh.n_touch = 0;
-    sh.any_bad = 0;
+  if ((int)threadIdx.x < min(kBatch, b.p_end - b.p_begin)) {
+    nxt_id = pair_ids[b.p_begin + threadIdx.x];
+    nxt = rec[nxt_id];
   }
+  if (threadIdx.x == 0) sh.any_bad = 0;
   for (int bs = b.p_begin; bs < b.p_end; bs += kFwdBatch) {
     const int cnt = min(kFwdBatch, b.p_end - bs);
     __syncthreads();
     // The second half of the batch (thread t stages pair kBatch + t) is read
-    // here; the first was loaded during the previous batch.
+    // here; the first was loaded during the previous batch.  The column-bound
+    // records are read with them: each thread computes the sub-brick masks of
+    // the pairs it stages.
     const int t2 = threadIdx.x + kBatch;
+    ZRec z1 = {}, z2 = {};
+    if (use_mask && (int)threadIdx.x < cnt) z1 = zt[nxt_id];
     GRec r2 = {};
-    if (t2 < cnt) r2 = rec[pair_ids[bs + t2]];
+    if (t2 < cnt) {
+      const int id = pair_ids[bs + t2];
+      r2 = rec[id];
+      if (use_mask) z2 = zt[id];
+    }
     const bool touch = (int)threadIdx.x < cnt && stage_pair(sh, threadIdx.x, nxt, nxt.gid, b);
+    if (use_mask && touch) stage_mask(sh, threadIdx.x, nxt, z1, b, bs, pair_cols);
     if (touch && (non_finite(nxt.rr) || non_finite(nxt.ri))) {
       // Zero weights times a non-finite density would be NaN: such a
       // Gaussian adds nothing here and is evaluated on its own after the
@@ -1720,21 +1735,18 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_forward(const GRec* __restr
       sh.any_bad = 1;
     }
     const int bn = bs + kFwdBatch;
-    if (bn < b.p_end && (int)threadIdx.x < min(kBatch, b.p_end - bn)) nxt = rec[pair_ids[bn + threadIdx.x]];
+    if (bn < b.p_end && (int)threadIdx.x < min(kBatch, b.p_end - bn)) {
+      nxt_id = pair_ids[bn + threadIdx.x];
+      nxt = rec[nxt_id];
+    }
     const bool touch2 = t2 < cnt && stage_pair(sh, t2, r2, r2.gid, b);
+    if (use_mask && touch2) stage_mask(sh, t2, r2, z2, b, bs, pair_cols);
     if (touch2 && (non_finite(r2.rr) || non_finite(r2.ri))) {
       sh.rr[t2] = 0.0f;
       sh.ri[t2] = 0.0f;
       sh.any_bad = 1;
     }
-    if (use_mask) {
-      append_touch(sh, touch, threadIdx.x, lane);
-      append_touch(sh, touch2, t2, lane);
-      __syncthreads();
-      batch_masks(sh, b, bs, zt, pair_cols);
-    }
     __syncthreads();
-    if (threadIdx.x == 0) sh.n_touch = 0;
     if (!b.active) continue;
     const int nhit = build_list(sh, cnt, b, wave, lane, use_mask);
     if (full)
@@ -2605,7 +2617,7 @@ __global__ __launch_bounds__(kTileThreads) void k_count(const GRec* __restrict__
     if (use_mask) {
       append_touch(sh, touch, threadIdx.x, lane);
       __syncthreads();
-      batch_masks(sh, b, bs, zt, nullptr);
+      batch_masks(sh, b, zt);
     }
     __syncthreads();
     if (threadIdx.x == 0) sh.n_touch = 0;
