@@ -275,6 +275,40 @@ struct __align__(16) GRec {
 };
 static_assert(sizeof(GRec) == 64, "record must be one 64-byte line");
 
+// The fields of record i that the tile kernels read (gid only with kGid), as
+// three 16-byte loads and one of 12 or 8 bytes.  No loaded word goes unused:
+// the compiler may give the register of an unused word to another value while
+// its load is in flight, and must then wait for the load at once, which would
+// undo reading records one batch ahead.  The tail's offset is hidden from the
+// compiler (an empty asm on a register), which would otherwise widen the
+// short load to the whole 16 bytes.
+template <bool kGid>
+__device__ __forceinline__ GRec load_rec(const GRec* __restrict__ rec, int i) {
+  const float4* p = reinterpret_cast<const float4*>(rec + i);
+  const float4 a = p[0], b = p[1], c = p[2];
+  int tail = 48;
+  asm volatile("" : "+s"(tail));
+  const float* q = reinterpret_cast<const float*>(reinterpret_cast<const char*>(rec + i) + tail);
+  GRec r;
+  r.c0 = a.x;
+  r.c1 = a.y;
+  r.c2 = a.z;
+  r.rr = a.w;
+  r.ri = b.x;
+  r.y1h = b.y;
+  r.y1d = b.z;
+  r.y1w = b.w;
+  r.y2d = c.x;
+  r.y2w = c.y;
+  r.y3w = c.z;
+  r.e0 = c.w;
+  r.e1 = q[0];
+  r.e2 = q[1];
+  r.gid = kGid ? __float_as_int(q[2]) : -1;
+  r.pad = 0.0f;
+  return r;
+}
+
 struct BinGeom {
   int size[3];
   int nt[3];
@@ -1436,13 +1470,15 @@ __device__ __forceinline__ void batch_masks(Stage& sh, const BlockCtx& b, const 
 }
 
 // Sub-brick mask of a pair staged at k from registers (its record r and
-// column-bound record z), and its column word at bs + k in pair_cols when
-// given.
-__device__ __forceinline__ void stage_mask(Stage& sh, int k, const GRec& r, const ZRec& z, const BlockCtx& b,
-                                           int bs, uint32_t* __restrict__ pair_cols) {
+// column-bound record z; 0 unless it touches the tile).  Its column word (0
+// unless it touches) replaces the box, which lists built from masks do not
+// read; k_forward copies the words to pair_cols after the batch's evaluation.
+__device__ __forceinline__ void stage_mask(Stage& sh, int k, const GRec& r, const ZRec& z, bool touch,
+                                           const BlockCtx& b) {
   uint32_t cols;
-  sh.sbm[k] = (uint8_t)pair_mask(r.c0, r.c1, r.c2, sh.box[k], ZCoefs{z.h, z.w, z.h.rho > 0.0f}, b, cols);
-  if (pair_cols != nullptr) pair_cols[bs + k] = cols;
+  const uint32_t m = pair_mask(r.c0, r.c1, r.c2, sh.box[k], ZCoefs{z.h, z.w, z.h.rho > 0.0f}, b, cols);
+  sh.sbm[k] = (uint8_t)(touch ? m : 0u);
+  sh.box[k] = touch ? cols : 0u;
 }
 
 // Compacts, in batch order, the staged Gaussians whose box touches this
@@ -1700,59 +1736,76 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_forward(const GRec* __restr
 
   // Records of the next batch are loaded into registers while the current
   // batch is evaluated (loads stay in flight across the LDS barriers).
+  // Thread t stages pairs t and kBatch + t of a batch; of the second, only
+  // the Gaussian id is read ahead.
   GRec nxt = {};
-  int nxt_id = 0;
+  int nxt_id = 0, nxt_id2 = 0;
   const bool use_mask = geo.nsb <= 2;
   if ((int)threadIdx.x < min(kBatch, b.p_end - b.p_begin)) {
     nxt_id = pair_ids[b.p_begin + threadIdx.x];
-    nxt = rec[nxt_id];
+    nxt = load_rec<true>(rec, nxt_id);
   }
+  if ((int)threadIdx.x + kBatch < min(kFwdBatch, b.p_end - b.p_begin)) nxt_id2 = pair_ids[b.p_begin + kBatch + threadIdx.x];
   if (threadIdx.x == 0) sh.any_bad = 0;
   for (int bs = b.p_begin; bs < b.p_end; bs += kFwdBatch) {
     const int cnt = min(kFwdBatch, b.p_end - bs);
     __syncthreads();
-    // The second half of the batch (thread t stages pair kBatch + t) is read
-    // here; the first was loaded during the previous batch.  The column-bound
-    // records are read with them: each thread computes the sub-brick masks of
-    // the pairs it stages.
+    // Thread t stages pair t (read during the previous batch) and pair
+    // kBatch + t (read here), each with its column-bound record, and computes
+    // their sub-brick masks.  A wave with any pair in a half stages that half
+    // in all its lanes (lanes past the batch restage a record they hold into
+    // slots no list reads), and a half's loads and staging share one
+    // wave-uniform branch: every loaded value is then consumed on every path,
+    // and the compiler keeps the next batch's loads in flight through the
+    // evaluation instead of waiting for them in it.
     const int t2 = threadIdx.x + kBatch;
-    ZRec z1 = {}, z2 = {};
-    if (use_mask && (int)threadIdx.x < cnt) z1 = zt[nxt_id];
-    GRec r2 = {};
-    if (t2 < cnt) {
-      const int id = pair_ids[bs + t2];
-      r2 = rec[id];
-      if (use_mask) z2 = zt[id];
-    }
-    const bool touch = (int)threadIdx.x < cnt && stage_pair(sh, threadIdx.x, nxt, nxt.gid, b);
-    if (use_mask && touch) stage_mask(sh, threadIdx.x, nxt, z1, b, bs, pair_cols);
-    if (touch && (non_finite(nxt.rr) || non_finite(nxt.ri))) {
-      // Zero weights times a non-finite density would be NaN: such a
-      // Gaussian adds nothing here and is evaluated on its own after the
-      // batches (fwd_non_finite).
-      sh.rr[threadIdx.x] = 0.0f;
-      sh.ri[threadIdx.x] = 0.0f;
-      sh.any_bad = 1;
+    auto stage_one = [&](int k, const GRec& r, const ZRec& z) {
+      const bool touch = stage_pair(sh, k, r, r.gid, b) && k < cnt;
+      if (use_mask) stage_mask(sh, k, r, z, touch, b);
+      if (touch && (non_finite(r.rr) || non_finite(r.ri))) {
+        // Zero weights times a non-finite density would be NaN: such a
+        // Gaussian adds nothing here and is evaluated on its own after the
+        // batches (fwd_non_finite).
+        sh.rr[k] = 0.0f;
+        sh.ri[k] = 0.0f;
+        sh.any_bad = 1;
+      }
+    };
+    if (kBatch + wave * 64 < cnt) {
+      // Both halves: the second half's loads overlap the first's staging.
+      ZRec z1 = {}, z2 = {};
+      const GRec r2 = load_rec<true>(rec, nxt_id2);
+      if (use_mask) {
+        z2 = zt[nxt_id2];
+        z1 = zt[nxt_id];
+      }
+      stage_one(threadIdx.x, nxt, z1);
+      stage_one(t2, r2, z2);
+    } else if (wave * 64 < cnt) {
+      ZRec z1 = {};
+      if (use_mask) z1 = zt[nxt_id];
+      stage_one(threadIdx.x, nxt, z1);
     }
     const int bn = bs + kFwdBatch;
-    if (bn < b.p_end && (int)threadIdx.x < min(kBatch, b.p_end - bn)) {
-      nxt_id = pair_ids[bn + threadIdx.x];
-      nxt = rec[nxt_id];
-    }
-    const bool touch2 = t2 < cnt && stage_pair(sh, t2, r2, r2.gid, b);
-    if (use_mask && touch2) stage_mask(sh, t2, r2, z2, b, bs, pair_cols);
-    if (touch2 && (non_finite(r2.rr) || non_finite(r2.ri))) {
-      sh.rr[t2] = 0.0f;
-      sh.ri[t2] = 0.0f;
-      sh.any_bad = 1;
+    if (bn < b.p_end) {
+      if ((int)threadIdx.x < min(kBatch, b.p_end - bn)) {
+        nxt_id = pair_ids[bn + threadIdx.x];
+        nxt = load_rec<true>(rec, nxt_id);
+      }
+      if (t2 < min(kFwdBatch, b.p_end - bn)) nxt_id2 = pair_ids[bn + t2];
     }
     __syncthreads();
-    if (!b.active) continue;
-    const int nhit = build_list(sh, cnt, b, wave, lane, use_mask);
-    if (full)
-      fwd_groups<false>(sh, nhit, b, wave, gl, xi, x_ok, acc_r, acc_i);
-    else
-      fwd_groups<true>(sh, nhit, b, wave, gl, xi, x_ok, acc_r, acc_i);
+    if (b.active) {
+      const int nhit = build_list(sh, cnt, b, wave, lane, use_mask);
+      if (full)
+        fwd_groups<false>(sh, nhit, b, wave, gl, xi, x_ok, acc_r, acc_i);
+      else
+        fwd_groups<true>(sh, nhit, b, wave, gl, xi, x_ok, acc_r, acc_i);
+    }
+    // Column words out after the evaluation: stores in flight during it would
+    // make the compiler wait for them before reusing their data registers.
+    if (use_mask && pair_cols != nullptr)
+      for (int k = threadIdx.x; k < cnt; k += kTileThreads) pair_cols[bs + k] = sh.box[k];
   }
 
   // Sum the 16 Gaussian slots of each W column (a row of 16 lanes), scattered
@@ -1997,6 +2050,24 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_backward(const GRec* __rest
   if (!b.valid) return;
   const bool partial = b.e1 < T || b.e2 < T;  // D rows are always clipped to the tile
 
+  // Records of the next batch are loaded into registers while the current
+  // batch is evaluated (loads stay in flight across the LDS barriers); the
+  // first batch's are requested before the gradient copy, so that the two
+  // reads overlap.
+  const bool use_cols = pair_cols != nullptr && nb <= 2;
+  const int bsz = kB;
+  // Thread t stages pairs t and kTileThreads + t of a batch; of the second,
+  // only the Gaussian id is read ahead.
+  GRec nxt = {};
+  int nxt_id = 0, nxt_id2 = 0;
+  uint32_t nxt_cols = 0;
+  if (tid < min(bsz, b.p_end - b.p_begin)) {
+    nxt_id = pair_ids[b.p_begin + tid];
+    nxt = load_rec<false>(rec, nxt_id);
+    if (use_cols) nxt_cols = pair_cols[b.p_begin + tid];
+  }
+  if (tid + kTileThreads < min(bsz, b.p_end - b.p_begin)) nxt_id2 = pair_ids[b.p_begin + kTileThreads + tid];
+
   int g_bad = 0;
   // For 8-voxel tiles the copy has 8 more slabs of zeros past the tile: a
   // group's row walk from any first row stays inside it.
@@ -2016,18 +2087,6 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_backward(const GRec* __rest
   // the tile down the select path (block-uniform).
   const bool tile_bad = __syncthreads_or(g_bad) != 0;
 
-  // Records of the next batch are loaded into registers while the current
-  // batch is evaluated (loads stay in flight across the LDS barriers).
-  const bool use_cols = pair_cols != nullptr && nb <= 2;
-  const int bsz = kB;
-  GRec nxt = {};
-  int nxt_id = 0;
-  uint32_t nxt_cols = 0;
-  if (tid < min(bsz, b.p_end - b.p_begin)) {
-    nxt_id = pair_ids[b.p_begin + tid];
-    nxt = rec[nxt_id];
-    if (use_cols) nxt_cols = pair_cols[b.p_begin + tid];
-  }
   for (int bs = b.p_begin; bs < b.p_end; bs += bsz) {
     const int cnt = min(bsz, b.p_end - bs);
     __syncthreads();
@@ -2124,16 +2183,16 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_backward(const GRec* __rest
     if (tid < cnt) stage(tid, nxt, nxt_id, nxt_cols);
     if (tid + kTileThreads < cnt) {
       const int t2 = tid + kTileThreads;
-      const int id2 = pair_ids[bs + t2];
-      stage(t2, rec[id2], id2, use_cols ? pair_cols[bs + t2] : 0u);
+      stage(t2, load_rec<false>(rec, nxt_id2), nxt_id2, use_cols ? pair_cols[bs + t2] : 0u);
     }
     {
       const int bn = bs + bsz;
       if (bn < b.p_end && tid < min(bsz, b.p_end - bn)) {
         nxt_id = pair_ids[bn + tid];
-        nxt = rec[nxt_id];
+        nxt = load_rec<false>(rec, nxt_id);
         if (use_cols) nxt_cols = pair_cols[bn + tid];
       }
+      if (bn < b.p_end && tid + kTileThreads < min(bsz, b.p_end - bn)) nxt_id2 = pair_ids[bn + kTileThreads + tid];
     }
     __syncthreads();
     // Key offsets (ascending keys) and, per column count, the sorted segment and
@@ -2251,6 +2310,9 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_backward(const GRec* __rest
         {
           const int m = lane & 15, jm = m >> 2;
           const int id = slot_ok ? sh.gid[k] : -1;
+          // All lane moves first, then the atomics: one wait for the moves.
+          float val[4];
+          int gid[4];
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int g = 4 * i + (lane >> 4);
@@ -2258,10 +2320,13 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_backward(const GRec* __rest
             const float a0 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(t3[0])));
             const float a1 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(t3[1])));
             const float a2 = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(t3[2])));
-            const float val = jm == 0 ? a0 : (jm == 1 ? a1 : a2);
-            const int gid = __builtin_amdgcn_ds_bpermute(g * 4, id);
-            if (m < kMoments && gid >= 0 && val != 0.0f) unsafeAtomicAdd(gacc + (int64_t)gid * kAccStride + m, val);
+            val[i] = jm == 0 ? a0 : (jm == 1 ? a1 : a2);
+            gid[i] = __builtin_amdgcn_ds_bpermute(g * 4, id);
           }
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            if (m < kMoments && gid[i] >= 0 && val[i] != 0.0f)
+              unsafeAtomicAdd(gacc + (int64_t)gid[i] * kAccStride + m, val[i]);
         }
       }
     }
