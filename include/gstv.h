@@ -219,8 +219,8 @@ int gstv_backward_noalloc(const float* center_vox, const float* cov_inv,
  *                       pair_columns [total pairs] (uint32, parallel to
  *                       pair_ids): for every pair whose extent box touches its
  *                       tile, the D rows that its column bound leaves in each
- *                       4x4 (H, W) column of the tile (7 bits per column; other
- *                       entries are left unwritten).  Required, and written,
+ *                       4x4 (H, W) column of the tile (7 bits per column; 0
+ *                       for the other pairs).  Required, and written,
  *                       when tile <= 8; ignored for larger tiles.  Any tile
  *                       lists may be given (those of gstv_bin_fill too).
  *   gstv_backward_keep  gstv_backward on the records, the zeroed accumulator
